@@ -1,0 +1,157 @@
+/*
+ * pointint_hip.h -- C ABI of the MI355X (gfx950) k-space intersection-volume
+ * library libpointint_hip.so.
+ *
+ * The path: per-object form factors A_i(k_q) over a Gauss-Legendre x Lebedev
+ * k-grid, the all-pairs volume matrix V = Re(J^H D J) / (8 pi^3), its adjoint
+ * dV/dvertex, and a serving loop around them.  Plain C types only: pointers,
+ * sizes, int status codes.  Every entry point returns PII_OK (0) or an error
+ * code, and pii_last_error() returns the calling thread's last message.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   - CudaKernelRegistry / CudaComputeAkFunc plug-in functors
+ *       cpp/include/dof/cuda/dof_cuda_interface.hpp:24-67
+ *   - compute_intersection_volume_matrix_cuda
+ *       cpp/include/compute_intersection_volume_multi_object.hpp:51-69
+ *   - compute_intersection_volume_cuda (pairwise)
+ *       cpp/include/compute_intersection_volume.hpp:48-59, :81-96
+ *   - compute_Ak_cuda / compute_Ak_gradient_cuda
+ *       cpp/include/compute_intersection_volume.hpp:120-135
+ *   - load_lebedev_txt, gauss_legendre_interval, build_kgrid
+ *       cpp/include/quadrature/lebedev_io.hpp:24-59,
+ *       cpp/include/quadrature/gauss_legendre.hpp:16-62,
+ *       cpp/include/quadrature/kgrid.hpp:31-47
+ *   - make_triangle_mesh / make_point_cloud / make_gaussian_splat
+ *       cpp/include/geometry/packing.hpp:31-45
+ */
+#ifndef POINTINT_HIP_H
+#define POINTINT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  PII_OK = 0,
+  PII_ERR_INVALID = 1,     /* bad argument, size mismatch, unknown type */
+  PII_ERR_HIP = 2,         /* HIP runtime error */
+  PII_ERR_NOMEM = 3,       /* device allocation failed */
+  PII_ERR_UNSUPPORTED = 4, /* DoF / geometry combination without a kernel */
+  PII_ERR_BUDGET = 5,      /* serving: request refused by the HBM budget */
+  PII_ERR_IO = 6,          /* file could not be read */
+  PII_ERR_BUSY = 7         /* serving: no result ready yet */
+};
+
+/* GeometryType: cpp/include/geometry/types.hpp:40-44 */
+enum { PII_GEOM_TRIANGLE = 0, PII_GEOM_DISK = 1, PII_GEOM_GAUSSIAN = 2 };
+
+/* DoF parameterisations: AffineDoF (12 = t.xyz + A row-major,
+ * cpp/include/dof/affine_dof.hpp:15-22) and TriangleMeshDoF (3V vertex
+ * coordinates, cpp/include/dof/triangle_mesh_dof.hpp:12-17). */
+enum { PII_DOF_AFFINE = 0, PII_DOF_TRIMESH = 1 };
+
+/* Form-factor precision.  FP64: the reference formula in FP64 on the GPU.
+ * FAST: FP64 dot products / phases / Taylor branch, FP32 transcendentals
+ * after FP64 range reduction, FP64 accumulation. */
+enum { PII_FF_FP64 = 0, PII_FF_FAST = 1 };
+
+/* Gram mode for V = X^T X with X = sqrt(w) [Re J; Im J]. */
+enum { PII_GRAM_FP64 = 0, PII_GRAM_BF16X3 = 1, PII_GRAM_FP16X3 = 2 };
+
+/* ComputationFlags: cpp/include/computation_flags.hpp:12-17 */
+enum { PII_FLAG_VOLUME = 1, PII_FLAG_GRADIENT = 2, PII_FLAG_HESSIAN = 4 };
+
+typedef struct pii_context pii_context;
+typedef struct pii_kgrid pii_kgrid;
+typedef struct pii_geometry pii_geometry;
+
+const char* pii_last_error(void);
+const char* pii_version(void);
+
+/* ---- host quadrature utilities ------------------------------------------ */
+/* Gauss-Legendre nodes/weights on [a,b] (ascending).
+ * Replaces gauss_legendre_interval, cpp/include/quadrature/gauss_legendre.hpp:16-62 */
+int pii_gauss_legendre(int n, double a, double b, double* x, double* w);
+
+/* Lebedev rule file of "theta phi w" lines in degrees.  With dirs/w NULL only
+ * the count is returned in *count.  Weights summing to ~1 are scaled by 4 pi.
+ * Replaces load_lebedev_txt, cpp/include/quadrature/lebedev_io.hpp:24-59 */
+int pii_lebedev_load(const char* path, double* dirs, double* w, int cap, int* count);
+
+/* k-grid: node q = ir*M + j, k = tan t_ir, w = w_leb[j] * w_t[ir] * sec^2 t_ir.
+ * Outputs dirs[Q*3], kmag[Q], w[Q] with Q = M*nrad.
+ * Replaces build_kgrid, cpp/include/quadrature/kgrid.hpp:31-47 */
+int pii_kgrid_build(const double* leb_dirs, const double* leb_w, int M, int nrad,
+                    double* dirs, double* kmag, double* w);
+
+/* ---- context ------------------------------------------------------------- */
+int pii_context_create(int device, pii_context** out);
+int pii_context_destroy(pii_context* ctx);
+int pii_context_synchronize(pii_context* ctx);
+/* Phase timing with HIP events on the context's stream (the reference's
+ * enable_profiling flag, e.g.
+ * cpp/src/compute_intersection_volume_multi_object.cu:626-695, which used host
+ * clocks).  Phases: 0 pre-pass, 1 form factor, 2 Gram, 3 adjoint; ms[p] holds
+ * the last call's duration of phase p (0 when it did not run). */
+int pii_context_set_profiling(pii_context* ctx, int enable);
+int pii_context_timings(pii_context* ctx, double* ms, int n);
+
+/* ---- k-grid resident in HBM ----------------------------------------------- */
+int pii_kgrid_create(pii_context* ctx, int Q, const double* dirs, const double* kmag,
+                     const double* w, pii_kgrid** out);
+int pii_kgrid_destroy(pii_kgrid* kg);
+/* *Q full node count; *Qh nodes actually evaluated by the Gram (half when the
+ * grid is centrally symmetric with equal antipodal weights). */
+int pii_kgrid_info(const pii_kgrid* kg, int* Q, int* Qh, int* symmetric);
+
+/* ---- geometry resident in HBM ---------------------------------------------- */
+/* Triangle mesh from V[nv*3], F[nf*3] (make_triangle_mesh,
+ * cpp/src/geometry/packing.cpp:10-28, :66-70). */
+int pii_geometry_create_mesh(pii_context* ctx, const double* V, int nv, const int32_t* F,
+                             int nf, pii_geometry** out);
+/* Packed triangles tri[nt*12] = {a, e1, e2, S} with vid[nt*3] (TriPacked,
+ * cpp/include/geometry/types.hpp:9-17).  Negative vids are allowed; such
+ * faces contribute nothing under TRIMESH DoFs, as in
+ * cpp/src/dof/triangle_mesh_dof.cpp:44-45. */
+int pii_geometry_create_tris(pii_context* ctx, const double* tri, const int32_t* vid, int nt,
+                             pii_geometry** out);
+/* Disks el[n*8] = {c, n(unit), rho, area} (DiskPacked, types.hpp:21-27). */
+int pii_geometry_create_disks(pii_context* ctx, const double* el, int n, pii_geometry** out);
+/* Gaussian surfels el[n*8] = {c, n(unit), sigma, w} (GaussianPacked, types.hpp:31-37). */
+int pii_geometry_create_gaussians(pii_context* ctx, const double* el, int n,
+                                  pii_geometry** out);
+int pii_geometry_destroy(pii_geometry* g);
+int pii_geometry_info(const pii_geometry* g, int* type, int* num_elements, int* num_dof_vertices);
+
+/* ---- form factors ---------------------------------------------------------- */
+/* A(k_q) for every node of kg, written to out[2*Q] (interleaved re, im, host
+ * memory).  ndofs must be 12 for AFFINE and 3*num_dof_vertices for TRIMESH.
+ * Replaces the CudaComputeAkFunc plug-ins compute_Ak_{affine_triangle,
+ * affine_disk,affine_gaussian,triangle_mesh}_cuda_wrapper
+ * (cpp/src/dof/cuda/affine_dof_cuda.cu:855-1243,
+ *  cpp/src/dof/cuda/triangle_mesh_dof_cuda.cu:314-384). */
+int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const double* dofs,
+                    int ndofs, const pii_kgrid* kg, int precision, double* out);
+
+/* ---- all-pairs volume matrix ----------------------------------------------- */
+/* V[n*n] (row-major, symmetric) for n objects, each with its own DoF kind and
+ * DoF vector.  Replaces compute_intersection_volume_matrix_cuda with
+ * VOLUME_ONLY (cpp/src/compute_intersection_volume_multi_object.cu:233-712). */
+int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                      const int* dof_kinds, const double* const* dofs, const int* ndofs,
+                      const pii_kgrid* kg, int ff_precision, int gram_mode, double* V);
+
+/* Gram of caller-provided form factors: J[n*Q] complex (object-major,
+ * interleaved re/im, host memory) over the full grid of kg. */
+int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int gram_mode,
+             double* V);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POINTINT_HIP_H */

@@ -1,0 +1,310 @@
+"""Python host API over the C ABI: resident k-grids and geometries, form
+factors and the all-pairs volume matrix on one MI355X.
+
+Names and argument meanings follow the reference's C++ API
+(cpp/include/compute_intersection_volume_multi_object.hpp:51-120,
+cpp/include/geometry/packing.hpp:31-45, cpp/include/quadrature/*.hpp).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+
+IDENTITY_AFFINE = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], dtype=np.float64)
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "lebedev")
+
+_FF = {"fp64": N.FF_FP64, "fast": N.FF_FAST}
+_GRAM = {"fp64": N.GRAM_FP64, "bf16x3": N.GRAM_BF16X3, "fp16x3": N.GRAM_FP16X3}
+_DOF = {"affine": N.DOF_AFFINE, "trimesh": N.DOF_TRIMESH}
+
+
+def lebedev_path(order: int) -> str:
+    """Path of a bundled Lebedev rule file, e.g. lebedev_path(29) (302 points)."""
+    return os.path.join(DATA_DIR, f"lebedev_{order:03d}.txt")
+
+
+class Context:
+    """One HIP device + stream.  All objects created from a context live on its device."""
+
+    _default = {}
+    _lock = threading.Lock()
+
+    def __init__(self, device: int = 0):
+        lib = N.load()
+        h = C.c_void_p()
+        N.check(lib.pii_context_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @classmethod
+    def default(cls, device: int = 0) -> "Context":
+        with cls._lock:
+            if device not in cls._default:
+                cls._default[device] = Context(device)
+            return cls._default[device]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self) -> None:
+        N.check(N.load().pii_context_synchronize(self._h))
+
+    def set_profiling(self, on: bool) -> None:
+        N.check(N.load().pii_context_set_profiling(self._h, 1 if on else 0))
+
+    def timings(self) -> dict:
+        """Last call's phase durations (ms) from HIP events on the context stream."""
+        ms = np.zeros(4)
+        N.check(N.load().pii_context_timings(self._h, N.ptr(ms), 4))
+        return {"prepass": ms[0], "form_factor": ms[1], "gram": ms[2], "adjoint": ms[3]}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            N.load().pii_context_destroy(self._h)
+            self._h = None
+
+
+# ---------------------------------------------------------------------------
+# Quadrature (host).
+# ---------------------------------------------------------------------------
+def gauss_legendre(n: int, a: float, b: float):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    N.check(N.load().pii_gauss_legendre(n, a, b, N.ptr(x), N.ptr(w)))
+    return x, w
+
+
+def load_lebedev(path: str):
+    """(dirs (M,3), weights (M,)) with weights summing to 4 pi."""
+    lib = N.load()
+    cnt = C.c_int()
+    N.check(lib.pii_lebedev_load(path.encode(), None, None, 0, C.byref(cnt)))
+    M = cnt.value
+    d = np.zeros((M, 3))
+    w = np.zeros(M)
+    N.check(lib.pii_lebedev_load(path.encode(), N.ptr(d), N.ptr(w), M, C.byref(cnt)))
+    return d, w
+
+
+class KGrid:
+    """k-grid (Gauss-Legendre in t, k = tan t, x Lebedev directions) resident in HBM."""
+
+    def __init__(self, dirs, kmag, w, ctx: Optional[Context] = None):
+        self.ctx = ctx or Context.default()
+        self.dirs = N.f64(dirs).reshape(-1, 3)
+        self.kmag = N.f64(kmag).reshape(-1)
+        self.w = N.f64(w).reshape(-1)
+        Q = self.kmag.shape[0]
+        if self.dirs.shape[0] != Q or self.w.shape[0] != Q:
+            raise ValueError("dirs, kmag and w must have the same length")
+        h = C.c_void_p()
+        N.check(N.load().pii_kgrid_create(self.ctx.handle, Q, N.ptr(self.dirs), N.ptr(self.kmag),
+                                          N.ptr(self.w), C.byref(h)))
+        self._h = h
+        q, qh, sym = C.c_int(), C.c_int(), C.c_int()
+        N.check(N.load().pii_kgrid_info(h, C.byref(q), C.byref(qh), C.byref(sym)))
+        self.Q, self.Qh, self.symmetric = q.value, qh.value, bool(sym.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return self.Q
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                N.load().pii_kgrid_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def build_kgrid_arrays(leb_dirs, leb_w, n_radial: int):
+    leb_dirs = N.f64(leb_dirs).reshape(-1, 3)
+    leb_w = N.f64(leb_w).reshape(-1)
+    M = leb_w.shape[0]
+    Q = M * n_radial
+    dirs = np.zeros((Q, 3))
+    kmag = np.zeros(Q)
+    w = np.zeros(Q)
+    N.check(N.load().pii_kgrid_build(N.ptr(leb_dirs), N.ptr(leb_w), M, n_radial, N.ptr(dirs),
+                                     N.ptr(kmag), N.ptr(w)))
+    return dirs, kmag, w
+
+
+def build_kgrid(leb_dirs, leb_w, n_radial: int, ctx: Optional[Context] = None) -> KGrid:
+    return KGrid(*build_kgrid_arrays(leb_dirs, leb_w, n_radial), ctx=ctx)
+
+
+def kgrid_from_rule(order: int, n_radial: int, ctx: Optional[Context] = None) -> KGrid:
+    d, w = load_lebedev(lebedev_path(order))
+    return build_kgrid(d, w, n_radial, ctx)
+
+
+# ---------------------------------------------------------------------------
+# Geometry.
+# ---------------------------------------------------------------------------
+class Geometry:
+    """Triangle mesh, oriented-disk cloud or Gaussian-surfel set resident in HBM."""
+
+    def __init__(self, handle, ctx: Context, kind: int, host: dict):
+        self._h = handle
+        self.ctx = ctx
+        self.type = kind
+        self.host = host
+        t, n, nv = C.c_int(), C.c_int(), C.c_int()
+        N.check(N.load().pii_geometry_info(handle, C.byref(t), C.byref(n), C.byref(nv)))
+        self._n = n.value
+        self.num_dof_vertices = nv.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_elements(self) -> int:
+        return self._n
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                N.load().pii_geometry_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def make_triangle_mesh(V, F, ctx: Optional[Context] = None) -> Geometry:
+    ctx = ctx or Context.default()
+    V = N.f64(V).reshape(-1, 3)
+    F = N.i32(F).reshape(-1, 3)
+    h = C.c_void_p()
+    N.check(N.load().pii_geometry_create_mesh(ctx.handle, N.ptr(V), V.shape[0], N.ptr(F),
+                                              F.shape[0], C.byref(h)))
+    return Geometry(h, ctx, N.GEOM_TRIANGLE, {"V": V, "F": F})
+
+
+def _unit_normals(Nrm):
+    Nrm = N.f64(Nrm).reshape(-1, 3).copy()
+    l = np.linalg.norm(Nrm, axis=1)
+    nz = l > 0
+    Nrm[nz] /= l[nz, None]
+    return Nrm
+
+
+def make_point_cloud(P, Nrm, radii_or_areas, is_radius: bool = True,
+                     ctx: Optional[Context] = None) -> Geometry:
+    """Oriented disks (cpp/src/geometry/packing.cpp:30-63): area = pi rho^2, or rho = sqrt(area/pi)."""
+    ctx = ctx or Context.default()
+    P = N.f64(P).reshape(-1, 3)
+    n = P.shape[0]
+    r = N.f64(radii_or_areas).reshape(-1)
+    if r.shape[0] != n:
+        raise ValueError("one radius/area per point required")
+    el = np.zeros((n, 8))
+    el[:, 0:3] = P
+    el[:, 3:6] = _unit_normals(Nrm)
+    if is_radius:
+        el[:, 6] = r
+        el[:, 7] = np.pi * r * r
+    else:
+        el[:, 7] = r
+        el[:, 6] = np.sqrt(r / np.pi)
+    h = C.c_void_p()
+    N.check(N.load().pii_geometry_create_disks(ctx.handle, N.ptr(el), n, C.byref(h)))
+    return Geometry(h, ctx, N.GEOM_DISK, {"el": el})
+
+
+def make_gaussian_splat(P, Nrm, sigmas, weights, ctx: Optional[Context] = None) -> Geometry:
+    """Gaussian surfels (cpp/src/geometry/packing.cpp:84-124)."""
+    ctx = ctx or Context.default()
+    P = N.f64(P).reshape(-1, 3)
+    n = P.shape[0]
+    el = np.zeros((n, 8))
+    el[:, 0:3] = P
+    el[:, 3:6] = _unit_normals(Nrm)
+    el[:, 6] = N.f64(sigmas).reshape(-1)
+    el[:, 7] = N.f64(weights).reshape(-1)
+    h = C.c_void_p()
+    N.check(N.load().pii_geometry_create_gaussians(ctx.handle, N.ptr(el), n, C.byref(h)))
+    return Geometry(h, ctx, N.GEOM_GAUSSIAN, {"el": el})
+
+
+def make_packed_triangles(tri, vid=None, ctx: Optional[Context] = None) -> Geometry:
+    """Triangles as TriPacked rows {a, e1, e2, S} (12 doubles) plus optional vids."""
+    ctx = ctx or Context.default()
+    tri = N.f64(tri).reshape(-1, 12)
+    nt = tri.shape[0]
+    vidp = None
+    if vid is not None:
+        vid = N.i32(vid).reshape(-1, 3)
+        vidp = N.ptr(vid)
+    h = C.c_void_p()
+    N.check(N.load().pii_geometry_create_tris(ctx.handle, N.ptr(tri), vidp, nt, C.byref(h)))
+    return Geometry(h, ctx, N.GEOM_TRIANGLE, {"tri": tri, "vid": vid})
+
+
+# ---------------------------------------------------------------------------
+# Form factors and volumes.
+# ---------------------------------------------------------------------------
+def _dof_args(geoms, dofs, dof_kinds):
+    n = len(geoms)
+    if dof_kinds is None:
+        dof_kinds = ["affine"] * n
+    kinds = [(_DOF[k] if isinstance(k, str) else int(k)) for k in dof_kinds]
+    if dofs is None:
+        dofs = [IDENTITY_AFFINE] * n
+    dofs = [N.f64(d).reshape(-1) for d in dofs]
+    if len(kinds) != n or len(dofs) != n:
+        raise ValueError("one DoF kind and one DoF vector per geometry")
+    return kinds, dofs
+
+
+def compute_form_factor(geom: Geometry, kgrid: KGrid, dofs=None, dof_kind="affine",
+                        precision: str = "fast") -> np.ndarray:
+    """A(k_q) for every node of the grid (complex128, length Q)."""
+    kinds, dl = _dof_args([geom], None if dofs is None else [dofs], [dof_kind])
+    out = np.zeros(2 * kgrid.Q)
+    N.check(N.load().pii_form_factor(geom.ctx.handle, geom.handle, kinds[0], N.ptr(dl[0]),
+                                     dl[0].shape[0], kgrid.handle, _FF[precision], N.ptr(out)))
+    return out[0::2] + 1j * out[1::2]
+
+
+def compute_volume_matrix(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None, dof_kinds=None,
+                          ff: str = "fast", gram: str = "fp16x3") -> np.ndarray:
+    """V (n x n) = Re(J^H D J)/(8 pi^3) on the GPU."""
+    n = len(geoms)
+    if n == 0:
+        return np.zeros((0, 0))
+    ctx = geoms[0].ctx
+    kinds, dl = _dof_args(geoms, dofs, dof_kinds)
+    gh = (C.c_void_p * n)(*[g.handle.value for g in geoms])
+    kh = (C.c_int * n)(*kinds)
+    dh = (C.c_void_p * n)(*[d.ctypes.data for d in dl])
+    nh = (C.c_int * n)(*[d.shape[0] for d in dl])
+    V = np.zeros((n, n))
+    N.check(N.load().pii_volume_matrix(ctx.handle, n, gh, kh, dh, nh, kgrid.handle, _FF[ff],
+                                       _GRAM[gram], N.ptr(V)))
+    return V
+
+
+def gram_from_J(J: np.ndarray, kgrid: KGrid, ctx: Optional[Context] = None) -> np.ndarray:
+    """FP64 Gram of caller form factors J (Q x n complex)."""
+    ctx = ctx or kgrid.ctx
+    J = np.asarray(J, dtype=np.complex128)
+    n = J.shape[1]
+    Jc = np.ascontiguousarray(J.T)  # object-major
+    V = np.zeros((n, n))
+    N.check(N.load().pii_gram(ctx.handle, N.ptr(Jc.view(np.float64)), n, kgrid.handle,
+                              N.GRAM_FP64, N.ptr(V)))
+    return V

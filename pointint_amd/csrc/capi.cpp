@@ -1,0 +1,585 @@
+// C-ABI implementation of libpointint_hip.so: context, resident k-grids and
+// geometries, and the form-factor / volume-matrix entry points.  Host code
+// only; kernels live in the .hip translation units.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "pii_host.hpp"
+#include "pii_internal.hpp"
+#include "pii_capi_internal.hpp"
+
+struct pii_context {
+  pii::Context c;
+};
+struct pii_kgrid {
+  pii::KGridDev k;
+  pii_context* ctx = nullptr;
+};
+struct pii_geometry {
+  pii::GeometryDev g;
+  pii_context* ctx = nullptr;
+};
+
+namespace pii {
+
+void DevBuf::release() {
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+}
+
+void DevBuf::ensure(size_t n) {
+  if (n <= bytes && p) return;
+  release();
+  if (n == 0) return;
+  const size_t want = std::max<size_t>(n, 256);
+  if (hipMalloc(&p, want) != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    throw Error(PII_ERR_NOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+  }
+  bytes = want;
+}
+
+void Context::phase_begin(int p) {
+  if (!profiling) return;
+  if (!ev[p][0]) {
+    PII_HIP(hipEventCreate(&ev[p][0]));
+    PII_HIP(hipEventCreate(&ev[p][1]));
+  }
+  PII_HIP(hipEventRecord(ev[p][0], stream));
+}
+
+void Context::phase_end(int p) {
+  if (!profiling) return;
+  PII_HIP(hipEventRecord(ev[p][1], stream));
+  ev_used[p] = true;
+}
+
+template <class F>
+int guarded(F&& f) {
+  clear_error();
+  try {
+    f();
+    return PII_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return PII_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return PII_ERR_INVALID;
+  }
+}
+
+static void require(bool cond, const std::string& msg) {
+  if (!cond) throw Error(PII_ERR_INVALID, msg);
+}
+
+template <class T>
+static void upload(DevBuf& b, const T* src, size_t count, hipStream_t s) {
+  b.ensure(count * sizeof(T));
+  if (count) PII_HIP(hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// ---------------------------------------------------------------------------
+// Batch set-up shared by every multi-object entry point.
+// ---------------------------------------------------------------------------
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+void validate_object(const GeometryDev& g, int dof_kind, int ndofs) {
+  if (dof_kind == PII_DOF_AFFINE) {
+    require(ndofs == 12, "AffineDoF requires 12 DoFs, got " + std::to_string(ndofs));
+  } else if (dof_kind == PII_DOF_TRIMESH) {
+    if (g.type != PII_GEOM_TRIANGLE)
+      throw Error(PII_ERR_UNSUPPORTED, "TriangleMeshDoF supports triangle geometry only");
+    require(ndofs == 3 * g.ndof_verts,
+            "TriangleMeshDoF requires " + std::to_string(3 * g.ndof_verts) + " DoFs, got " +
+                std::to_string(ndofs));
+  } else {
+    throw Error(PII_ERR_UNSUPPORTED, "unknown DoF kind " + std::to_string(dof_kind));
+  }
+}
+
+void Batch::build(Context& c, int n, const GeometryDev* const* geoms, const int* kinds,
+                  const double* const* dofs, const int* ndofs, bool fast) {
+  N = n;
+  hipStream_t s = c.stream;
+  // Largest objects first: the dispatcher then starts the longest work early.
+  order.resize(N);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return geoms[a]->n > geoms[b]->n; });
+  // DoF vectors, concatenated.
+  std::vector<size_t> dof_off(N);
+  size_t total_dofs = 0;
+  for (int i = 0; i < N; ++i) {
+    validate_object(*geoms[i], kinds[i], ndofs[i]);
+    dof_off[i] = total_dofs;
+    total_dofs += align_up((size_t)ndofs[i], 32);
+  }
+  h_dofs.assign(total_dofs, 0.0);
+  for (int i = 0; i < N; ++i)
+    std::memcpy(h_dofs.data() + dof_off[i], dofs[i], sizeof(double) * ndofs[i]);
+  upload(c.ws_dofs, h_dofs.data(), total_dofs, s);
+  // World vertices and element records.
+  std::vector<size_t> wv_off(N), rec_off(N);
+  size_t total_w = 0, total_rec = 0;
+  h_vprefix.assign(N + 1, 0);
+  h_eprefix.assign(N + 1, 0);
+  for (int slot = 0; slot < N; ++slot) {
+    const GeometryDev& g = *geoms[order[slot]];
+    const int nv = (g.type == PII_GEOM_TRIANGLE) ? g.nv : 0;
+    h_vprefix[slot + 1] = h_vprefix[slot] + nv;
+    h_eprefix[slot + 1] = h_eprefix[slot] + g.n;
+    wv_off[slot] = total_w;
+    total_w += align_up((size_t)nv * 3, 32);
+    rec_off[slot] = total_rec;
+    const int per = (g.type == PII_GEOM_TRIANGLE) ? (fast ? kRecFastTri : kRecPlainTri)
+                                                  : kRecSurfel;
+    total_rec += align_up((size_t)g.n * per, 32);
+  }
+  c.ws_world.ensure(std::max<size_t>(total_w, 1) * sizeof(double));
+  c.ws_faces.ensure(std::max<size_t>(total_rec, 1) * sizeof(double));
+  h_objs.resize(N);
+  for (int slot = 0; slot < N; ++slot) {
+    const int i = order[slot];
+    const GeometryDev& g = *geoms[i];
+    ObjDesc& d = h_objs[slot];
+    std::memset(&d, 0, sizeof(d));
+    d.type = g.type;
+    d.n = g.n;
+    d.nv = g.nv;
+    d.npatch = g.npatch;
+    d.dof_kind = kinds[i];
+    d.col = i;
+    d.verts = g.verts.as<double>();
+    d.faces = g.faces.as<int32_t>();
+    d.dofvid = g.dofvid.as<int32_t>();
+    d.patch_hdr = g.patch_hdr.as<int32_t>();
+    d.patch_verts = g.patch_verts.as<int32_t>();
+    d.patch_faces = g.patch_faces.as<uint32_t>();
+    d.patch_floc = g.patch_floc.as<uint32_t>();
+    d.el = g.el.as<double>();
+    d.dofs = c.ws_dofs.as<double>() + dof_off[i];
+    d.world = c.ws_world.as<double>() + wv_off[slot];
+    d.rec = c.ws_faces.as<double>() + rec_off[slot];
+  }
+  // Descriptor table + prefix sums in one upload.
+  const size_t obj_bytes = align_up(sizeof(ObjDesc) * N, 256);
+  const size_t pre_bytes = align_up(sizeof(int) * (N + 1), 256);
+  h_blob.assign(obj_bytes + 2 * pre_bytes, 0);
+  std::memcpy(h_blob.data(), h_objs.data(), sizeof(ObjDesc) * N);
+  std::memcpy(h_blob.data() + obj_bytes, h_vprefix.data(), sizeof(int) * (N + 1));
+  std::memcpy(h_blob.data() + obj_bytes + pre_bytes, h_eprefix.data(), sizeof(int) * (N + 1));
+  upload(c.ws_misc, h_blob.data(), h_blob.size(), s);
+  d_objs = reinterpret_cast<const ObjDesc*>(c.ws_misc.p);
+  d_vprefix = reinterpret_cast<const int*>(static_cast<char*>(c.ws_misc.p) + obj_bytes);
+  d_eprefix = reinterpret_cast<const int*>(static_cast<char*>(c.ws_misc.p) + obj_bytes + pre_bytes);
+  total_v = h_vprefix[N];
+  total_e = h_eprefix[N];
+  c.phase_begin(0);
+  launch_prepass(s, d_objs, d_vprefix, total_v, d_eprefix, total_e, N, fast ? 1 : 0);
+  c.phase_end(0);
+  PII_HIP(hipGetLastError());
+}
+
+// Plain FP64 volume matrix: FP64 form factors over the full grid, FP64 Gram.
+void volume_matrix_fp64(Context& c, const Batch& b, const KGridDev& kg, double* V_host) {
+  hipStream_t s = c.stream;
+  const int N = b.N, Q = kg.Q;
+  const int ldJ = (int)align_up((size_t)Q, 64);
+  c.ws_J.ensure(sizeof(double2) * (size_t)ldJ * N);
+  c.phase_begin(1);
+  launch_ff_fp64(s, b.d_objs, N, kg.full.as<double>(), Q, c.ws_J.as<double2>(), ldJ);
+  c.phase_end(1);
+  PII_HIP(hipGetLastError());
+  const int nsplit = gram_fp64_splits(Q, N);
+  c.ws_part.ensure(sizeof(double) * (size_t)nsplit * N * N);
+  c.ws_V.ensure(sizeof(double) * (size_t)N * N);
+  c.phase_begin(2);
+  launch_gram_fp64(s, c.ws_J.as<double2>(), Q, N, ldJ, kg.full.as<double>() + 7 * (size_t)Q,
+                   c.ws_part.as<double>(), nsplit, c.ws_V.as<double>());
+  c.phase_end(2);
+  PII_HIP(hipGetLastError());
+  PII_HIP(hipMemcpyAsync(V_host, c.ws_V.p, sizeof(double) * (size_t)N * N,
+                         hipMemcpyDeviceToHost, s));
+  PII_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace pii
+
+using namespace pii;
+
+extern "C" {
+
+const char* pii_last_error(void) { return pii::last_error(); }
+const char* pii_version(void) { return "pointint_hip 0.1 (gfx950)"; }
+
+int pii_gauss_legendre(int n, double a, double b, double* x, double* w) {
+  return guarded([&] {
+    require(n > 0 && x && w, "pii_gauss_legendre: n > 0 and output arrays required");
+    gauss_legendre(n, a, b, x, w);
+  });
+}
+
+int pii_lebedev_load(const char* path, double* dirs, double* w, int cap, int* count) {
+  return guarded([&] {
+    require(path != nullptr, "pii_lebedev_load: path is NULL");
+    std::vector<double> d, ww;
+    if (lebedev_load(path, d, ww) != PII_OK)
+      throw Error(PII_ERR_IO, std::string("failed to read Lebedev file: ") + path);
+    const int M = (int)ww.size();
+    if (count) *count = M;
+    if (dirs && w) {
+      const int m = std::min(M, cap);
+      std::memcpy(dirs, d.data(), sizeof(double) * 3 * m);
+      std::memcpy(w, ww.data(), sizeof(double) * m);
+    }
+  });
+}
+
+int pii_kgrid_build(const double* leb_dirs, const double* leb_w, int M, int nrad, double* dirs,
+                    double* kmag, double* w) {
+  return guarded([&] {
+    require(M > 0 && nrad > 0, "pii_kgrid_build: M and nrad must be positive");
+    require(leb_dirs && leb_w && dirs && kmag && w, "pii_kgrid_build: NULL array");
+    kgrid_build(leb_dirs, leb_w, M, nrad, dirs, kmag, w);
+  });
+}
+
+int pii_context_create(int device, pii_context** out) {
+  return guarded([&] {
+    require(out != nullptr, "pii_context_create: out is NULL");
+    int ndev = 0;
+    PII_HIP(hipGetDeviceCount(&ndev));
+    require(device >= 0 && device < ndev, "pii_context_create: no HIP device " +
+                                              std::to_string(device));
+    PII_HIP(hipSetDevice(device));
+    auto* ctx = new pii_context();
+    ctx->c.device = device;
+    if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+      delete ctx;
+      throw Error(PII_ERR_HIP, "hipStreamCreate failed");
+    }
+    *out = ctx;
+  });
+}
+
+int pii_context_destroy(pii_context* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    for (auto& pe : ctx->c.ev)
+      for (auto& e : pe)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+  });
+}
+
+int pii_context_synchronize(pii_context* ctx) {
+  return guarded([&] {
+    require(ctx != nullptr, "NULL context");
+    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int pii_context_set_profiling(pii_context* ctx, int enable) {
+  return guarded([&] {
+    require(ctx != nullptr, "NULL context");
+    ctx->c.profiling = enable != 0;
+    for (bool& u : ctx->c.ev_used) u = false;
+  });
+}
+
+int pii_context_timings(pii_context* ctx, double* ms, int n) {
+  return guarded([&] {
+    require(ctx && ms && n > 0, "pii_context_timings: bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    for (int p = 0; p < n; ++p) {
+      ms[p] = 0.0;
+      if (p < Context::kPhases && ctx->c.ev_used[p]) {
+        float t = 0.f;
+        PII_HIP(hipEventSynchronize(ctx->c.ev[p][1]));
+        PII_HIP(hipEventElapsedTime(&t, ctx->c.ev[p][0], ctx->c.ev[p][1]));
+        ms[p] = t;
+      }
+    }
+  });
+}
+
+int pii_kgrid_create(pii_context* ctx, int Q, const double* dirs, const double* kmag,
+                     const double* w, pii_kgrid** out) {
+  return guarded([&] {
+    require(ctx && out && dirs && kmag && w && Q > 0, "pii_kgrid_create: bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    auto kg = std::make_unique<pii_kgrid>();
+    kg->ctx = ctx;
+    KGridDev& k = kg->k;
+    k.Q = Q;
+    k.h_dirs.assign(dirs, dirs + 3 * (size_t)Q);
+    k.h_kmag.assign(kmag, kmag + Q);
+    k.h_w.assign(w, w + Q);
+    k.symmetric = kgrid_pairs(Q, dirs, kmag, w, k.h_anti, k.h_rep);
+    k.Qh = (int)k.h_rep.size();
+    auto soa = [&](const std::vector<int>& idx, bool sum_pair) {
+      const size_t n = idx.size();
+      std::vector<double> a(8 * n);
+      for (size_t i = 0; i < n; ++i) {
+        const int q = idx[i];
+        for (int c = 0; c < 3; ++c) {
+          a[c * n + i] = kmag[q] * dirs[3 * (size_t)q + c];
+          a[(3 + c) * n + i] = dirs[3 * (size_t)q + c];
+        }
+        a[6 * n + i] = kmag[q];
+        a[7 * n + i] = (sum_pair && k.h_anti[q] >= 0) ? w[q] + w[k.h_anti[q]] : w[q];
+      }
+      return a;
+    };
+    std::vector<int> all(Q);
+    std::iota(all.begin(), all.end(), 0);
+    const std::vector<double> full = soa(all, false);
+    const std::vector<double> rep = soa(k.h_rep, k.symmetric);
+    upload(k.full, full.data(), full.size(), ctx->c.stream);
+    upload(k.rep, rep.data(), rep.size(), ctx->c.stream);
+    upload(k.rep_index, k.h_rep.data(), k.h_rep.size(), ctx->c.stream);
+    upload(k.anti_index, k.h_anti.data(), k.h_anti.size(), ctx->c.stream);
+    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+    *out = kg.release();
+  });
+}
+
+int pii_kgrid_destroy(pii_kgrid* kg) {
+  return guarded([&] {
+    if (!kg) return;
+    (void)hipSetDevice(kg->ctx->c.device);
+    delete kg;
+  });
+}
+
+int pii_kgrid_info(const pii_kgrid* kg, int* Q, int* Qh, int* symmetric) {
+  return guarded([&] {
+    require(kg != nullptr, "NULL k-grid");
+    if (Q) *Q = kg->k.Q;
+    if (Qh) *Qh = kg->k.Qh;
+    if (symmetric) *symmetric = kg->k.symmetric ? 1 : 0;
+  });
+}
+
+static void finish_triangles(pii_context* ctx, GeometryDev& g) {
+  hipStream_t s = ctx->c.stream;
+  upload(g.verts, g.h_verts.data(), g.h_verts.size(), s);
+  upload(g.faces, g.h_faces.data(), g.h_faces.size(), s);
+  upload(g.dofvid, g.h_dofvid.data(), g.h_dofvid.size(), s);
+  std::vector<int> hdr, pverts;
+  std::vector<uint32_t> pfaces, ploc;
+  build_patches(g.nv, g.n, g.h_faces.data(), hdr, pverts, pfaces, ploc);
+  g.npatch = (int)hdr.size() / 4;
+  upload(g.patch_hdr, hdr.data(), hdr.size(), s);
+  upload(g.patch_verts, pverts.data(), pverts.size(), s);
+  upload(g.patch_faces, pfaces.data(), pfaces.size(), s);
+  upload(g.patch_floc, ploc.data(), ploc.size(), s);
+  PII_HIP(hipStreamSynchronize(s));
+}
+
+int pii_geometry_create_mesh(pii_context* ctx, const double* V, int nv, const int32_t* F, int nf,
+                             pii_geometry** out) {
+  return guarded([&] {
+    require(ctx && out && nv >= 0 && nf >= 0, "pii_geometry_create_mesh: bad arguments");
+    require((nv == 0 || V) && (nf == 0 || F), "pii_geometry_create_mesh: NULL array");
+    for (int i = 0; i < 3 * nf; ++i)
+      require(F[i] >= 0 && F[i] < nv, "face index out of range at " + std::to_string(i));
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    auto g = std::make_unique<pii_geometry>();
+    g->ctx = ctx;
+    GeometryDev& d = g->g;
+    d.type = PII_GEOM_TRIANGLE;
+    d.n = nf;
+    d.nv = nv;
+    d.h_verts.assign(V, V + 3 * (size_t)nv);
+    d.h_faces.assign(F, F + 3 * (size_t)nf);
+    d.h_dofvid.resize(nv);
+    std::iota(d.h_dofvid.begin(), d.h_dofvid.end(), 0);
+    d.ndof_verts = nv;
+    finish_triangles(ctx, d);
+    *out = g.release();
+  });
+}
+
+int pii_geometry_create_tris(pii_context* ctx, const double* tri, const int32_t* vid, int nt,
+                             pii_geometry** out) {
+  return guarded([&] {
+    require(ctx && out && nt >= 0 && (nt == 0 || tri), "pii_geometry_create_tris: bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    auto g = std::make_unique<pii_geometry>();
+    g->ctx = ctx;
+    GeometryDev& d = g->g;
+    d.type = PII_GEOM_TRIANGLE;
+    d.n = nt;
+    d.nv = 3 * nt;
+    d.h_verts.resize(9 * (size_t)nt);
+    d.h_faces.resize(3 * (size_t)nt);
+    d.h_dofvid.resize(3 * (size_t)nt);
+    int maxv = -1;
+    for (int i = 0; i < nt; ++i) {
+      const double* r = tri + 12 * (size_t)i;
+      for (int c = 0; c < 3; ++c) {
+        const double off0 = (c == 1) ? r[3] : (c == 2) ? r[6] : 0.0;
+        const double off1 = (c == 1) ? r[4] : (c == 2) ? r[7] : 0.0;
+        const double off2 = (c == 1) ? r[5] : (c == 2) ? r[8] : 0.0;
+        d.h_verts[9 * (size_t)i + 3 * c + 0] = r[0] + off0;
+        d.h_verts[9 * (size_t)i + 3 * c + 1] = r[1] + off1;
+        d.h_verts[9 * (size_t)i + 3 * c + 2] = r[2] + off2;
+        d.h_faces[3 * (size_t)i + c] = 3 * i + c;
+        const int v = vid ? vid[3 * (size_t)i + c] : -1;
+        d.h_dofvid[3 * (size_t)i + c] = v;
+        maxv = std::max(maxv, v);
+      }
+      // A face with any negative vid contributes nothing under vertex DoFs.
+      if (vid && (vid[3 * (size_t)i] < 0 || vid[3 * (size_t)i + 1] < 0 || vid[3 * (size_t)i + 2] < 0))
+        for (int c = 0; c < 3; ++c) d.h_dofvid[3 * (size_t)i + c] = -1;
+    }
+    d.ndof_verts = maxv + 1;
+    finish_triangles(ctx, d);
+    *out = g.release();
+  });
+}
+
+static int create_surfels(pii_context* ctx, int type, const double* el, int n, pii_geometry** out) {
+  return guarded([&] {
+    require(ctx && out && n >= 0 && (n == 0 || el), "pii_geometry_create: bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    auto g = std::make_unique<pii_geometry>();
+    g->ctx = ctx;
+    GeometryDev& d = g->g;
+    d.type = type;
+    d.n = n;
+    d.h_el.assign(el, el + 8 * (size_t)n);
+    upload(d.el, d.h_el.data(), d.h_el.size(), ctx->c.stream);
+    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+    *out = g.release();
+  });
+}
+
+int pii_geometry_create_disks(pii_context* ctx, const double* el, int n, pii_geometry** out) {
+  return create_surfels(ctx, PII_GEOM_DISK, el, n, out);
+}
+
+int pii_geometry_create_gaussians(pii_context* ctx, const double* el, int n, pii_geometry** out) {
+  return create_surfels(ctx, PII_GEOM_GAUSSIAN, el, n, out);
+}
+
+int pii_geometry_destroy(pii_geometry* g) {
+  return guarded([&] {
+    if (!g) return;
+    (void)hipSetDevice(g->ctx->c.device);
+    delete g;
+  });
+}
+
+int pii_geometry_info(const pii_geometry* g, int* type, int* num_elements, int* num_dof_vertices) {
+  return guarded([&] {
+    require(g != nullptr, "NULL geometry");
+    if (type) *type = g->g.type;
+    if (num_elements) *num_elements = g->g.n;
+    if (num_dof_vertices) *num_dof_vertices = g->g.ndof_verts;
+  });
+}
+
+int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const double* dofs,
+                    int ndofs, const pii_kgrid* kg, int precision, double* out) {
+  return guarded([&] {
+    require(ctx && g && kg && dofs && out, "pii_form_factor: NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    const GeometryDev* gp = &g->g;
+    Batch b;
+    if (precision == PII_FF_FP64) {
+      b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
+      const int Q = kg->k.Q;
+      const int ldJ = (int)align_up((size_t)Q, 64);
+      ctx->c.ws_J.ensure(sizeof(double2) * (size_t)ldJ);
+      launch_ff_fp64(ctx->c.stream, b.d_objs, 1, kg->k.full.as<double>(), Q,
+                     ctx->c.ws_J.as<double2>(), ldJ);
+      PII_HIP(hipGetLastError());
+      PII_HIP(hipMemcpyAsync(out, ctx->c.ws_J.p, sizeof(double2) * Q, hipMemcpyDeviceToHost,
+                             ctx->c.stream));
+      PII_HIP(hipStreamSynchronize(ctx->c.stream));
+    } else if (precision == PII_FF_FAST) {
+      b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, true);
+      form_factor_fast_full(ctx->c, b, kg->k, out);
+    } else {
+      throw Error(PII_ERR_INVALID, "unknown form-factor precision");
+    }
+  });
+}
+
+int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                      const int* dof_kinds, const double* const* dofs, const int* ndofs,
+                      const pii_kgrid* kg, int ff_precision, int gram_mode, double* V) {
+  return guarded([&] {
+    require(ctx && kg && V, "pii_volume_matrix: NULL argument");
+    require(n >= 0, "pii_volume_matrix: negative object count");
+    if (n == 0) return;
+    require(geoms && dof_kinds && dofs && ndofs, "pii_volume_matrix: NULL array");
+    for (int i = 0; i < n; ++i) require(geoms[i] && dofs[i], "pii_volume_matrix: NULL object");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    std::vector<const GeometryDev*> gs(n);
+    for (int i = 0; i < n; ++i) gs[i] = &geoms[i]->g;
+    Batch b;
+    if (ff_precision == PII_FF_FP64 && gram_mode == PII_GRAM_FP64) {
+      b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, false);
+      volume_matrix_fp64(ctx->c, b, kg->k, V);
+    } else if (ff_precision == PII_FF_FAST) {
+      b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true);
+      volume_matrix_fast(ctx->c, b, kg->k, gram_mode, V);
+    } else {
+      throw Error(PII_ERR_INVALID, "unsupported precision / Gram mode combination");
+    }
+  });
+}
+
+int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int gram_mode,
+             double* V) {
+  return guarded([&] {
+    require(ctx && J && kg && V && n > 0, "pii_gram: bad arguments");
+    require(gram_mode == PII_GRAM_FP64, "pii_gram: only the FP64 mode takes caller J");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    const int Q = kg->k.Q;
+    const int ldJ = Q;
+    upload(ctx->c.ws_J, reinterpret_cast<const double2*>(J), (size_t)Q * n, ctx->c.stream);
+    const int nsplit = gram_fp64_splits(Q, n);
+    ctx->c.ws_part.ensure(sizeof(double) * (size_t)nsplit * n * n);
+    ctx->c.ws_V.ensure(sizeof(double) * (size_t)n * n);
+    launch_gram_fp64(ctx->c.stream, ctx->c.ws_J.as<double2>(), Q, n, ldJ,
+                     kg->k.full.as<double>() + 7 * (size_t)Q, ctx->c.ws_part.as<double>(), nsplit,
+                     ctx->c.ws_V.as<double>());
+    PII_HIP(hipGetLastError());
+    PII_HIP(hipMemcpyAsync(V, ctx->c.ws_V.p, sizeof(double) * (size_t)n * n,
+                           hipMemcpyDeviceToHost, ctx->c.stream));
+    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// Device-side types and math shared by the form-factor, Gram and adjoint
+// kernels.  Everything here is written for gfx950 (wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pii {
+
+// Per-object descriptor of one batched launch.  Built on the host for every
+// call; device pointers refer to resident geometry and per-call workspaces.
+struct ObjDesc {
+  int type;            // PII_GEOM_*
+  int n;               // faces / disks / Gaussians
+  int nv;              // geometry vertices (triangles)
+  int npatch;          // triangle patches
+  int dof_kind;        // PII_DOF_*
+  int col;             // output column
+  int pad0, pad1;
+  const double* verts;        // nv*3 reference-frame vertices
+  const int32_t* faces;       // n*3
+  const int32_t* dofvid;      // nv
+  const int32_t* patch_hdr;   // npatch*4 {vbeg, vcnt, fbeg, fcnt}
+  const int32_t* patch_verts; // geometry vertex id per patch slot
+  const uint32_t* patch_faces;// face id per patch-ordered face
+  const uint32_t* patch_floc; // 3 x 8-bit local corner slots per patch-ordered face
+  const double* el;           // n*8 disks / Gaussians
+  const double* dofs;         // device DoFs (12 affine or 3V mesh)
+  double* world;              // nv*3 world vertices (written by the pre-pass)
+  double* rec;                // element records (written by the pre-pass)
+};
+
+// Element record layouts written by the pre-pass.
+//   plain triangle : 12 doubles {a, e1, e2, S}            (world frame)
+//   fast triangle  :  8 doubles {e1, e2, (2S.xyz f32, loc u32)} in patch order
+//   disk/Gaussian  :  8 doubles {c' = A c + t, A n, rho|sigma, area|w}
+constexpr int kRecPlainTri = 12;
+constexpr int kRecFastTri = 8;
+constexpr int kRecSurfel = 8;
+
+struct cplx {
+  double x, y;
+};
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// ---------------------------------------------------------------------------
+// Reference element functions in FP64 (cpp/src/element_functions.cpp:17-93).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ cplx ref_E(double z) {
+  if (fabs(z) < 1e-4) {
+    const double z2 = z * z, z4 = z2 * z2;
+    return {1.0 - z2 / 6.0 + z4 / 120.0, z * 0.5 - z * z2 / 24.0 + z4 * z / 720.0};
+  }
+  double s, c;
+  sincos(z, &s, &c);
+  return {s / z, (1.0 - c) / z};
+}
+
+__device__ __forceinline__ cplx ref_Ep(double z) {
+  if (fabs(z) < 1e-4) {
+    const double z2 = z * z, z3 = z2 * z, z4 = z2 * z2;
+    return {-z / 3.0 + z3 / 30.0, 0.5 - z2 / 8.0 + z4 / 144.0};
+  }
+  double s, c;
+  sincos(z, &s, &c);
+  const double z2 = z * z;
+  return {(z * c - s) / z2, (z * s - (1.0 - c)) / z2};
+}
+
+__device__ __forceinline__ cplx ref_Epp(double z) {
+  const double z2 = z * z;
+  if (fabs(z) < 1e-4) return {-1.0 / 3.0 + z2 / 10.0, -z / 4.0 + z2 * z / 36.0};
+  double s, c;
+  sincos(z, &s, &c);
+  const double z3 = z2 * z;
+  return {(2.0 * s - 2.0 * z * c - z2 * s) / z3, (2.0 * (1.0 - c) - 2.0 * z * s + z2 * c) / z3};
+}
+
+// Phi(a, b) = -2i (E(b) - E(a)) / (b - a); -2i E'((a+b)/2) when |b-a| < 1e-3.
+__device__ __forceinline__ cplx ref_Phi(double a, double b) {
+  const double d = b - a;
+  if (fabs(d) < 1e-3) {
+    const cplx e = ref_Ep(0.5 * (a + b));
+    return {2.0 * e.y, -2.0 * e.x};  // -2i * e
+  }
+  const cplx eb = ref_E(b), ea = ref_E(a);
+  const double inv = 1.0 / d;
+  const double dx = (eb.x - ea.x) * inv, dy = (eb.y - ea.y) * inv;
+  return {2.0 * dy, -2.0 * dx};
+}
+
+// ---------------------------------------------------------------------------
+// J1(x)/x in FP64.  Power series sum_m (-x^2/4)^m / (2 m! (m+1)!) for x <= 12
+// (terms summed until below 1e-17 of the running sum), Hankel asymptotic
+// expansion J1(x) = sqrt(2/(pi x)) (P cos chi - Q sin chi), chi = x - 3pi/4,
+// with P, Q summed to the smallest term beyond.  Accuracy ~1e-10 relative near
+// x = 12, better elsewhere.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double j1_over_x_f64(double x) {
+  const double ax = fabs(x);
+  if (ax < 1e-5) {
+    const double x2 = x * x;
+    return 0.5 - x2 / 16.0 + x2 * x2 / 384.0;
+  }
+  if (ax <= 12.0) {
+    const double q = 0.25 * x * x;
+    double term = 0.5, sum = 0.5;
+    for (int m = 1; m < 60; ++m) {
+      term *= -q / ((double)m * (double)(m + 1));
+      sum += term;
+      if (fabs(term) < 1e-17 * fabs(sum)) break;
+    }
+    return sum;
+  }
+  // a_k(nu=1) = prod_{j=1..k} (4 - (2j-1)^2) / (k! 8^k)
+  const double inv = 1.0 / ax;
+  double P = 1.0, Qs = 0.0, ak = 1.0, pw = 1.0, last = 1e300;
+  for (int k = 1; k < 40; ++k) {
+    const double f = (4.0 - (2.0 * k - 1.0) * (2.0 * k - 1.0)) / (8.0 * k);
+    ak *= f;
+    pw *= inv;
+    const double t = ak * pw;
+    if (fabs(t) > last) break;  // asymptotic series: stop at the smallest term
+    last = fabs(t);
+    // k odd -> Q term with sign (-1)^((k-1)/2); k even -> P term with sign (-1)^(k/2)
+    if (k & 1) Qs += ((((k - 1) >> 1) & 1) ? -t : t);
+    else P += (((k >> 1) & 1) ? -t : t);
+    if (last < 1e-17) break;
+  }
+  double s, c;
+  sincos(ax - 0.75 * 3.14159265358979323846, &s, &c);
+  const double j1 = sqrt(2.0 / (3.14159265358979323846 * ax)) * (P * c - Qs * s);
+  return j1 / ax;
+}
+
+// Binary search of the object owning flat index i in prefix[0..N].
+__device__ __forceinline__ int find_owner(const int* prefix, int N, int i) {
+  int lo = 0, hi = N;  // prefix[lo] <= i < prefix[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace pii

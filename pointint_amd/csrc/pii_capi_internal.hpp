@@ -1,0 +1,37 @@
+// Host-side batch state shared by the C-ABI entry points and the
+// fast-path drivers.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "device_common.hpp"
+#include "pii_internal.hpp"
+
+namespace pii {
+
+size_t align_up(size_t x, size_t a);
+
+// One call's objects on the device: DoFs uploaded, descriptors built, the
+// pre-pass (world vertices + element records) launched on the context stream.
+struct Batch {
+  int N = 0;
+  std::vector<int> order;         // slot -> caller index (largest first)
+  std::vector<ObjDesc> h_objs;    // per slot
+  std::vector<int> h_vprefix, h_eprefix;
+  std::vector<double> h_dofs;
+  std::vector<char> h_blob;
+  const ObjDesc* d_objs = nullptr;
+  const int* d_vprefix = nullptr;
+  const int* d_eprefix = nullptr;
+  int total_v = 0, total_e = 0;
+  void build(Context& c, int n, const GeometryDev* const* geoms, const int* kinds,
+             const double* const* dofs, const int* ndofs, bool fast);
+};
+
+// Fast path (FP32 transcendentals, FP64 phases and Taylor branch).
+void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out_host);
+void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                        double* V_host);
+
+}  // namespace pii

@@ -1,0 +1,157 @@
+// Internal declarations shared by the host-side C-ABI implementation and the
+// HIP kernel translation units of libpointint_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pointint_hip.h"
+
+namespace pii {
+
+constexpr double kPi = 3.14159265358979323846;
+// 1 / (8 pi^3): the volume normalisation of V = Re(J^H D J) / (8 pi^3).
+constexpr double kInv8Pi3 = 1.0 / (8.0 * kPi * kPi * kPi);
+
+// ---------------------------------------------------------------------------
+// Errors: every entry point returns a status and records a thread-local
+// message readable through pii_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+void clear_error();
+
+#define PII_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      throw ::pii::Error(PII_ERR_HIP, std::string(#call " failed: ") +         \
+                                          hipGetErrorString(e_));              \
+    }                                                                          \
+  } while (0)
+
+struct Error {
+  int code;
+  std::string msg;
+  Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// ---------------------------------------------------------------------------
+// Device buffer owned by the library (hipMalloc / hipFree).
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release();
+  // Grow-only: keeps the allocation when it is already large enough.
+  void ensure(size_t n);
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// k-grid resident in HBM.
+//
+// Full grid: node q = ir*M + j (radial-major, as the reference builds it).
+// Hemisphere grid: when every direction of every shell has an antipode with
+// the same weight, A(-k) = -conj(A(k)) makes the +k and -k terms of the Gram
+// identical, so the Gram and the adjoint run over one representative per pair
+// with the pair's summed weight.  The representatives keep shell-major order
+// so a 64-node tile mostly shares one |k|.
+// ---------------------------------------------------------------------------
+struct KGridDev {
+  int Q = 0;                    // full node count
+  int Qh = 0;                   // representative count (== Q when not symmetric)
+  bool symmetric = false;
+  std::vector<double> h_dirs, h_kmag, h_w;  // host copies of the full grid
+  std::vector<int> h_rep;       // representative -> full index
+  std::vector<int> h_anti;      // full index -> antipode full index (or -1)
+  // Full grid, SoA (FP64): kx, ky, kz, dir.x, dir.y, dir.z, kmag, w
+  DevBuf full;                  // 8*Q doubles
+  // Representative grid, same SoA with w replaced by the pair's summed weight
+  DevBuf rep;                   // 8*Qh doubles
+  DevBuf rep_index;             // Qh int32: rep -> full index
+  DevBuf anti_index;            // Q int32: full -> antipode
+};
+
+// ---------------------------------------------------------------------------
+// Geometry resident in HBM.
+// ---------------------------------------------------------------------------
+// Faces of a triangle mesh are grouped into patches of at most kPatchVerts
+// distinct vertices and kPatchFaces faces, so that a workgroup can hold the
+// per-(vertex, k) phasors of one patch in LDS and share them between faces.
+constexpr int kPatchVerts = 64;
+constexpr int kPatchFaces = 128;
+
+struct GeometryDev {
+  int type = 0;     // PII_GEOM_*
+  int n = 0;        // element count (faces / disks / Gaussians)
+  // ---- triangles ----
+  int nv = 0;                        // geometry vertices
+  std::vector<double> h_verts;       // nv*3 reference-frame vertex positions
+  std::vector<int32_t> h_faces;      // n*3 indices into h_verts
+  std::vector<int32_t> h_dofvid;     // nv: geometry vertex -> DoF vertex (or -1)
+  int ndof_verts = 0;                // DoF vertex count for TRIMESH DoFs
+  DevBuf verts;                      // nv*3 FP64 (AoS)
+  DevBuf faces;                      // n*3 int32
+  DevBuf dofvid;                     // nv int32
+  // patches
+  int npatch = 0;
+  DevBuf patch_hdr;                  // npatch * 4 int32: vbeg, vcnt, fbeg, fcnt
+  DevBuf patch_verts;                // sum vcnt int32 (geometry vertex ids)
+  DevBuf patch_faces;                // n uint32: face id (low 24 bits... see below)
+  DevBuf patch_floc;                 // n uint32: local corner indices (3 x 8 bit)
+  // ---- disks / Gaussians: n x 8 FP64 (c.xyz, n.xyz, rho|sigma, area|w) ----
+  std::vector<double> h_el;
+  DevBuf el;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;                     // serialises calls on one context
+  // Reusable workspaces (grow-only).
+  DevBuf ws_world, ws_faces, ws_J, ws_X, ws_part, ws_V, ws_misc, ws_dofs, ws_cot, ws_grad;
+  // Phase timing with HIP events on the context stream (enable_profiling of
+  // the reference drivers, e.g. compute_intersection_volume_multi_object.cu:626-695).
+  bool profiling = false;
+  static constexpr int kPhases = 4;   // 0 pre-pass, 1 form factor, 2 Gram, 3 adjoint
+  hipEvent_t ev[kPhases][2] = {};
+  bool ev_used[kPhases] = {};
+  void phase_begin(int p);
+  void phase_end(int p);
+};
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (defined in the .hip translation units).
+// ---------------------------------------------------------------------------
+
+struct ObjDesc;
+
+// Pre-pass over all objects of a call: world vertices, then element records
+// (fast != 0: patch-ordered triangle records for the shared-vertex kernel).
+void launch_prepass(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix, int total_v,
+                    const int* d_eprefix, int total_e, int N, int fast);
+
+// Plain FP64 form factor (reference formula) over the full grid:
+// J[col*ldJ + q] = A_col(k_q), complex FP64.
+void launch_ff_fp64(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_kfull, int Q,
+                    double2* d_J, int ldJ);
+
+// Plain FP64 Gram over the full grid: V = Re(J^H diag(w) J)/(8 pi^3).
+int gram_fp64_splits(int Q, int N);
+void launch_gram_fp64(hipStream_t s, const double2* d_J, int Q, int N, int ldJ,
+                      const double* d_w, double* d_part, int nsplit, double* d_V);
+
+}  // namespace pii

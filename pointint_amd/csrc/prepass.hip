@@ -1,0 +1,109 @@
+// Per-call pre-pass: apply each object's DoFs once, producing world-space
+// vertices and the element records the form-factor kernels stream.  One
+// batched launch covers every object of a call (flat index over the prefix
+// sums of vertex / element counts).
+#include "device_common.hpp"
+#include "pii_internal.hpp"
+
+namespace pii {
+
+namespace {
+
+__global__ void __launch_bounds__(256) k_world_vertices(const ObjDesc* __restrict__ objs,
+                                                        const int* __restrict__ vprefix, int N,
+                                                        int total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int o = find_owner(vprefix, N, i);
+  const ObjDesc& d = objs[o];
+  const int v = i - vprefix[o];
+  double w0, w1, w2;
+  if (d.dof_kind == PII_DOF_AFFINE) {
+    // cpp/src/dof/affine_dof.cpp:83 -- x' = A x + t, A row-major in dofs[3:12]
+    const double* p = d.dofs;
+    const double x = d.verts[3 * v], y = d.verts[3 * v + 1], z = d.verts[3 * v + 2];
+    w0 = p[3] * x + p[4] * y + p[5] * z + p[0];
+    w1 = p[6] * x + p[7] * y + p[8] * z + p[1];
+    w2 = p[9] * x + p[10] * y + p[11] * z + p[2];
+  } else {
+    // cpp/src/dof/triangle_mesh_dof.cpp:46-47 -- vertices are the DoFs
+    const int dv = d.dofvid[v];
+    if (dv >= 0) {
+      w0 = d.dofs[3 * dv]; w1 = d.dofs[3 * dv + 1]; w2 = d.dofs[3 * dv + 2];
+    } else {
+      w0 = w1 = w2 = 0.0;  // degenerate: the face contributes exactly zero
+    }
+  }
+  d.world[3 * v] = w0;
+  d.world[3 * v + 1] = w1;
+  d.world[3 * v + 2] = w2;
+}
+
+// Element records.  fast != 0 writes the patch-ordered triangle layout.
+__global__ void __launch_bounds__(256) k_records(const ObjDesc* __restrict__ objs,
+                                                 const int* __restrict__ eprefix, int N,
+                                                 int total, int fast) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int o = find_owner(eprefix, N, i);
+  const ObjDesc& d = objs[o];
+  const int e = i - eprefix[o];
+  if (d.type == PII_GEOM_TRIANGLE) {
+    const int f = fast ? (int)d.patch_faces[e] : e;
+    const int i0 = d.faces[3 * f], i1 = d.faces[3 * f + 1], i2 = d.faces[3 * f + 2];
+    const double* W = d.world;
+    const double ax = W[3 * i0], ay = W[3 * i0 + 1], az = W[3 * i0 + 2];
+    const double e1x = W[3 * i1] - ax, e1y = W[3 * i1 + 1] - ay, e1z = W[3 * i1 + 2] - az;
+    const double e2x = W[3 * i2] - ax, e2y = W[3 * i2 + 1] - ay, e2z = W[3 * i2 + 2] - az;
+    // S = (e1 x e2)/2: cpp/src/dof/affine_dof.cpp:86, triangle_mesh_dof.cpp:54
+    const double sx = 0.5 * (e1y * e2z - e1z * e2y);
+    const double sy = 0.5 * (e1z * e2x - e1x * e2z);
+    const double sz = 0.5 * (e1x * e2y - e1y * e2x);
+    if (!fast) {
+      double* r = d.rec + (size_t)kRecPlainTri * e;
+      r[0] = ax; r[1] = ay; r[2] = az;
+      r[3] = e1x; r[4] = e1y; r[5] = e1z;
+      r[6] = e2x; r[7] = e2y; r[8] = e2z;
+      r[9] = sx; r[10] = sy; r[11] = sz;
+    } else {
+      double* r = d.rec + (size_t)kRecFastTri * e;
+      r[0] = e1x; r[1] = e1y; r[2] = e1z;
+      r[3] = e2x; r[4] = e2y; r[5] = e2z;
+      float* rf = reinterpret_cast<float*>(r + 6);
+      rf[0] = (float)(2.0 * sx);
+      rf[1] = (float)(2.0 * sy);
+      rf[2] = (float)(2.0 * sz);
+      reinterpret_cast<uint32_t*>(rf)[3] = d.patch_floc[e];
+    }
+  } else {
+    // cpp/src/dof/affine_dof.cpp:115-118: c' = A c + t; k'.n = (A^T k).n = k.(A n)
+    const double* p = d.dofs;
+    const double* s = d.el + (size_t)kRecSurfel * e;
+    const double cx = s[0], cy = s[1], cz = s[2], nx = s[3], ny = s[4], nz = s[5];
+    double* r = d.rec + (size_t)kRecSurfel * e;
+    r[0] = p[3] * cx + p[4] * cy + p[5] * cz + p[0];
+    r[1] = p[6] * cx + p[7] * cy + p[8] * cz + p[1];
+    r[2] = p[9] * cx + p[10] * cy + p[11] * cz + p[2];
+    r[3] = p[3] * nx + p[4] * ny + p[5] * nz;
+    r[4] = p[6] * nx + p[7] * ny + p[8] * nz;
+    r[5] = p[9] * nx + p[10] * ny + p[11] * nz;
+    r[6] = s[6];
+    r[7] = s[7];
+  }
+}
+
+}  // namespace
+
+void launch_prepass(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix, int total_v,
+                    const int* d_eprefix, int total_e, int N, int fast) {
+  if (total_v > 0) {
+    hipLaunchKernelGGL(k_world_vertices, dim3((total_v + 255) / 256), dim3(256), 0, s, d_objs,
+                       d_vprefix, N, total_v);
+  }
+  if (total_e > 0) {
+    hipLaunchKernelGGL(k_records, dim3((total_e + 255) / 256), dim3(256), 0, s, d_objs,
+                       d_eprefix, N, total_e, fast);
+  }
+}
+
+}  // namespace pii

@@ -1,0 +1,338 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each test checks the oracle against something that does not restate its own
+formula: closed-form transforms, invariants, brute-force quadrature, library
+routines (numpy / scipy), finite differences, and the stored fixtures.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.special as sp
+
+from pointint_amd import workloads as W
+
+LEB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pointint_amd",
+                   "data", "lebedev")
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def grid(O, order, nrad):
+    d, w = O.load_lebedev(os.path.join(LEB, f"lebedev_{order:03d}.txt"))
+    return O.build_kgrid(d, w, nrad)
+
+
+# ---------------------------------------------------------------------------
+# Quadrature
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 5, 16, 32, 64, 96])
+def test_gauss_legendre_matches_numpy(oracle, n):
+    x, w = oracle.gauss_legendre(n, 0.0, 0.5 * math.pi)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    xr = 0.25 * math.pi * (xr + 1.0)
+    wr = 0.25 * math.pi * wr
+    assert np.allclose(x, xr, rtol=0, atol=1e-13)
+    assert np.allclose(w, wr, rtol=1e-11, atol=0)
+    assert np.all(np.diff(x) > 0)
+
+
+@pytest.mark.parametrize("order", [3, 5, 9, 29, 41, 59, 89])
+def test_lebedev_rules(oracle, order):
+    d, w = oracle.load_lebedev(os.path.join(LEB, f"lebedev_{order:03d}.txt"))
+    assert abs(w.sum() - 4 * math.pi) < 1e-10
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-14)
+    # Exact for low-degree polynomials: int x^2 dOmega = 4 pi / 3, int x dOmega = 0
+    assert abs((w * d[:, 0] ** 2).sum() - 4 * math.pi / 3) < 1e-10
+    assert abs((w * d[:, 1]).sum()) < 1e-12
+    # centrally symmetric with equal weights
+    for j in range(0, len(w), max(1, len(w) // 50)):
+        m = np.argmin(np.linalg.norm(d + d[j], axis=1))
+        assert np.linalg.norm(d[m] + d[j]) < 1e-12 and abs(w[m] - w[j]) < 1e-14
+
+
+def test_kgrid_radial_integral(oracle):
+    dirs, kmag, w = grid(oracle, 29, 32)
+    # int_0^inf dk/(1+k^2) = pi/2 exactly under the tan map; times 4 pi.
+    assert abs((w / (1.0 + kmag ** 2)).sum() - 2 * math.pi ** 2) < 1e-10
+    # node ordering q = ir*M + j
+    M = 302
+    assert np.all(kmag[:M] == kmag[0]) and kmag[M] > kmag[0]
+    assert np.allclose(dirs[M:2 * M], dirs[:M])
+
+
+# ---------------------------------------------------------------------------
+# Element functions
+# ---------------------------------------------------------------------------
+def _phi_exact(a, b):
+    import mpmath as mp
+    mp.mp.dps = 40
+    a = mp.mpf(a)
+    b = mp.mpf(b)
+    # Phi = 2 * int_simplex e^{i(a u + b v)} du dv, by nested quadrature
+    f = lambda u, v: mp.exp(1j * (a * u + b * v))  # noqa: E731
+    return complex(2 * mp.quad(lambda u: mp.quad(lambda v: f(u, v), [0, 1 - u]), [0, 1]))
+
+
+@pytest.mark.parametrize("a,b", [(0.0, 0.0), (1e-5, -2e-5), (0.3, 0.3 + 5e-4), (0.3, 0.3 + 2e-3),
+                                 (2.0, -1.5), (7.0, 7.0004), (-3.0, 11.0), (25.0, 24.0)])
+def test_phi_against_quadrature(oracle, a, b):
+    ex = _phi_exact(a, b)
+    assert abs(oracle.Phi(a, b) - ex) <= 5e-8 * max(abs(ex), 1e-3)
+
+
+def test_phi_gradient_vs_fd(oracle):
+    for a, b in [(0.4, 1.3), (2.0, 2.0005), (-5.0, 3.0)]:
+        h = 1e-6
+        da = (oracle.Phi(a + h, b) - oracle.Phi(a - h, b)) / (2 * h)
+        db = (oracle.Phi(a, b + h) - oracle.Phi(a, b - h)) / (2 * h)
+        ga, gb = oracle.Phi_grad(a, b)
+        assert abs(ga - da) < 1e-6 and abs(gb - db) < 1e-6
+
+
+def test_E_continuity_across_taylor_window(oracle):
+    # Just above |z| = 1e-4 the reference's (1 - cos z)/z loses ~7 digits to
+    # cancellation (absolute error ~4e-13); the Taylor side is exact to 1e-20.
+    for z, tol in [(1e-4 * (1 - 1e-9), 4e-16), (1e-4 * (1 + 1e-9), 1e-12)]:
+        ex = complex(math.sin(z) / z, 2 * math.sin(z / 2) ** 2 / z)
+        assert abs(oracle.E(z) - ex) < tol
+
+
+@pytest.mark.parametrize("x", [1e-7, 1e-3, 0.5, 3.0, 11.9, 12.1, 40.0, 90.0])
+def test_j1_over_x_matches_scipy(oracle, x):
+    assert abs(oracle.J1_over_x(x) - sp.j1(x) / x) < 1e-14 * max(1.0, abs(sp.j1(x) / x))
+    assert abs(oracle.J1_over_x_prime(x) + sp.jv(2, x) / x) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# Form factors
+# ---------------------------------------------------------------------------
+def test_cube_transform_is_exact(oracle):
+    V, F = W.unit_cube()
+    g = oracle.geom_mesh(V, F)
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        k = rng.normal(size=3) * 10 ** rng.uniform(-2, 2)
+        ex = 1j * np.linalg.norm(k) * oracle.exact_cube_ff(k)
+        # The reference's E(z) formula loses up to ~1e-9 relative just above its
+        # 1e-4 Taylor window, which shows at small |k|.
+        assert abs(oracle.compute_A(g, k) - ex) < 1e-9 * abs(ex) + 1e-12 * np.linalg.norm(k)
+
+
+def test_affine_cube_is_mapped_box(oracle):
+    """A(k) of A*cube + t equals i|k| |det A| e^{ik.t} F_cube(A^T k)."""
+    V, F = W.unit_cube()
+    g = oracle.geom_mesh(V, F)
+    rng = np.random.default_rng(1)
+    A = np.eye(3) + 0.3 * rng.normal(size=(3, 3))
+    if np.linalg.det(A) < 0:
+        A[:, 0] *= -1
+    t = rng.normal(size=3)
+    dofs = np.concatenate([t, A.reshape(-1)])
+    for _ in range(10):
+        k = rng.normal(size=3) * 3
+        ex = 1j * np.linalg.norm(k) * np.linalg.det(A) * np.exp(1j * k @ t) * oracle.exact_cube_ff(A.T @ k)
+        assert abs(oracle.compute_A(g, k, dofs) - ex) < 1e-11
+
+
+def test_triangle_brute_force_quadrature(oracle):
+    """One triangle: gamma e^{ik.a} Phi == khat.S * (1/area) int_T e^{ik.x} dS, by
+    product Gauss-Legendre quadrature on the Duffy-mapped triangle."""
+    rng = np.random.default_rng(2)
+    xg, wg = np.polynomial.legendre.leggauss(60)
+    u = 0.5 * (xg + 1)
+    wu = 0.5 * wg
+    for _ in range(5):
+        a, b, c = rng.normal(size=(3, 3)) * 0.3
+        V = np.stack([a, b, c])
+        g = oracle.geom_mesh(V, np.array([[0, 1, 2]], dtype=np.int32))
+        k = rng.normal(size=3) * 8
+        e1, e2 = b - a, c - a
+        S = 0.5 * np.cross(e1, e2)
+        # x(s, t) = a + s e1 + s t (e2 - e1) ... use (u, v) = (s(1-t), s t), jac = s
+        s, t = np.meshgrid(u, u, indexing="ij")
+        ws = np.outer(wu, wu) * s
+        uu, vv = s * (1 - t), s * t
+        ph = np.exp(1j * (k @ a + uu * (k @ e1) + vv * (k @ e2)))
+        integral = 2.0 * (ws * ph).sum()  # 2 * int_simplex
+        ex = (k / np.linalg.norm(k)) @ S * integral
+        assert abs(oracle.compute_A(g, k) - ex) < 1e-12
+
+
+def test_invariants_on_icosphere(oracle):
+    V, F = W.icosphere(2)
+    g = oracle.geom_mesh(V * 0.7, F)
+    rng = np.random.default_rng(3)
+    t = np.array([0.3, -0.4, 0.2])
+    dt = np.concatenate([t, np.eye(3).reshape(-1)])
+    vol = oracle.compute_volume(g)
+    for _ in range(5):
+        k = rng.normal(size=3) * 4
+        a0 = oracle.compute_A(g, k)
+        assert abs(oracle.compute_A(g, -k) + np.conj(a0)) < 1e-13
+        assert abs(oracle.compute_A(g, k, dt) - np.exp(1j * k @ t) * a0) < 1e-12
+    k = np.array([1e-4, 2e-4, -1e-4])
+    assert abs(oracle.compute_A(g, k) / np.linalg.norm(k) - 1j * vol) < 1e-6
+
+
+def test_disk_and_gaussian_brute_force(oracle):
+    """Planar 2D quadrature of a disk / Gaussian footprint: int e^{ik.x} dS."""
+    rng = np.random.default_rng(4)
+    c = np.array([0.1, -0.2, 0.3])
+    n = rng.normal(size=3); n /= np.linalg.norm(n)
+    t1 = np.cross(n, [1.0, 0, 0]); t1 /= np.linalg.norm(t1)
+    t2 = np.cross(n, t1)
+    rho = 0.3
+    xr, wr = np.polynomial.legendre.leggauss(80)
+    rr = 0.5 * rho * (xr + 1)
+    wrr = 0.5 * rho * wr
+    th = np.linspace(0, 2 * math.pi, 256, endpoint=False)
+    R, T = np.meshgrid(rr, th, indexing="ij")
+    Wt = np.outer(wrr, np.full(256, 2 * math.pi / 256)) * R
+    pts = c[None, None, :] + R[..., None] * (np.cos(T)[..., None] * t1 + np.sin(T)[..., None] * t2)
+    gd = oracle.geom_disks(c[None], n[None], np.array([rho]))
+    sig = 0.07
+    gg = oracle.geom_gaussians(c[None], n[None], np.array([sig]), np.array([0.8]))
+    xg = np.linspace(-8 * sig, 8 * sig, 321)
+    dx = xg[1] - xg[0]
+    X, Y = np.meshgrid(xg, xg, indexing="ij")
+    pg = c[None, None, :] + X[..., None] * t1 + Y[..., None] * t2
+    dens = 0.8 / (2 * math.pi * sig ** 2) * np.exp(-(X ** 2 + Y ** 2) / (2 * sig ** 2))
+    for _ in range(4):
+        k = rng.normal(size=3) * 6
+        kh = k / np.linalg.norm(k)
+        ex = (kh @ n) * (Wt * np.exp(1j * pts @ k)).sum()
+        assert abs(oracle.compute_A(gd, k) - ex) < 1e-10
+        exg = (kh @ n) * (dens * np.exp(1j * pg @ k)).sum() * dx * dx
+        assert abs(oracle.compute_A(gg, k) - exg) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# Volumes
+# ---------------------------------------------------------------------------
+def test_cube_pair_volume_matrix(oracle):
+    """Mesh-based V equals V from the closed-form cube transform on the same grid."""
+    dirs, kmag, w = grid(oracle, 29, 32)
+    k = dirs * kmag[:, None]
+    shift = np.array([0.3, 0.2, 0.1])
+    Fc = np.array([oracle.exact_cube_ff(kk) for kk in k])
+    A0 = 1j * kmag * Fc
+    J_exact = np.stack([A0, A0 * np.exp(1j * k @ shift)], axis=1)
+    V_exact = (J_exact.conj().T @ (w[:, None] * J_exact)).real / (8 * math.pi ** 3)
+    wl = W.config_cube_pair()
+    Vm, _ = oracle.volume_matrix([oracle.geom_mesh(o.V, o.F) for o in wl.objects], dirs, kmag, w)
+    assert np.allclose(Vm, V_exact, rtol=1e-11, atol=0)
+    assert abs(Vm[0, 0] - 0.967991598925) < 5e-11 and abs(Vm[0, 1] - 0.519715487081) < 5e-11
+    # analytic values are sanity bounds only (quadrature error, few %)
+    assert abs(Vm[0, 0] - 1.0) < 0.05 and abs(Vm[0, 1] - 0.504) < 0.05
+
+
+def test_gram_matches_numpy(oracle):
+    rng = np.random.default_rng(5)
+    J = rng.normal(size=(500, 7)) + 1j * rng.normal(size=(500, 7))
+    w = rng.uniform(0.1, 2.0, size=500)
+    ref = (J.conj().T @ (w[:, None] * J)).real / (8 * math.pi ** 3)
+    assert np.allclose(oracle.gram(J, w), ref, rtol=1e-13, atol=1e-15)
+
+
+def test_self_volume_divergence_theorem(oracle):
+    V, F = W.unit_cube()
+    assert abs(oracle.compute_volume(oracle.geom_mesh(V, F)) - 1.0) < 1e-14
+    Vi, Fi = W.icosphere(3)
+    tets = np.einsum("ij,ij->i", Vi[Fi[:, 0]], np.cross(Vi[Fi[:, 1]], Vi[Fi[:, 2]])).sum() / 6
+    g = oracle.geom_mesh(Vi, Fi)
+    assert abs(oracle.compute_volume(g) - tets) < 1e-12
+    assert abs(oracle.compute_volume(g, Vi.reshape(-1), oracle.DOF_MESH) - tets) < 1e-12
+
+
+def test_analytic_intersections(oracle):
+    assert oracle.box_box([0, 0, 0], [0.5] * 3, [0.3, 0.2, 0.1], [0.5] * 3) == pytest.approx(0.504)
+    assert oracle.sphere_sphere([0, 0, 0], 1.0, [0, 0, 0], 0.5) == pytest.approx(4 * math.pi / 3 * 0.125)
+    assert oracle.sphere_sphere([0, 0, 0], 1.0, [2.0, 0, 0], 1.0) == 0.0
+    # lens volume for equal spheres: pi (4r + d)(2r - d)^2 / 12
+    d = 0.7
+    assert oracle.sphere_sphere([0, 0, 0], 1.0, [d, 0, 0], 1.0) == pytest.approx(
+        math.pi * (4 + d) * (2 - d) ** 2 / 12)
+
+
+def test_golden_fixtures_reproduce(oracle):
+    z = np.load(os.path.join(GOLD, "cube_pair_L32_M302.npz"))
+    dirs, kmag, w = grid(oracle, 29, 32)
+    g0 = oracle.geom_mesh(z["V0"], z["F"])
+    g1 = oracle.geom_mesh(z["V0"] + z["shift"], z["F"])
+    Vm, J = oracle.volume_matrix([g0, g1], dirs, kmag, w)
+    assert np.allclose(Vm, z["Vmat"], rtol=1e-13, atol=0)
+    assert np.allclose(J, z["J"], rtol=0, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# Gradients
+# ---------------------------------------------------------------------------
+def test_mesh_gradient_vs_fd(oracle):
+    Vi, Fi = W.icosphere(1)
+    V0 = Vi * 0.5
+    g = oracle.geom_mesh(V0, Fi)
+    x = V0.reshape(-1).copy()
+    k = np.array([2.0, -1.0, 3.0])
+    gr = oracle.compute_A_gradient(g, k, x, oracle.DOF_MESH)
+    h = 1e-6
+    for d in [0, 5, 17, 29]:
+        xp, xm = x.copy(), x.copy()
+        xp[d] += h; xm[d] -= h
+        fd = (oracle.compute_A(g, k, xp, oracle.DOF_MESH) - oracle.compute_A(g, k, xm, oracle.DOF_MESH)) / (2 * h)
+        assert abs(gr[d] - fd) < 1e-7
+    # rigid shift: sum over vertices of dA/dv = i k A
+    tot = gr.reshape(-1, 3).sum(axis=0)
+    a = oracle.compute_A(g, k, x, oracle.DOF_MESH)
+    assert np.allclose(tot, 1j * k * a, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["tri", "disk", "gauss"])
+def test_affine_gradient_vs_fd(oracle, kind):
+    rng = np.random.default_rng(6)
+    if kind == "tri":
+        Vi, Fi = W.icosphere(1)
+        g = oracle.geom_mesh(Vi * 0.5, Fi)
+    else:
+        P, Nn, r = W.sphere_pointcloud(60)
+        g = oracle.geom_disks(P * 0.5, Nn, r * 0.5) if kind == "disk" else \
+            oracle.geom_gaussians(P * 0.5, Nn, np.full(60, 0.05), np.full(60, 0.05))
+    A = np.eye(3) + 0.2 * rng.normal(size=(3, 3))
+    x = np.concatenate([rng.normal(size=3) * 0.2, A.reshape(-1)])
+    k = np.array([1.5, -2.0, 0.7])
+    gr = oracle.compute_A_gradient(g, k, x, oracle.DOF_AFFINE)
+    h = 1e-6
+    for d in range(12):
+        xp, xm = x.copy(), x.copy()
+        xp[d] += h; xm[d] -= h
+        fd = (oracle.compute_A(g, k, xp) - oracle.compute_A(g, k, xm)) / (2 * h)
+        assert abs(gr[d] - fd) < 1e-6 * max(1.0, abs(fd)), (d, gr[d], fd)
+
+
+def test_vjp_equals_fd_of_pair_volume(oracle):
+    """d V_01 / d vertices of object 0 by the VJP against central differences."""
+    dirs, kmag, w = grid(oracle, 5, 6)
+    Vi, Fi = W.icosphere(1)
+    V0 = Vi * 0.5
+    V1 = Vi * 0.45 + np.array([0.3, 0.1, 0.0])
+    g0, g1 = oracle.geom_mesh(V0, Fi), oracle.geom_mesh(V1, Fi)
+    x0 = V0.reshape(-1)
+    A1 = oracle.compute_A_grid(g1, dirs, kmag, V1.reshape(-1), oracle.DOF_MESH)
+    grad = oracle.vjp(g0, x0, oracle.DOF_MESH, dirs, kmag, w, A1)
+
+    def v01(x):
+        A0 = oracle.compute_A_grid(g0, dirs, kmag, x, oracle.DOF_MESH)
+        return (w * (A0 * np.conj(A1)).real).sum() / (8 * math.pi ** 3)
+
+    h = 1e-6
+    for d in [0, 4, 11, 20, 35]:
+        xp, xm = x0.copy(), x0.copy()
+        xp[d] += h; xm[d] -= h
+        assert abs(grad[d] - (v01(xp) - v01(xm)) / (2 * h)) < 1e-7
+    # volume gradient (divergence theorem) vs FD
+    gv = oracle.compute_volume_gradient_mesh(g0, x0)
+    for d in [0, 7]:
+        xp, xm = x0.copy(), x0.copy()
+        xp[d] += h; xm[d] -= h
+        fd = (oracle.compute_volume(g0, xp, oracle.DOF_MESH) - oracle.compute_volume(g0, xm, oracle.DOF_MESH)) / (2 * h)
+        assert abs(gv[d] - fd) < 1e-8
