@@ -145,6 +145,20 @@ int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
                       const int* dof_kinds, const double* const* dofs, const int* ndofs,
                       const pii_kgrid* kg, int ff_precision, int gram_mode, double* V);
 
+/* Adjoint: gradients of L = sum_ij dL_dV[i*n+j] V_ij with respect to every
+ * object's DoFs, written to grads[i] (ndofs[i] doubles), without forming
+ * dA/dtheta (vector-Jacobian product over the k-nodes).  V (optional, may be
+ * NULL) receives the volume matrix of the same call (FP64 Gram of the fast
+ * form factors).  Triangle geometry, AFFINE or TRIMESH DoFs.
+ * Replaces the GRADIENT flag of compute_intersection_volume_matrix_cuda
+ * (cpp/src/compute_intersection_volume_multi_object.cu:437-495), whose
+ * grad_matrix[i][j] = dV_ij/dtheta_i is this call with dL_dV = e_i e_j^T
+ * (and 2x for i == j, cpp/include/compute_intersection_volume_multi_object.hpp:22-26). */
+int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                          const int* dof_kinds, const double* const* dofs, const int* ndofs,
+                          const pii_kgrid* kg, const double* dL_dV, double* const* grads,
+                          double* V);
+
 /* Gram of caller-provided form factors: J[n*Q] complex (object-major,
  * interleaved re/im, host memory) over the full grid of kg. */
 int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int gram_mode,

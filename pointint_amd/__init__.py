@@ -23,4 +23,5 @@ from .core import (  # noqa: F401
     make_packed_triangles,
     make_point_cloud,
     make_triangle_mesh,
+    volume_matrix_vjp,
 )

@@ -58,6 +58,7 @@ SIGNATURES = {
     "pii_form_factor": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),
     "pii_volume_matrix": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _P], _I),
     "pii_gram": ([_P, _P, _I, _P, _I, _P], _I),
+    "pii_volume_matrix_vjp": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _P], _I),
 }
 
 _lib = None

@@ -298,6 +298,26 @@ def compute_volume_matrix(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None, do
     return V
 
 
+def volume_matrix_vjp(geoms: Sequence[Geometry], kgrid: KGrid, dL_dV, dofs=None,
+                      dof_kinds=None, return_volume: bool = True):
+    """Gradients of L = sum_ij dL_dV[i,j] V_ij w.r.t. each object's DoFs
+    (list of arrays), and V of the same call when return_volume."""
+    n = len(geoms)
+    ctx = geoms[0].ctx
+    kinds, dl = _dof_args(geoms, dofs, dof_kinds)
+    G = N.f64(dL_dV).reshape(n, n)
+    grads = [np.zeros(d.shape[0]) for d in dl]
+    gh = (C.c_void_p * n)(*[g.handle.value for g in geoms])
+    kh = (C.c_int * n)(*kinds)
+    dh = (C.c_void_p * n)(*[d.ctypes.data for d in dl])
+    nh = (C.c_int * n)(*[d.shape[0] for d in dl])
+    oh = (C.c_void_p * n)(*[g.ctypes.data for g in grads])
+    V = np.zeros((n, n)) if return_volume else None
+    N.check(N.load().pii_volume_matrix_vjp(ctx.handle, n, gh, kh, dh, nh, kgrid.handle, N.ptr(G),
+                                           oh, N.ptr(V) if V is not None else None))
+    return grads, V
+
+
 def gram_from_J(J: np.ndarray, kgrid: KGrid, ctx: Optional[Context] = None) -> np.ndarray:
     """FP64 Gram of caller form factors J (Q x n complex)."""
     ctx = ctx or kgrid.ctx

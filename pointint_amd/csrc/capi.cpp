@@ -559,6 +559,25 @@ int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
   });
 }
 
+int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                          const int* dof_kinds, const double* const* dofs, const int* ndofs,
+                          const pii_kgrid* kg, const double* dL_dV, double* const* grads,
+                          double* V) {
+  return guarded([&] {
+    require(ctx && kg && dL_dV && grads, "pii_volume_matrix_vjp: NULL argument");
+    require(n > 0 && geoms && dof_kinds && dofs && ndofs, "pii_volume_matrix_vjp: bad arrays");
+    for (int i = 0; i < n; ++i)
+      require(geoms[i] && dofs[i] && grads[i], "pii_volume_matrix_vjp: NULL object");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    std::vector<const GeometryDev*> gs(n);
+    for (int i = 0; i < n; ++i) gs[i] = &geoms[i]->g;
+    Batch b;
+    b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true);
+    volume_matrix_vjp(ctx->c, b, gs.data(), kg->k, dL_dV, grads, V);
+  });
+}
+
 int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int gram_mode,
              double* V) {
   return guarded([&] {

@@ -1,0 +1,303 @@
+// Fast form factor for gfx950: FP64 dot products, phases, range reduction and
+// small-span Taylor branch; FP32 for the phasor polynomials and the complex
+// combination; FP64 accumulation across patches.
+//
+// Triangles (meshes and triangle soups) use a cancellation-free form of the
+// reference's element transform (cpp/src/element_functions.cpp:63-70 with the
+// accumulation of cpp/src/dof/triangle_mesh_dof.cpp:36-71):
+//
+//   A_tri(k) = gamma e^{i k.a} Phi(k.e1, k.e2) = gamma * 2 g[u0, u1, u2]
+//
+// where u_j = k.v_j, g(z) = -e^{iz} and g[...] is the second divided
+// difference.  With half phasors Q_j = e^{i u_j / 2}, m the node whose u lies
+// between the other two (p, q):
+//
+//   g[p, m, q] = -i Q_m (Q_q sinc(h_q) - Q_p sinc(h_p)) / (u_q - u_p),
+//   h_p = (u_m - u_p)/2,  h_q = (u_q - u_m)/2,
+//
+// and sin(h) = Im(Q_b conj Q_a) comes from the phasors themselves.  When the
+// span u_q - u_p is below kSpanTaylor the divided difference is summed from
+// its Taylor series in FP64:  Phi(a, b) = sum_n 2 i^n h_n(a, b) / (n+2)!,
+// h_n the complete homogeneous symmetric polynomial.
+//
+// A workgroup (4 waves) owns 64 k-nodes (one per lane, same nodes in every
+// wave) of one object.  Faces are grouped in patches of <= 64 vertices; per
+// patch the waves first fill an LDS table of the vertices' half phasors for
+// the 64 nodes, then each wave streams a quarter of the patch's face records
+// (wave-uniform, scalar loads) and reads its three phasors from LDS.
+#include "device_common.hpp"
+#include "pii_internal.hpp"
+
+namespace pii {
+
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kWaves = 4;
+constexpr double kSpanTaylor = 1.0;   // Taylor branch below this span (radians)
+constexpr float kSincPoly = 0.5f;     // sinc(h) by polynomial below |h| = 0.5
+
+// e^{i 2 pi r} for r in revolutions (|r| <= 0.5), FP32 result.  Quadrant
+// reduction in FP64, polynomials in FP32 on |x| <= pi/4 (Taylor to x^11 / x^10:
+// truncation < 1e-9, far below FP32 rounding).
+__device__ __forceinline__ float2 phasor_rev(double r) {
+  const double j = rint(4.0 * r);
+  const float x = (float)((r - 0.25 * j) * 6.283185307179586);
+  const float x2 = x * x;
+  float s = x2 * (-1.0f / 39916800.0f) + (1.0f / 362880.0f);
+  s = s * x2 - (1.0f / 5040.0f);
+  s = s * x2 + (1.0f / 120.0f);
+  s = s * x2 - (1.0f / 6.0f);
+  s = (s * x2) * x + x;
+  float c = x2 * (-1.0f / 3628800.0f) + (1.0f / 40320.0f);
+  c = c * x2 - (1.0f / 720.0f);
+  c = c * x2 + (1.0f / 24.0f);
+  c = c * x2 - 0.5f;
+  c = c * x2 + 1.0f;
+  const int q = ((int)j) & 3;
+  float2 o;
+  o.x = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
+  o.y = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
+  return o;
+}
+
+// e^{i u/2} (half phasor) and e^{i u} (full phasor) of an FP64 phase.
+__device__ __forceinline__ float2 half_phasor(double u) {
+  const double t = u * 0.07957747154594767;  // 1/(4 pi)
+  return phasor_rev(t - rint(t));
+}
+__device__ __forceinline__ float2 full_phasor(double u) {
+  const double t = u * 0.15915494309189535;  // 1/(2 pi)
+  return phasor_rev(t - rint(t));
+}
+
+__device__ __forceinline__ float sinc_from(float h, float sin_h) {
+  const float ah = fabsf(h);
+  if (ah < kSincPoly) {
+    const float h2 = h * h;
+    float p = h2 * (1.0f / 362880.0f) - (1.0f / 5040.0f);
+    p = p * h2 + (1.0f / 120.0f);
+    p = p * h2 - (1.0f / 6.0f);
+    return p * h2 + 1.0f;
+  }
+  return sin_h * __builtin_amdgcn_rcpf(h);
+}
+
+// gamma * e^{i u0} * Phi(alpha, beta), from the three half phasors.
+__device__ __forceinline__ float2 tri_term(double al, double be, float g2, float2 Q0, float2 Q1,
+                                           float2 Q2) {
+  const double lo = fmin(0.0, fmin(al, be));
+  const double hi = fmax(0.0, fmax(al, be));
+  const double H = hi - lo;
+  if (H < kSpanTaylor) {
+    // Phi(a, b) = sum_n c_n h_n(a, b), c_n = 2 i^n / (n+2)!, in FP64.
+    double an = 1.0, hn = 1.0;
+    double re = 1.0, im = 0.0;
+    an *= al; hn = fma(be, hn, an); im += hn * (1.0 / 3.0);
+    an *= al; hn = fma(be, hn, an); re -= hn * (1.0 / 12.0);
+    an *= al; hn = fma(be, hn, an); im -= hn * (1.0 / 60.0);
+    an *= al; hn = fma(be, hn, an); re += hn * (1.0 / 360.0);
+    an *= al; hn = fma(be, hn, an); im += hn * (1.0 / 2520.0);
+    an *= al; hn = fma(be, hn, an); re -= hn * (1.0 / 20160.0);
+    an *= al; hn = fma(be, hn, an); im -= hn * (1.0 / 181440.0);
+    an *= al; hn = fma(be, hn, an); re += hn * (1.0 / 1814400.0);
+    an *= al; hn = fma(be, hn, an); im += hn * (1.0 / 19958400.0);
+    an *= al; hn = fma(be, hn, an); re -= hn * (1.0 / 239500800.0);
+    an *= al; hn = fma(be, hn, an); im -= hn * (1.0 / 3113510400.0);
+    // e^{i u0} = Q0^2; contribution = (g2/2) e^{i u0} Phi
+    const float px = Q0.x * Q0.x - Q0.y * Q0.y;
+    const float py = 2.0f * Q0.x * Q0.y;
+    const float tr = (float)re * (0.5f * g2), ti = (float)im * (0.5f * g2);
+    return make_float2(px * tr - py * ti, px * ti + py * tr);
+  }
+  // Middle node m and the other two (p, q); see the file comment.
+  const bool m0 = (al * be <= 0.0);
+  const bool m1 = !m0 && (fabs(al) < fabs(be));
+  float2 Qm, Qp, Qq;
+  double hp, hq, D;
+  if (m0) {
+    Qm = Q0; Qp = Q1; Qq = Q2; hp = -0.5 * al; hq = 0.5 * be; D = be - al;
+  } else if (m1) {
+    Qm = Q1; Qp = Q0; Qq = Q2; hp = 0.5 * al; hq = 0.5 * (be - al); D = be;
+  } else {
+    Qm = Q2; Qp = Q0; Qq = Q1; hp = 0.5 * be; hq = 0.5 * (al - be); D = al;
+  }
+  const float sin_p = Qm.y * Qp.x - Qm.x * Qp.y;   // Im(Qm conj Qp) = sin(hp)
+  const float sin_q = Qq.y * Qm.x - Qq.x * Qm.y;   // Im(Qq conj Qm) = sin(hq)
+  const float sp = sinc_from((float)hp, sin_p);
+  const float sq = sinc_from((float)hq, sin_q);
+  const float nx = Qq.x * sq - Qp.x * sp;
+  const float ny = Qq.y * sq - Qp.y * sp;
+  const float tx = Qm.x * nx - Qm.y * ny;
+  const float ty = Qm.x * ny + Qm.y * nx;
+  const float f = g2 * __builtin_amdgcn_rcpf((float)D);
+  // -i * (tx + i ty) = ty - i tx
+  return make_float2(ty * f, -tx * f);
+}
+
+// J1(x)/x for x >= 0 in FP64: series below 12, Hankel asymptotics above
+// (6 terms each of P and Q: relative error < 1e-9 at x = 12).
+__device__ __forceinline__ double j1ox_fast(double x) {
+  if (x < 12.0) {
+    const double q = 0.25 * x * x;
+    double term = 0.5, sum = 0.5;
+    for (int m = 1; m < 40; ++m) {
+      term *= -q / (double)(m * (m + 1));
+      sum += term;
+      if (fabs(term) < 1e-12 * fabs(sum)) break;
+    }
+    return sum;
+  }
+  const double inv = 1.0 / x;
+  const double i2 = inv * inv;
+  // P = 1 - a2/x^2 + a4/x^4 - ..., Q = a1/x - a3/x^3 + ...  (nu = 1)
+  const double P = 1.0 + i2 * (0.1171875 + i2 * (-0.144195556640625 + i2 * (0.6765925884246826 +
+                   i2 * (-6.883914268109947 + i2 * (121.59789187653587 + i2 * -3302.2722944808525)))));
+  const double Qs = inv * (0.375 + i2 * (-0.1025390625 + i2 * (0.2775764465332031 +
+                    i2 * (-1.993531733751297 + i2 * (27.24882731126854 + i2 * -603.8440767050702)))));
+  const float2 ph = full_phasor(x - 2.356194490192345);  // chi = x - 3 pi/4
+  const double amp = sqrt(0.6366197723675814 * inv);
+  return amp * (P * (double)ph.x - Qs * (double)ph.y) * inv;
+}
+
+__global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ objs,
+                                                 const double* __restrict__ krep, int Qh,
+                                                 int out_mode, double2* __restrict__ Aout,
+                                                 int ldA, float2* __restrict__ Xout, int ldX,
+                                                 unsigned* __restrict__ colmax) {
+  // Vertex phasor table; after the last patch its first 4 KB hold the
+  // cross-wave reduction (every wave has passed the patch loop's final barrier).
+  __shared__ float2 tab[kPatchVerts][kLanes];
+  static_assert(sizeof(double2) * kWaves * kLanes <= sizeof(float2) * kPatchVerts * kLanes,
+                "reduction buffer must fit in the phasor table");
+  double2 (*red)[kLanes] = reinterpret_cast<double2 (*)[kLanes]>(&tab[0][0]);
+  const ObjDesc& d = objs[blockIdx.y];
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = threadIdx.x >> 6;
+  const int q = blockIdx.x * kLanes + lane;
+  const int qc = q < Qh ? q : Qh - 1;
+  const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
+  const float hx = (float)krep[3 * (size_t)Qh + qc], hy = (float)krep[4 * (size_t)Qh + qc],
+              hz = (float)krep[5 * (size_t)Qh + qc];
+  const double km = krep[6 * (size_t)Qh + qc];
+  double accx = 0.0, accy = 0.0;
+
+  if (d.type == PII_GEOM_TRIANGLE) {
+    for (int p = 0; p < d.npatch; ++p) {
+      const int vbeg = d.patch_hdr[4 * p], vcnt = d.patch_hdr[4 * p + 1];
+      const int fbeg = d.patch_hdr[4 * p + 2], fcnt = d.patch_hdr[4 * p + 3];
+      for (int s = wave; s < vcnt; s += kWaves) {
+        const int gv = d.patch_verts[vbeg + s];
+        const double* W = d.world + 3 * (size_t)gv;
+        const double u = kx * W[0] + ky * W[1] + kz * W[2];
+        tab[s][lane] = half_phasor(u);
+      }
+      __syncthreads();
+      float px = 0.f, py = 0.f;
+      for (int f = wave; f < fcnt; f += kWaves) {
+        const double* r = d.rec + (size_t)kRecFastTri * (fbeg + f);
+        const double al = kx * r[0] + ky * r[1] + kz * r[2];
+        const double be = kx * r[3] + ky * r[4] + kz * r[5];
+        const float* rf = reinterpret_cast<const float*>(r + 6);
+        const float g2 = hx * rf[0] + hy * rf[1] + hz * rf[2];
+        const uint32_t loc = reinterpret_cast<const uint32_t*>(rf)[3];
+        const float2 Q0 = tab[loc & 0xff][lane];
+        const float2 Q1 = tab[(loc >> 8) & 0xff][lane];
+        const float2 Q2 = tab[(loc >> 16) & 0xff][lane];
+        const float2 t = tri_term(al, be, g2, Q0, Q1, Q2);
+        px += t.x;
+        py += t.y;
+      }
+      accx += (double)px;
+      accy += (double)py;
+      __syncthreads();
+    }
+  } else {
+    // Disks / Gaussians: A = S(|k'_perp|) (k'.n)/|k| e^{i k.c'}, k' = A^T k,
+    // with c' and A n precomputed (cpp/src/dof/affine_dof.cpp:103-167).
+    const double* p = d.dofs;
+    const double kpx = p[3] * kx + p[6] * ky + p[9] * kz;
+    const double kpy = p[4] * kx + p[7] * ky + p[10] * kz;
+    const double kpz = p[5] * kx + p[8] * ky + p[11] * kz;
+    const double kp2 = kpx * kpx + kpy * kpy + kpz * kpz;
+    const double inv_km = 1.0 / km;
+    const bool disk = (d.type == PII_GEOM_DISK);
+    float px = 0.f, py = 0.f;
+    int cnt = 0;
+    for (int e = wave; e < d.n; e += kWaves) {
+      const double* r = d.rec + (size_t)kRecSurfel * e;
+      const double kdn = kx * r[3] + ky * r[4] + kz * r[5];
+      double S;
+      if (disk) {
+        const double rr = r[6] * sqrt(fmax(0.0, kp2 - kdn * kdn));
+        S = (rr < 1e-10) ? r[7] : 2.0 * r[7] * j1ox_fast(rr);
+      } else {
+        S = r[7] * (double)__expf((float)(-0.5 * r[6] * r[6] * (kp2 - kdn * kdn)));
+      }
+      const float Ap = (float)(S * kdn * inv_km);
+      const float2 ph = full_phasor(kx * r[0] + ky * r[1] + kz * r[2]);
+      px += Ap * ph.x;
+      py += Ap * ph.y;
+      if (++cnt == 64) {
+        accx += (double)px; accy += (double)py;
+        px = py = 0.f; cnt = 0;
+      }
+    }
+    accx += (double)px;
+    accy += (double)py;
+  }
+  if (km < 1e-10) { accx = 0.0; accy = 0.0; }
+
+  // Sum the four waves' partial sums of the same 64 nodes.
+  red[wave][lane] = make_double2(accx, accy);
+  __syncthreads();
+  if (wave == 0) {
+    double sx = 0.0, sy = 0.0;
+    for (int w = 0; w < kWaves; ++w) { sx += red[w][lane].x; sy += red[w][lane].y; }
+    if (out_mode == 0) {
+      if (q < Qh) Aout[(size_t)d.col * ldA + q] = make_double2(sx, sy);
+    } else {
+      const double sw = sqrt(krep[7 * (size_t)Qh + qc]);
+      const float2 x = make_float2((float)(sx * sw), (float)(sy * sw));
+      if (q < ldX) Xout[(size_t)d.col * ldX + q] = (q < Qh) ? x : make_float2(0.f, 0.f);
+      // column maximum of |X| (all 64 lanes take part in the shuffles)
+      float m = (q < Qh) ? fmaxf(fabsf(x.x), fabsf(x.y)) : 0.0f;
+      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+      if (lane == 0) atomicMax(colmax + d.col, __float_as_uint(m));
+    }
+  }
+}
+
+// Full-grid expansion of representative-node values: A(-k) = -conj(A(k)).
+__global__ void __launch_bounds__(256) k_expand(const double2* __restrict__ Arep, int ldA,
+                                                const int* __restrict__ rep_index,
+                                                const int* __restrict__ anti, int Qh, int Q,
+                                                int N, double2* __restrict__ Afull) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y;
+  if (i >= Qh || col >= N) return;
+  const double2 a = Arep[(size_t)col * ldA + i];
+  const int q = rep_index[i];
+  Afull[(size_t)col * Q + q] = a;
+  const int qa = anti[q];
+  if (qa >= 0 && qa != q) Afull[(size_t)col * Q + qa] = make_double2(-a.x, a.y);
+}
+
+}  // namespace
+
+void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
+                    int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                    unsigned* d_colmax) {
+  if (N <= 0 || Qh <= 0) return;
+  hipLaunchKernelGGL(k_ff_fast, dim3((Qh + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves), 0, s,
+                     d_objs, d_krep, Qh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+}
+
+void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,
+                   const int* d_anti, int Qh, int Q, int N, double2* d_Afull) {
+  if (N <= 0 || Qh <= 0) return;
+  hipLaunchKernelGGL(k_expand, dim3((Qh + 255) / 256, N), dim3(256), 0, s, d_Arep, ldA, d_rep,
+                     d_anti, Qh, Q, N, d_Afull);
+}
+
+}  // namespace pii

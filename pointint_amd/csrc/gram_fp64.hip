@@ -17,7 +17,10 @@ namespace {
 constexpr int kT = 32;   // tile edge (objects)
 constexpr int kQc = 32;  // q-chunk staged in LDS
 
-__global__ void __launch_bounds__(256) k_gram_fp64(const double2* __restrict__ J, int Q, int N,
+// T = double2 (form factors, weights applied here) or float2 (fast-path X,
+// sqrt(w) already folded in, w == nullptr).
+template <class T>
+__global__ void __launch_bounds__(256) k_gram_fp64(const T* __restrict__ J, int Q, int N,
                                                    int ldJ, const double* __restrict__ w,
                                                    double* __restrict__ part, int qslice) {
   // Upper-triangular tile pair from the linear index.
@@ -43,12 +46,14 @@ __global__ void __launch_bounds__(256) k_gram_fp64(const double2* __restrict__ J
       const int oi = ti * kT + r, oj = tj * kT + r;
       double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
       if (q < q_end) {
-        const double wq = w[q];
-        if (oi < N) a = J[(size_t)oi * ldJ + q];
+        const double wq = w ? w[q] : 1.0;
+        if (oi < N) {
+          const T v = J[(size_t)oi * ldJ + q];
+          a = make_double2((double)v.x, (double)v.y);
+        }
         if (oj < N) {
-          b = J[(size_t)oj * ldJ + q];
-          b.x *= wq;
-          b.y *= wq;
+          const T v = J[(size_t)oj * ldJ + q];
+          b = make_double2((double)v.x * wq, (double)v.y * wq);
         }
       }
       sa[r][c] = a;
@@ -98,18 +103,29 @@ int gram_fp64_splits(int Q, int N) {
   return s;
 }
 
-void launch_gram_fp64(hipStream_t s, const double2* d_J, int Q, int N, int ldJ,
-                      const double* d_w, double* d_part, int nsplit, double* d_V) {
+template <class T>
+static void gram_fp64_impl(hipStream_t s, const T* d_J, int Q, int N, int ldJ, const double* d_w,
+                           double* d_part, int nsplit, double* d_V) {
   if (N <= 0) return;
   const int nt = (N + kT - 1) / kT;
   const int tiles = nt * (nt + 1) / 2;
   int qslice = (Q + nsplit - 1) / nsplit;
   qslice = (qslice + kQc - 1) / kQc * kQc;
-  hipLaunchKernelGGL(k_gram_fp64, dim3(tiles, nsplit), dim3(256), 0, s, d_J, Q, N, ldJ, d_w,
+  hipLaunchKernelGGL(k_gram_fp64<T>, dim3(tiles, nsplit), dim3(256), 0, s, d_J, Q, N, ldJ, d_w,
                      d_part, qslice);
   const size_t total = (size_t)N * N;
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      d_part, nsplit, N, kInv8Pi3, d_V);
+}
+
+void launch_gram_fp64(hipStream_t s, const double2* d_J, int Q, int N, int ldJ,
+                      const double* d_w, double* d_part, int nsplit, double* d_V) {
+  gram_fp64_impl(s, d_J, Q, N, ldJ, d_w, d_part, nsplit, d_V);
+}
+
+void launch_gram_fp64_x(hipStream_t s, const float2* d_X, int Qh, int N, int ldX, double* d_part,
+                        int nsplit, double* d_V) {
+  gram_fp64_impl(s, d_X, Qh, N, ldX, (const double*)nullptr, d_part, nsplit, d_V);
 }
 
 }  // namespace pii

@@ -34,4 +34,9 @@ void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, doubl
 void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
                         double* V_host);
 
+// Vertex-DoF / affine-DoF gradients of L = sum_ij G_ij V_ij (triangles).
+void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geoms,
+                       const KGridDev& kg, const double* G, double* const* grads,
+                       double* V_host);
+
 }  // namespace pii

@@ -123,6 +123,7 @@ struct Context {
   std::mutex mu;                     // serialises calls on one context
   // Reusable workspaces (grow-only).
   DevBuf ws_world, ws_faces, ws_J, ws_X, ws_part, ws_V, ws_misc, ws_dofs, ws_cot, ws_grad;
+  DevBuf ws_B, ws_C, ws_gw;
   // Phase timing with HIP events on the context stream (enable_profiling of
   // the reference drivers, e.g. compute_intersection_volume_multi_object.cu:626-695).
   bool profiling = false;
@@ -153,5 +154,33 @@ void launch_ff_fp64(hipStream_t s, const ObjDesc* d_objs, int N, const double* d
 int gram_fp64_splits(int Q, int N);
 void launch_gram_fp64(hipStream_t s, const double2* d_J, int Q, int N, int ldJ,
                       const double* d_w, double* d_part, int nsplit, double* d_V);
+
+// FP64 Gram of fast-path X (float2, sqrt(w) folded in, Qh representative nodes).
+void launch_gram_fp64_x(hipStream_t s, const float2* d_X, int Qh, int N, int ldX, double* d_part,
+                        int nsplit, double* d_V);
+
+// Fast form factor over the representative nodes.  out_mode 0: A (FP64) to
+// d_A[col*ldA + q]; out_mode 1: X = sqrt(w) A (FP32) to d_X[col*ldX + q] for
+// q < ldX (zeros beyond Qh) and the column max |X| to d_colmax[col].
+void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
+                    int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                    unsigned* d_colmax);
+// Full-grid values from representative values: A(-k) = -conj(A(k)).
+void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,
+                   const int* d_anti, int Qh, int Q, int N, double2* d_Afull);
+
+// Split-precision MFMA Gram (fp16 != 0: FP16x3 with column scaling; else BF16x3).
+int gram_split_splits(int K, int N);
+void launch_gram_split(hipStream_t s, int fp16, const float* d_X, int N, int ldK,
+                       const unsigned* d_colmax, double* d_part, int nsplit, double* d_V);
+
+// Adjoint: Y = X C (complex columns mixed by a real N x N matrix), then the
+// per-face vertex VJP of triangle objects into world-vertex gradients
+// d_gworld + d_goff[slot] (FP64, zeroed by the caller, atomically accumulated).
+void launch_mix_columns(hipStream_t s, const float2* d_X, int N, int ldX, int Qh,
+                        const double* d_C, float2* d_Y);
+void launch_adjoint_tri(hipStream_t s, const ObjDesc* d_objs, int nobj, int max_patches,
+                        const double* d_krep, int Qh, const float2* d_Bw, int ldB, int ksplit,
+                        double* d_gworld, const int* d_goff);
 
 }  // namespace pii

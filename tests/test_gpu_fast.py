@@ -1,0 +1,113 @@
+"""GPU parity of the fast path (FP32 transcendentals / FP64 phases form
+factor, split-precision MFMA Gram) against the FP64 oracle.
+
+Error measures (stated with each assertion):
+  form factors : ||A - A64||_w / ||A64||_w with ||x||_w^2 = sum_q w_q |x_q|^2
+                 (the norm that bounds volume errors), gate 1e-6;
+  volumes      : max_ij |V_ij - V64_ij| / sqrt(V64_ii V64_jj), gate 1e-5.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pointint_amd as P
+from pointint_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def wrel(a, b, w):
+    return np.sqrt((w * np.abs(a - b) ** 2).sum() / (w * np.abs(b) ** 2).sum())
+
+
+def vscaled(V, V64):
+    d = np.sqrt(np.abs(np.outer(np.diag(V64), np.diag(V64))))
+    return float(np.max(np.abs(V - V64) / d))
+
+
+@pytest.fixture(scope="module")
+def small():
+    z = np.load(os.path.join(GOLD, "small_mixed_L8_M38.npz"))
+    kg = P.kgrid_from_rule(9, 8)
+    geoms = [
+        P.make_triangle_mesh(z["ico_V"] * 0.6, z["ico_F"]),
+        P.make_triangle_mesh(z["ico_V"] * 0.5, z["ico_F"]),
+        P.make_point_cloud(z["P"] * 0.4, z["N"], z["r"] * 0.4),
+        P.make_gaussian_splat(z["P"] * 0.3, z["N"], z["sig"], z["wt"]),
+    ]
+    dofs = [z["dofs0"], z["dofs1"], z["dofs2"], z["dofs3"]]
+    kinds = ["affine", "trimesh", "affine", "affine"]
+    return z, kg, geoms, dofs, kinds
+
+
+def test_grid_symmetry_detected(small):
+    _, kg, *_ = small
+    assert kg.symmetric and kg.Qh * 2 == kg.Q
+
+
+def test_form_factor_fast_matches_golden(small):
+    z, kg, geoms, dofs, kinds = small
+    for i in range(4):
+        A = P.compute_form_factor(geoms[i], kg, dofs[i], kinds[i], precision="fast")
+        e = wrel(A, z["J"][:, i], kg.w)
+        assert e < 1e-6, (i, e)
+
+
+def test_volume_matrix_fast_matches_golden(small):
+    z, kg, geoms, dofs, kinds = small
+    for gram in ("fp64", "bf16x3", "fp16x3"):
+        V = P.compute_volume_matrix(geoms, kg, dofs, kinds, ff="fast", gram=gram)
+        assert np.array_equal(V, V.T)
+        assert vscaled(V, z["Vmat"]) < 1e-5, gram
+
+
+@pytest.fixture(scope="module")
+def ico_case(oracle):
+    """12 icospheres (level 3) on Lebedev 302 x 24 -- large k-range, real meshes."""
+    wl = W.config_icospheres(n_obj=12, level=3, seed=9, centre_box=1.0)
+    d, w = oracle.load_lebedev(P.lebedev_path(29))
+    dirs, kmag, wq = oracle.build_kgrid(d, w, 24)
+    Vo, Jo = oracle.volume_matrix([oracle.geom_mesh(o.V, o.F) for o in wl.objects], dirs, kmag, wq)
+    kg = P.KGrid(dirs, kmag, wq)
+    geoms = [P.make_triangle_mesh(o.V, o.F) for o in wl.objects]
+    return wl, kg, geoms, Vo, Jo
+
+
+def test_icosphere_form_factor_fast(ico_case):
+    wl, kg, geoms, Vo, Jo = ico_case
+    worst = 0.0
+    for i in (0, 5, 11):
+        A = P.compute_form_factor(geoms[i], kg, precision="fast")
+        worst = max(worst, wrel(A, Jo[:, i], kg.w))
+    assert worst < 1e-6, worst
+
+
+@pytest.mark.parametrize("gram", ["fp64", "bf16x3", "fp16x3"])
+def test_icosphere_volume_matrix_fast(ico_case, gram):
+    wl, kg, geoms, Vo, Jo = ico_case
+    V = P.compute_volume_matrix(geoms, kg, ff="fast", gram=gram)
+    assert vscaled(V, Vo) < 1e-5, (gram, vscaled(V, Vo))
+
+
+def test_split_gram_against_fp64_gram_same_inputs(ico_case):
+    """Isolates the Gram: both Grams consume the same fast-path X."""
+    wl, kg, geoms, Vo, Jo = ico_case
+    V64 = P.compute_volume_matrix(geoms, kg, ff="fast", gram="fp64")
+    for gram, tol in (("bf16x3", 1e-5), ("fp16x3", 2e-6)):
+        V = P.compute_volume_matrix(geoms, kg, ff="fast", gram=gram)
+        assert vscaled(V, V64) < tol, (gram, vscaled(V, V64))
+
+
+def test_asymmetric_grid_uses_full_grid(oracle):
+    d, w = oracle.load_lebedev(P.lebedev_path(9))
+    keep = np.arange(0, len(w), 2)[:-1]       # drops antipodes
+    dirs, kmag, wq = oracle.build_kgrid(d[keep], w[keep], 6)
+    kg = P.KGrid(dirs, kmag, wq)
+    assert not kg.symmetric and kg.Qh == kg.Q
+    Vi, Fi = W.icosphere(2)
+    g = P.make_triangle_mesh(Vi * 0.4 + 0.1, Fi)
+    A = P.compute_form_factor(g, kg, precision="fast")
+    ref = oracle.compute_A_grid(oracle.geom_mesh(Vi * 0.4 + 0.1, Fi), dirs, kmag)
+    assert wrel(A, ref, wq) < 1e-6
