@@ -113,10 +113,10 @@ void validate_object(const GeometryDev& g, int dof_kind, int ndofs) {
   }
 }
 
-void Batch::build(Context& c, int n, const GeometryDev* const* geoms, const int* kinds,
-                  const double* const* dofs, const int* ndofs, bool fast) {
+void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
+                  const GeometryDev* const* geoms, const int* kinds, const double* const* dofs,
+                  const int* ndofs, bool fast) {
   N = n;
-  hipStream_t s = c.stream;
   // Largest objects first: the dispatcher then starts the longest work early.
   order.resize(N);
   std::iota(order.begin(), order.end(), 0);
@@ -133,7 +133,7 @@ void Batch::build(Context& c, int n, const GeometryDev* const* geoms, const int*
   h_dofs.assign(total_dofs, 0.0);
   for (int i = 0; i < N; ++i)
     std::memcpy(h_dofs.data() + dof_off[i], dofs[i], sizeof(double) * ndofs[i]);
-  upload(c.ws_dofs, h_dofs.data(), total_dofs, s);
+  upload(w.dofs, h_dofs.data(), total_dofs, s);
   // World vertices and element records.
   std::vector<size_t> wv_off(N), rec_off(N);
   size_t total_w = 0, total_rec = 0;
@@ -151,8 +151,8 @@ void Batch::build(Context& c, int n, const GeometryDev* const* geoms, const int*
                                                   : kRecSurfel;
     total_rec += align_up((size_t)g.n * per, 32);
   }
-  c.ws_world.ensure(std::max<size_t>(total_w, 1) * sizeof(double));
-  c.ws_faces.ensure(std::max<size_t>(total_rec, 1) * sizeof(double));
+  w.world.ensure(std::max<size_t>(total_w, 1) * sizeof(double));
+  w.faces.ensure(std::max<size_t>(total_rec, 1) * sizeof(double));
   h_objs.resize(N);
   for (int slot = 0; slot < N; ++slot) {
     const int i = order[slot];
@@ -173,9 +173,9 @@ void Batch::build(Context& c, int n, const GeometryDev* const* geoms, const int*
     d.patch_faces = g.patch_faces.as<uint32_t>();
     d.patch_floc = g.patch_floc.as<uint32_t>();
     d.el = g.el.as<double>();
-    d.dofs = c.ws_dofs.as<double>() + dof_off[i];
-    d.world = c.ws_world.as<double>() + wv_off[slot];
-    d.rec = c.ws_faces.as<double>() + rec_off[slot];
+    d.dofs = w.dofs.as<double>() + dof_off[i];
+    d.world = w.world.as<double>() + wv_off[slot];
+    d.rec = w.faces.as<double>() + rec_off[slot];
   }
   // Descriptor table + prefix sums in one upload.
   const size_t obj_bytes = align_up(sizeof(ObjDesc) * N, 256);
@@ -184,15 +184,15 @@ void Batch::build(Context& c, int n, const GeometryDev* const* geoms, const int*
   std::memcpy(h_blob.data(), h_objs.data(), sizeof(ObjDesc) * N);
   std::memcpy(h_blob.data() + obj_bytes, h_vprefix.data(), sizeof(int) * (N + 1));
   std::memcpy(h_blob.data() + obj_bytes + pre_bytes, h_eprefix.data(), sizeof(int) * (N + 1));
-  upload(c.ws_misc, h_blob.data(), h_blob.size(), s);
-  d_objs = reinterpret_cast<const ObjDesc*>(c.ws_misc.p);
-  d_vprefix = reinterpret_cast<const int*>(static_cast<char*>(c.ws_misc.p) + obj_bytes);
-  d_eprefix = reinterpret_cast<const int*>(static_cast<char*>(c.ws_misc.p) + obj_bytes + pre_bytes);
+  upload(w.misc, h_blob.data(), h_blob.size(), s);
+  d_objs = reinterpret_cast<const ObjDesc*>(w.misc.p);
+  d_vprefix = reinterpret_cast<const int*>(static_cast<char*>(w.misc.p) + obj_bytes);
+  d_eprefix = reinterpret_cast<const int*>(static_cast<char*>(w.misc.p) + obj_bytes + pre_bytes);
   total_v = h_vprefix[N];
   total_e = h_eprefix[N];
-  c.phase_begin(0);
+  if (prof) prof->phase_begin(0);
   launch_prepass(s, d_objs, d_vprefix, total_v, d_eprefix, total_e, N, fast ? 1 : 0);
-  c.phase_end(0);
+  if (prof) prof->phase_end(0);
   PII_HIP(hipGetLastError());
 }
 
@@ -201,20 +201,20 @@ void volume_matrix_fp64(Context& c, const Batch& b, const KGridDev& kg, double* 
   hipStream_t s = c.stream;
   const int N = b.N, Q = kg.Q;
   const int ldJ = (int)align_up((size_t)Q, 64);
-  c.ws_J.ensure(sizeof(double2) * (size_t)ldJ * N);
+  c.w.J.ensure(sizeof(double2) * (size_t)ldJ * N);
   c.phase_begin(1);
-  launch_ff_fp64(s, b.d_objs, N, kg.full.as<double>(), Q, c.ws_J.as<double2>(), ldJ);
+  launch_ff_fp64(s, b.d_objs, N, kg.full.as<double>(), Q, c.w.J.as<double2>(), ldJ);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   const int nsplit = gram_fp64_splits(Q, N);
-  c.ws_part.ensure(sizeof(double) * (size_t)nsplit * N * N);
-  c.ws_V.ensure(sizeof(double) * (size_t)N * N);
+  c.w.part.ensure(sizeof(double) * (size_t)nsplit * N * N);
+  c.w.V.ensure(sizeof(double) * (size_t)N * N);
   c.phase_begin(2);
-  launch_gram_fp64(s, c.ws_J.as<double2>(), Q, N, ldJ, kg.full.as<double>() + 7 * (size_t)Q,
-                   c.ws_part.as<double>(), nsplit, c.ws_V.as<double>());
+  launch_gram_fp64(s, c.w.J.as<double2>(), Q, N, ldJ, kg.full.as<double>() + 7 * (size_t)Q,
+                   c.w.part.as<double>(), nsplit, c.w.V.as<double>());
   c.phase_end(2);
   PII_HIP(hipGetLastError());
-  PII_HIP(hipMemcpyAsync(V_host, c.ws_V.p, sizeof(double) * (size_t)N * N,
+  PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N,
                          hipMemcpyDeviceToHost, s));
   PII_HIP(hipStreamSynchronize(s));
 }
@@ -517,11 +517,11 @@ int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const
       b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
       const int Q = kg->k.Q;
       const int ldJ = (int)align_up((size_t)Q, 64);
-      ctx->c.ws_J.ensure(sizeof(double2) * (size_t)ldJ);
+      ctx->c.w.J.ensure(sizeof(double2) * (size_t)ldJ);
       launch_ff_fp64(ctx->c.stream, b.d_objs, 1, kg->k.full.as<double>(), Q,
-                     ctx->c.ws_J.as<double2>(), ldJ);
+                     ctx->c.w.J.as<double2>(), ldJ);
       PII_HIP(hipGetLastError());
-      PII_HIP(hipMemcpyAsync(out, ctx->c.ws_J.p, sizeof(double2) * Q, hipMemcpyDeviceToHost,
+      PII_HIP(hipMemcpyAsync(out, ctx->c.w.J.p, sizeof(double2) * Q, hipMemcpyDeviceToHost,
                              ctx->c.stream));
       PII_HIP(hipStreamSynchronize(ctx->c.stream));
     } else if (precision == PII_FF_FAST) {
@@ -587,15 +587,15 @@ int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int 
     PII_HIP(hipSetDevice(ctx->c.device));
     const int Q = kg->k.Q;
     const int ldJ = Q;
-    upload(ctx->c.ws_J, reinterpret_cast<const double2*>(J), (size_t)Q * n, ctx->c.stream);
+    upload(ctx->c.w.J, reinterpret_cast<const double2*>(J), (size_t)Q * n, ctx->c.stream);
     const int nsplit = gram_fp64_splits(Q, n);
-    ctx->c.ws_part.ensure(sizeof(double) * (size_t)nsplit * n * n);
-    ctx->c.ws_V.ensure(sizeof(double) * (size_t)n * n);
-    launch_gram_fp64(ctx->c.stream, ctx->c.ws_J.as<double2>(), Q, n, ldJ,
-                     kg->k.full.as<double>() + 7 * (size_t)Q, ctx->c.ws_part.as<double>(), nsplit,
-                     ctx->c.ws_V.as<double>());
+    ctx->c.w.part.ensure(sizeof(double) * (size_t)nsplit * n * n);
+    ctx->c.w.V.ensure(sizeof(double) * (size_t)n * n);
+    launch_gram_fp64(ctx->c.stream, ctx->c.w.J.as<double2>(), Q, n, ldJ,
+                     kg->k.full.as<double>() + 7 * (size_t)Q, ctx->c.w.part.as<double>(), nsplit,
+                     ctx->c.w.V.as<double>());
     PII_HIP(hipGetLastError());
-    PII_HIP(hipMemcpyAsync(V, ctx->c.ws_V.p, sizeof(double) * (size_t)n * n,
+    PII_HIP(hipMemcpyAsync(V, ctx->c.w.V.p, sizeof(double) * (size_t)n * n,
                            hipMemcpyDeviceToHost, ctx->c.stream));
     PII_HIP(hipStreamSynchronize(ctx->c.stream));
   });

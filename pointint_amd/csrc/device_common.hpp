@@ -39,6 +39,16 @@ constexpr int kRecPlainTri = 12;
 constexpr int kRecFastTri = 8;
 constexpr int kRecSurfel = 8;
 
+// Read-only data that a kernel indexes with wave-uniform indices (element
+// records, world vertices, patch tables) is read through the constant address
+// space so that the loads become scalar (SMEM) loads.  Only valid for memory
+// the kernel itself does not write.
+#define PII_CONST_AS __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const PII_CONST_AS T* as_const(const T* p) {
+  return (const PII_CONST_AS T*)(p);
+}
+
 struct cplx {
   double x, y;
 };

@@ -10,17 +10,17 @@ void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, doubl
   hipStream_t s = c.stream;
   const int N = b.N, Qh = kg.Qh, Q = kg.Q;
   const int ldA = (int)align_up((size_t)Qh, 64);
-  c.ws_X.ensure(sizeof(double2) * (size_t)ldA * N);
-  c.ws_J.ensure(sizeof(double2) * (size_t)Q * N);
+  c.w.X.ensure(sizeof(double2) * (size_t)ldA * N);
+  c.w.J.ensure(sizeof(double2) * (size_t)Q * N);
   c.phase_begin(1);
-  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 0, c.ws_X.as<double2>(), ldA, nullptr,
+  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 0, c.w.X.as<double2>(), ldA, nullptr,
                  0, nullptr);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
-  launch_expand(s, c.ws_X.as<double2>(), ldA, kg.rep_index.as<int>(), kg.anti_index.as<int>(), Qh,
-                Q, N, c.ws_J.as<double2>());
+  launch_expand(s, c.w.X.as<double2>(), ldA, kg.rep_index.as<int>(), kg.anti_index.as<int>(), Qh,
+                Q, N, c.w.J.as<double2>());
   PII_HIP(hipGetLastError());
-  PII_HIP(hipMemcpyAsync(out_host, c.ws_J.p, sizeof(double2) * (size_t)Q * N,
+  PII_HIP(hipMemcpyAsync(out_host, c.w.J.p, sizeof(double2) * (size_t)Q * N,
                          hipMemcpyDeviceToHost, s));
   PII_HIP(hipStreamSynchronize(s));
 }
@@ -30,34 +30,34 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
   hipStream_t s = c.stream;
   const int N = b.N, Qh = kg.Qh;
   const int ldX = (int)align_up((size_t)Qh, 64);
-  c.ws_X.ensure(sizeof(float2) * (size_t)ldX * N);
-  c.ws_cot.ensure(sizeof(unsigned) * (size_t)N);
-  unsigned* colmax = c.ws_cot.as<unsigned>();
+  c.w.X.ensure(sizeof(float2) * (size_t)ldX * N);
+  c.w.colmax.ensure(sizeof(unsigned) * (size_t)N);
+  unsigned* colmax = c.w.colmax.as<unsigned>();
   PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s));
   c.phase_begin(1);
-  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 1, nullptr, 0, c.ws_X.as<float2>(), ldX,
+  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 1, nullptr, 0, c.w.X.as<float2>(), ldX,
                  colmax);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
-  c.ws_V.ensure(sizeof(double) * (size_t)N * N);
+  c.w.V.ensure(sizeof(double) * (size_t)N * N);
   c.phase_begin(2);
   if (gram_mode == PII_GRAM_FP64) {
     const int nsplit = gram_fp64_splits(Qh, N);
-    c.ws_part.ensure(sizeof(double) * (size_t)nsplit * N * N);
-    launch_gram_fp64_x(s, c.ws_X.as<float2>(), Qh, N, ldX, c.ws_part.as<double>(), nsplit,
-                       c.ws_V.as<double>());
+    c.w.part.ensure(sizeof(double) * (size_t)nsplit * N * N);
+    launch_gram_fp64_x(s, c.w.X.as<float2>(), Qh, N, ldX, c.w.part.as<double>(), nsplit,
+                       c.w.V.as<double>());
   } else if (gram_mode == PII_GRAM_BF16X3 || gram_mode == PII_GRAM_FP16X3) {
     const int K = 2 * ldX;
     const int nsplit = gram_split_splits(K, N);
-    c.ws_part.ensure(sizeof(double) * (size_t)nsplit * N * N);
-    launch_gram_split(s, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.ws_X.as<float>(), N, K, colmax,
-                      c.ws_part.as<double>(), nsplit, c.ws_V.as<double>());
+    c.w.part.ensure(sizeof(double) * (size_t)nsplit * N * N);
+    launch_gram_split(s, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.X.as<float>(), N, K, colmax,
+                      c.w.part.as<double>(), nsplit, c.w.V.as<double>());
   } else {
     throw Error(PII_ERR_INVALID, "unknown Gram mode");
   }
   c.phase_end(2);
   PII_HIP(hipGetLastError());
-  PII_HIP(hipMemcpyAsync(V_host, c.ws_V.p, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost,
+  PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost,
                          s));
   PII_HIP(hipStreamSynchronize(s));
 }
@@ -71,23 +71,23 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
   for (int i = 0; i < N; ++i)
     if (geoms[i]->type != PII_GEOM_TRIANGLE)
       throw Error(PII_ERR_UNSUPPORTED, "adjoint supports triangle geometry only");
-  c.ws_X.ensure(sizeof(float2) * (size_t)ldX * N);
-  c.ws_B.ensure(sizeof(float2) * (size_t)ldX * N);
-  c.ws_cot.ensure(sizeof(unsigned) * (size_t)N);
-  unsigned* colmax = c.ws_cot.as<unsigned>();
+  c.w.X.ensure(sizeof(float2) * (size_t)ldX * N);
+  c.w.B.ensure(sizeof(float2) * (size_t)ldX * N);
+  c.w.colmax.ensure(sizeof(unsigned) * (size_t)N);
+  unsigned* colmax = c.w.colmax.as<unsigned>();
   PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s));
   c.phase_begin(1);
-  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 1, nullptr, 0, c.ws_X.as<float2>(), ldX,
+  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 1, nullptr, 0, c.w.X.as<float2>(), ldX,
                  colmax);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   if (V_host) {
     const int nsplit = gram_fp64_splits(Qh, N);
-    c.ws_part.ensure(sizeof(double) * (size_t)nsplit * N * N);
-    c.ws_V.ensure(sizeof(double) * (size_t)N * N);
+    c.w.part.ensure(sizeof(double) * (size_t)nsplit * N * N);
+    c.w.V.ensure(sizeof(double) * (size_t)N * N);
     c.phase_begin(2);
-    launch_gram_fp64_x(s, c.ws_X.as<float2>(), Qh, N, ldX, c.ws_part.as<double>(), nsplit,
-                       c.ws_V.as<double>());
+    launch_gram_fp64_x(s, c.w.X.as<float2>(), Qh, N, ldX, c.w.part.as<double>(), nsplit,
+                       c.w.V.as<double>());
     c.phase_end(2);
     PII_HIP(hipGetLastError());
   }
@@ -95,8 +95,8 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
   std::vector<double> C((size_t)N * N);
   for (int i = 0; i < N; ++i)
     for (int j = 0; j < N; ++j) C[(size_t)i * N + j] = G[(size_t)i * N + j] + G[(size_t)j * N + i];
-  c.ws_C.ensure(sizeof(double) * C.size());
-  PII_HIP(hipMemcpyAsync(c.ws_C.p, C.data(), sizeof(double) * C.size(), hipMemcpyHostToDevice, s));
+  c.w.C.ensure(sizeof(double) * C.size());
+  PII_HIP(hipMemcpyAsync(c.w.C.p, C.data(), sizeof(double) * C.size(), hipMemcpyHostToDevice, s));
   // World-vertex gradient slices per slot.
   std::vector<int> goff(N + 1, 0);
   int max_patches = 0;
@@ -104,25 +104,25 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
     goff[slot + 1] = goff[slot] + 3 * b.h_objs[slot].nv;
     max_patches = std::max(max_patches, b.h_objs[slot].npatch);
   }
-  c.ws_gw.ensure(sizeof(double) * std::max(goff[N], 1) + sizeof(int) * (N + 1) + 256);
-  double* d_gw = c.ws_gw.as<double>();
-  int* d_goff = reinterpret_cast<int*>(reinterpret_cast<char*>(c.ws_gw.p) +
+  c.w.gw.ensure(sizeof(double) * std::max(goff[N], 1) + sizeof(int) * (N + 1) + 256);
+  double* d_gw = c.w.gw.as<double>();
+  int* d_goff = reinterpret_cast<int*>(reinterpret_cast<char*>(c.w.gw.p) +
                                        align_up(sizeof(double) * std::max(goff[N], 1), 256));
   PII_HIP(hipMemcpyAsync(d_goff, goff.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, s));
   PII_HIP(hipMemsetAsync(d_gw, 0, sizeof(double) * std::max(goff[N], 1), s));
   c.phase_begin(3);
-  launch_mix_columns(s, c.ws_X.as<float2>(), N, ldX, Qh, c.ws_C.as<double>(), c.ws_B.as<float2>());
+  launch_mix_columns(s, c.w.X.as<float2>(), N, ldX, Qh, c.w.C.as<double>(), c.w.B.as<float2>());
   // enough workgroups to fill the chip: split the k-range when objects are few
   const int wgs = std::max(1, max_patches * N);
   const int ksplit = std::max(1, std::min(64, (4096 + wgs - 1) / wgs));
-  launch_adjoint_tri(s, b.d_objs, N, max_patches, kg.rep.as<double>(), Qh, c.ws_B.as<float2>(),
+  launch_adjoint_tri(s, b.d_objs, N, max_patches, kg.rep.as<double>(), Qh, c.w.B.as<float2>(),
                      ldX, ksplit, d_gw, d_goff);
   c.phase_end(3);
   PII_HIP(hipGetLastError());
   std::vector<double> gw(std::max(goff[N], 1));
   PII_HIP(hipMemcpyAsync(gw.data(), d_gw, sizeof(double) * goff[N], hipMemcpyDeviceToHost, s));
   if (V_host)
-    PII_HIP(hipMemcpyAsync(V_host, c.ws_V.p, sizeof(double) * (size_t)N * N,
+    PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N,
                            hipMemcpyDeviceToHost, s));
   PII_HIP(hipStreamSynchronize(s));
   // World-vertex gradients -> DoF gradients.

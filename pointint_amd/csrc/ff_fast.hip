@@ -83,34 +83,32 @@ __device__ __forceinline__ float sinc_from(float h, float sin_h) {
   return sin_h * __builtin_amdgcn_rcpf(h);
 }
 
-// gamma * e^{i u0} * Phi(alpha, beta), from the three half phasors.
-__device__ __forceinline__ float2 tri_term(double al, double be, float g2, float2 Q0, float2 Q1,
-                                           float2 Q2) {
-  const double lo = fmin(0.0, fmin(al, be));
-  const double hi = fmax(0.0, fmax(al, be));
-  const double H = hi - lo;
-  if (H < kSpanTaylor) {
-    // Phi(a, b) = sum_n c_n h_n(a, b), c_n = 2 i^n / (n+2)!, in FP64.
-    double an = 1.0, hn = 1.0;
-    double re = 1.0, im = 0.0;
-    an *= al; hn = fma(be, hn, an); im += hn * (1.0 / 3.0);
-    an *= al; hn = fma(be, hn, an); re -= hn * (1.0 / 12.0);
-    an *= al; hn = fma(be, hn, an); im -= hn * (1.0 / 60.0);
-    an *= al; hn = fma(be, hn, an); re += hn * (1.0 / 360.0);
-    an *= al; hn = fma(be, hn, an); im += hn * (1.0 / 2520.0);
-    an *= al; hn = fma(be, hn, an); re -= hn * (1.0 / 20160.0);
-    an *= al; hn = fma(be, hn, an); im -= hn * (1.0 / 181440.0);
-    an *= al; hn = fma(be, hn, an); re += hn * (1.0 / 1814400.0);
-    an *= al; hn = fma(be, hn, an); im += hn * (1.0 / 19958400.0);
-    an *= al; hn = fma(be, hn, an); re -= hn * (1.0 / 239500800.0);
-    an *= al; hn = fma(be, hn, an); im -= hn * (1.0 / 3113510400.0);
-    // e^{i u0} = Q0^2; contribution = (g2/2) e^{i u0} Phi
-    const float px = Q0.x * Q0.x - Q0.y * Q0.y;
-    const float py = 2.0f * Q0.x * Q0.y;
-    const float tr = (float)re * (0.5f * g2), ti = (float)im * (0.5f * g2);
-    return make_float2(px * tr - py * ti, px * ti + py * tr);
+// Phi(a, b) = sum_{n<NT} c_n h_n(a, b), c_n = 2 i^n / (n+2)!, in FP64, times
+// (g2/2) e^{i u0} with e^{i u0} = Q0^2.  Truncation of the first omitted term
+// for span H: NT = 6: 2(NT+1)H^NT/(NT+2)! < 4e-10 at H = 0.1; NT = 8 at
+// H = 0.25 and NT = 9 at H = 0.5 stay below 1e-9; NT = 12 below 3e-10 at H = 1.
+template <int NT>
+__device__ __forceinline__ float2 tri_taylor(double al, double be, float g2, float2 Q0) {
+  constexpr double c[12] = {1.0, 1.0 / 3.0, -1.0 / 12.0, -1.0 / 60.0, 1.0 / 360.0,
+                            1.0 / 2520.0, -1.0 / 20160.0, -1.0 / 181440.0, 1.0 / 1814400.0,
+                            1.0 / 19958400.0, -1.0 / 239500800.0, -1.0 / 3113510400.0};
+  double an = 1.0, hn = 1.0, re = 1.0, im = 0.0;
+#pragma unroll
+  for (int n = 1; n < NT; ++n) {
+    an *= al;
+    hn = fma(be, hn, an);
+    if (n & 1) im = fma(hn, c[n], im);
+    else re = fma(hn, c[n], re);
   }
-  // Middle node m and the other two (p, q); see the file comment.
+  const float px = Q0.x * Q0.x - Q0.y * Q0.y;
+  const float py = 2.0f * Q0.x * Q0.y;
+  const float tr = (float)re * (0.5f * g2), ti = (float)im * (0.5f * g2);
+  return make_float2(px * tr - py * ti, px * ti + py * tr);
+}
+
+// Main branch: middle node m and the other two (p, q); see the file comment.
+__device__ __forceinline__ float2 tri_main(double al, double be, float g2, float2 Q0, float2 Q1,
+                                           float2 Q2) {
   const bool m0 = (al * be <= 0.0);
   const bool m1 = !m0 && (fabs(al) < fabs(be));
   float2 Qm, Qp, Qq;
@@ -133,6 +131,14 @@ __device__ __forceinline__ float2 tri_term(double al, double be, float g2, float
   const float f = g2 * __builtin_amdgcn_rcpf((float)D);
   // -i * (tx + i ty) = ty - i tx
   return make_float2(ty * f, -tx * f);
+}
+
+// gamma * e^{i u0} * Phi(alpha, beta) with a per-lane branch on the span.
+__device__ __forceinline__ float2 tri_term(double al, double be, float g2, float2 Q0, float2 Q1,
+                                           float2 Q2) {
+  const double H = fmax(0.0, fmax(al, be)) - fmin(0.0, fmin(al, be));
+  if (H < kSpanTaylor) return tri_taylor<12>(al, be, g2, Q0);
+  return tri_main(al, be, g2, Q0, Q1, Q2);
 }
 
 // J1(x)/x for x >= 0 in FP64: series below 12, Hankel asymptotics above
@@ -173,7 +179,9 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
   double2 (*red)[kLanes] = reinterpret_cast<double2 (*)[kLanes]>(&tab[0][0]);
   const ObjDesc& d = objs[blockIdx.y];
   const int lane = threadIdx.x & (kLanes - 1);
-  const int wave = threadIdx.x >> 6;
+  // wave index as an SGPR value: loop indices derived from it stay uniform,
+  // so element records are fetched with scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int q = blockIdx.x * kLanes + lane;
   const int qc = q < Qh ? q : Qh - 1;
   const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
@@ -183,28 +191,55 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
   double accx = 0.0, accy = 0.0;
 
   if (d.type == PII_GEOM_TRIANGLE) {
+    // Span bound per face: H <= |k| * 2^code (code in the record's top byte).
+    // cut[j] = largest code with kmax * 2^code below the Taylor thresholds
+    // 0.1, 0.25, 0.5, 1 for the 64 nodes of this workgroup.
+    double kmax = km;
+    for (int off = 32; off > 0; off >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, off));
+    int cut[4];
+    const double thr[4] = {0.1, 0.25, 0.5, 1.0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = (kmax > 0.0) ? (int)floor(log2(thr[j] / kmax)) - 1 : 127;
+      cut[j] = __builtin_amdgcn_readfirstlane(min(127, max(-128, c)) + 64);
+    }
+    const auto* hdr = as_const(d.patch_hdr);
+    const auto* pverts = as_const(d.patch_verts);
+    const auto* world = as_const(d.world);
+    const auto* rec = as_const(d.rec);
     for (int p = 0; p < d.npatch; ++p) {
-      const int vbeg = d.patch_hdr[4 * p], vcnt = d.patch_hdr[4 * p + 1];
-      const int fbeg = d.patch_hdr[4 * p + 2], fcnt = d.patch_hdr[4 * p + 3];
+      const int vbeg = hdr[4 * p], vcnt = hdr[4 * p + 1];
+      const int fbeg = hdr[4 * p + 2], fcnt = hdr[4 * p + 3];
       for (int s = wave; s < vcnt; s += kWaves) {
-        const int gv = d.patch_verts[vbeg + s];
-        const double* W = d.world + 3 * (size_t)gv;
-        const double u = kx * W[0] + ky * W[1] + kz * W[2];
+        const int gv = pverts[vbeg + s];
+        const double u = kx * world[3 * gv] + ky * world[3 * gv + 1] + kz * world[3 * gv + 2];
         tab[s][lane] = half_phasor(u);
       }
       __syncthreads();
       float px = 0.f, py = 0.f;
       for (int f = wave; f < fcnt; f += kWaves) {
-        const double* r = d.rec + (size_t)kRecFastTri * (fbeg + f);
+        const auto* r = rec + (size_t)kRecFastTri * (fbeg + f);
         const double al = kx * r[0] + ky * r[1] + kz * r[2];
         const double be = kx * r[3] + ky * r[4] + kz * r[5];
-        const float* rf = reinterpret_cast<const float*>(r + 6);
+        const auto* rf = reinterpret_cast<const PII_CONST_AS float*>(r + 6);
         const float g2 = hx * rf[0] + hy * rf[1] + hz * rf[2];
-        const uint32_t loc = reinterpret_cast<const uint32_t*>(rf)[3];
+        const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
+        const int code = (int)(loc >> 24);
         const float2 Q0 = tab[loc & 0xff][lane];
-        const float2 Q1 = tab[(loc >> 8) & 0xff][lane];
-        const float2 Q2 = tab[(loc >> 16) & 0xff][lane];
-        const float2 t = tri_term(al, be, g2, Q0, Q1, Q2);
+        float2 t;
+        if (code <= cut[0]) {
+          t = tri_taylor<6>(al, be, g2, Q0);
+        } else if (code <= cut[1]) {
+          t = tri_taylor<8>(al, be, g2, Q0);
+        } else if (code <= cut[2]) {
+          t = tri_taylor<9>(al, be, g2, Q0);
+        } else if (code <= cut[3]) {
+          t = tri_taylor<12>(al, be, g2, Q0);
+        } else {
+          const float2 Q1 = tab[(loc >> 8) & 0xff][lane];
+          const float2 Q2 = tab[(loc >> 16) & 0xff][lane];
+          t = tri_term(al, be, g2, Q0, Q1, Q2);
+        }
         px += t.x;
         py += t.y;
       }
@@ -215,7 +250,7 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
   } else {
     // Disks / Gaussians: A = S(|k'_perp|) (k'.n)/|k| e^{i k.c'}, k' = A^T k,
     // with c' and A n precomputed (cpp/src/dof/affine_dof.cpp:103-167).
-    const double* p = d.dofs;
+    const auto* p = as_const(d.dofs);
     const double kpx = p[3] * kx + p[6] * ky + p[9] * kz;
     const double kpy = p[4] * kx + p[7] * ky + p[10] * kz;
     const double kpz = p[5] * kx + p[8] * ky + p[11] * kz;
@@ -224,8 +259,9 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
     const bool disk = (d.type == PII_GEOM_DISK);
     float px = 0.f, py = 0.f;
     int cnt = 0;
+    const auto* rec = as_const(d.rec);
     for (int e = wave; e < d.n; e += kWaves) {
-      const double* r = d.rec + (size_t)kRecSurfel * e;
+      const auto* r = rec + (size_t)kRecSurfel * e;
       const double kdn = kx * r[3] + ky * r[4] + kz * r[5];
       double S;
       if (disk) {

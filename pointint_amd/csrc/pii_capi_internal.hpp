@@ -25,8 +25,13 @@ struct Batch {
   const int* d_vprefix = nullptr;
   const int* d_eprefix = nullptr;
   int total_v = 0, total_e = 0;
+  // prof (may be null) records the pre-pass phase with HIP events.
+  void build(Workspace& w, hipStream_t s, Context* prof, int n, const GeometryDev* const* geoms,
+             const int* kinds, const double* const* dofs, const int* ndofs, bool fast);
   void build(Context& c, int n, const GeometryDev* const* geoms, const int* kinds,
-             const double* const* dofs, const int* ndofs, bool fast);
+             const double* const* dofs, const int* ndofs, bool fast) {
+    build(c.w, c.stream, &c, n, geoms, kinds, dofs, ndofs, fast);
+  }
 };
 
 // Fast path (FP32 transcendentals, FP64 phases and Taylor branch).

@@ -117,13 +117,16 @@ struct GeometryDev {
   DevBuf el;
 };
 
+// Per-call device workspaces (grow-only, reused across calls on one stream).
+struct Workspace {
+  DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;                     // serialises calls on one context
-  // Reusable workspaces (grow-only).
-  DevBuf ws_world, ws_faces, ws_J, ws_X, ws_part, ws_V, ws_misc, ws_dofs, ws_cot, ws_grad;
-  DevBuf ws_B, ws_C, ws_gw;
+  Workspace w;
   // Phase timing with HIP events on the context stream (enable_profiling of
   // the reference drivers, e.g. compute_intersection_volume_multi_object.cu:626-695).
   bool profiling = false;
