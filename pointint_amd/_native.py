@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -70,11 +71,26 @@ class PiiError(RuntimeError):
         self.code = code
 
 
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm wheels bundle their own libamdhip64.so.7.  When torch is
+    installed it is imported before the library is opened, so the library's
+    NEEDED libamdhip64.so.7 resolves to the copy torch already loaded and the
+    process holds one HIP runtime (two runtimes in one process leave the second
+    one without devices)."""
+    if "torch" in sys.modules:
+        return
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def load(path: str = LIB_PATH):
     """Load libpointint_hip.so (raises if it is missing or incomplete)."""
     global _lib
     if _lib is not None:
         return _lib
+    _share_torch_hip_runtime()
     if not os.path.exists(path):
         raise ImportError(
             f"{path} is missing: build it with `python -m pointint_amd.build_lib` "

@@ -111,3 +111,24 @@ def test_asymmetric_grid_uses_full_grid(oracle):
     A = P.compute_form_factor(g, kg, precision="fast")
     ref = oracle.compute_A_grid(oracle.geom_mesh(Vi * 0.4 + 0.1, Fi), dirs, kmag)
     assert wrel(A, ref, wq) < 1e-6
+
+
+def test_full_size_grid_errors(oracle):
+    """Three objects of the 256-icosphere workload on its full k-grid
+    (Lebedev 1202 x L=64, k up to ~1.8e3): form factors vs the FP64 oracle
+    (weighted norm) and the 3 x 3 V (scaled), both Gram modes."""
+    wl = W.config_icospheres(n_obj=256, level=3, seed=2)
+    objs = [wl.objects[i] for i in (0, 101, 202)]
+    d, w = oracle.load_lebedev(P.lebedev_path(59))
+    dirs, kmag, wq = oracle.build_kgrid(d, w, 64)
+    Vo, Jo = oracle.volume_matrix([oracle.geom_mesh(o.V, o.F) for o in objs], dirs, kmag, wq)
+    kg = P.KGrid(dirs, kmag, wq)
+    geoms = [P.make_triangle_mesh(o.V, o.F) for o in objs]
+    errs = [wrel(P.compute_form_factor(g, kg, precision="fast"), Jo[:, i], wq)
+            for i, g in enumerate(geoms)]
+    print("form factor weighted rel errors", errs)
+    assert max(errs) < 1e-6
+    for gram in ("bf16x3", "fp16x3"):
+        V = P.compute_volume_matrix(geoms, kg, ff="fast", gram=gram)
+        print(gram, "scaled V error", vscaled(V, Vo))
+        assert vscaled(V, Vo) < 1e-5
