@@ -54,7 +54,7 @@ def cpu_baseline(wl, kg, target_seconds=15.0):
     pairs in parallel) on a k-node subsample, extrapolated linearly in Q."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    geoms = [O.geom_mesh(o.V, o.F) for o in wl.objects]
+    geoms = oracle_geoms(O, wl)
     Q = kg.Q
     # calibrate on a tiny sample, then size the sample to ~target_seconds
     stride = max(1, Q // 64)
@@ -73,6 +73,126 @@ def cpu_baseline(wl, kg, target_seconds=15.0):
     return t_full, len(idx), t_s, O.threads()
 
 
+def make_geoms(wl):
+    out = []
+    for o in wl.objects:
+        if isinstance(o, W.MeshObj):
+            out.append(P.make_triangle_mesh(o.V, o.F))
+        else:
+            out.append(P.make_point_cloud(o.P, o.N, o.r))
+    return out
+
+
+def oracle_geoms(O, wl):
+    out = []
+    for o in wl.objects:
+        if isinstance(o, W.MeshObj):
+            out.append(O.geom_mesh(o.V, o.F))
+        else:
+            out.append(O.geom_disks(o.P, o.N, o.r))
+    return out
+
+
+def run_diff(args):
+    """Forward V + hand-written adjoint of L = sum_{i<j} V_ij for 64 icospheres
+    (level 4, 5,120 triangles, vertex DoFs) on Lebedev 590 x L=64; the
+    gradient is checked against FP64 central differences (h = 1e-4) of the FP64
+    GPU forward on --fd-vertices random vertices."""
+    wl = W.config_diff()
+    ctx = P.Context.default()
+    kg = P.kgrid_from_rule(wl.lebedev_order, wl.n_radial)
+    geoms = [P.make_triangle_mesh(o.V, o.F) for o in wl.objects]
+    n = len(geoms)
+    xs = [o.V.astype(np.float32).astype(np.float64).reshape(-1) for o in wl.objects]
+    G = np.triu(np.ones((n, n)), 1)
+    kinds = ["trimesh"] * n
+    for _ in range(args.warmup):
+        P.volume_matrix_vjp(geoms, kg, G, xs, kinds)
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        grads, V = P.volume_matrix_vjp(geoms, kg, G, xs, kinds)
+    t1 = time.perf_counter()
+    tm = ctx.timings()
+    ctx.set_profiling(False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    d = np.sqrt(np.outer(np.diag(V), np.diag(V)))
+    frac_overlap = float((V[np.triu_indices(n, 1)] > 1e-3 * d[np.triu_indices(n, 1)]).mean())
+    # FD against the FP64 forward: only object i's column moves, so
+    # L(x_i) = Re<A_i, sum_{j != i} A_j>_w / (8 pi^3) + const.
+    J = np.stack([P.compute_form_factor(g, kg, x, "trimesh", precision="fp64")
+                  for g, x in zip(geoms, xs)], axis=1)
+    rng = np.random.default_rng(7)
+    h = 1e-4
+    errs = []
+    tot = J.sum(axis=1)
+    for _ in range(args.fd_vertices):
+        i = int(rng.integers(n))
+        v = int(rng.integers(xs[i].shape[0] // 3))
+        c = int(rng.integers(3))
+        B = tot - J[:, i]
+        vals = []
+        for sgn in (1, -1):
+            x = xs[i].copy()
+            x[3 * v + c] += sgn * h
+            Ai = P.compute_form_factor(geoms[i], kg, x, "trimesh", precision="fp64")
+            vals.append((kg.w * (Ai * np.conj(B)).real).sum() / (8 * np.pi ** 3))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        scale = np.abs(grads[i]).max()
+        errs.append(abs(grads[i][3 * v + c] - fd) / scale)
+    out = {
+        "metric": "adjoint: dLoss/dvertex evaluations (forward V + VJP) per second",
+        "value": 1e3 / ms, "unit": "gradient evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded icospheres)",
+        "config": {"workload": "64 icospheres L4 (5,120 tris, vertex DoFs), Lebedev 590 x L=64 "
+                               "(Q=37,760), loss sum_{i<j} V_ij", "objects": n, "Q": kg.Q},
+        "phase_ms": tm, "overlapping_pair_fraction": frac_overlap,
+        "fd_check": {"vertices": args.fd_vertices, "h": h,
+                     "max_err_rel_to_max_grad": float(max(errs)),
+                     "median_err_rel_to_max_grad": float(np.median(errs))},
+    }
+    print(json.dumps(out))
+
+
+def run_serving(args):
+    """Poisson query stream against a resident pool of star-shaped meshes."""
+    from pointint_amd import serving as S
+    pool = W.serving_pool(n_mesh=args.pool, seed=5)
+    print(f"[bench] pool of {len(pool)} meshes, {sum(m.n for m in pool)} triangles",
+          file=sys.stderr, flush=True)
+    ctx = P.Context.default()
+    kg = P.kgrid_from_rule(59, 64)
+    geoms = [P.make_triangle_mesh(m.V, m.F) for m in pool]
+    with S.Server(kg, gram=args.gram, hbm_budget=args.budget_gb * 1e9,
+                  max_batch_objects=args.max_batch) as srv:
+        # warm the engine (workspace growth) outside the measured stream
+        rng = np.random.default_rng(1)
+        q = srv.submit(geoms[:16], S.rigid_dofs(rng, 4.0, 16))
+        srv.wait(q, 16)
+        rep = S.poisson_load(srv, geoms, rate=args.rate, duration=args.duration, kmin=8,
+                             kmax=512, box=4.0, seed=11)
+        st = srv.stats()
+    s = rep.summary()
+    out = {
+        "metric": "serving: completed pairs/s and p50/p99 query latency under a Poisson stream",
+        "value": s["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": s["completed"],
+        "warmup": 1, "ms_per_step": 1e3 / s["completed_qps"] if s["completed_qps"] else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded star-shaped meshes, random rigid transforms)",
+        "config": {"workload": f"pool {len(pool)} meshes (T log-uniform [80, 20480]); "
+                               f"K log-uniform [8, 512]; Lebedev 1202 x L=64; Poisson "
+                               f"{args.rate:g} q/s for {args.duration:g} s",
+                   "gram": args.gram, "hbm_budget_gb": args.budget_gb,
+                   "max_batch_objects": args.max_batch},
+        "latency_ms": {"p50": s["p50_ms"], "p99": s["p99_ms"], "mean": s["mean_ms"]},
+        "queries": {k: s[k] for k in ("submitted", "admitted", "refused", "completed")},
+        "completed_qps": s["completed_qps"], "batches": st["batches"],
+        "objects_per_batch": st["objects"] / max(st["batches"], 1),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +204,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--n-obj", type=int, default=256)
+    ap.add_argument("--workload", default="icospheres",
+                    choices=["icospheres", "mixed", "diff", "serving"])
+    ap.add_argument("--fd-vertices", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=2000.0)
+    ap.add_argument("--duration", type=float, default=10.0)
+    ap.add_argument("--pool", type=int, default=4096)
+    ap.add_argument("--budget-gb", type=float, default=2.0)
+    ap.add_argument("--max-batch", type=int, default=1024)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -92,12 +220,21 @@ def main():
               "does not run.", file=sys.stderr)
         sys.exit(2)
 
-    wl = W.config_icospheres(n_obj=args.n_obj, level=3, seed=2)
+    if args.workload == "diff":
+        return run_diff(args)
+    if args.workload == "serving":
+        return run_serving(args)
+    if args.workload == "mixed":
+        wl = W.config_mixed()
+        if args.gram == "bf16x3" and "--gram" not in sys.argv:
+            args.gram = "fp16x3"
+    else:
+        wl = W.config_icospheres(n_obj=args.n_obj, level=3, seed=2)
     print(f"[bench] workload {wl.name}: {len(wl.objects)} objects, "
-          f"{wl.prim_count()} triangles", file=sys.stderr, flush=True)
+          f"{wl.prim_count()} elements", file=sys.stderr, flush=True)
     ctx = P.Context.default()
     kg = P.kgrid_from_rule(wl.lebedev_order, wl.n_radial)
-    geoms = [P.make_triangle_mesh(o.V, o.F) for o in wl.objects]
+    geoms = make_geoms(wl)
     N = len(geoms)
     pairs = N * (N - 1) // 2
     primk = wl.prim_count() * kg.Q
@@ -179,9 +316,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if args.ff == "fast" else "f64",
-        "data": "synthetic (seeded icospheres)",
-        "config": {"workload": "256 icospheres L3 (1,280 tris), Lebedev 1202 x L=64 "
-                               "(Q=76,928), FP32 form factors, BF16x3 Gram",
+        "data": "synthetic (seeded)",
+        "config": {"workload": ("256 icospheres L3 (1,280 tris), Lebedev 1202 x L=64 "
+                                "(Q=76,928), FP32 form factors, BF16x3 Gram")
+                               if args.workload == "icospheres" else
+                               ("1,024 mixed objects (512 star meshes T in [80, 20480], "
+                                "512 disk clouds in [1k, 50k]), Lebedev 2702 x L=96 "
+                                f"(Q=259,392), {args.gram} Gram"),
                    "objects": N, "triangles": wl.prim_count(), "Q": kg.Q,
                    "ff": args.ff, "gram": args.gram, "parallelism": "single GPU"},
         "prim_k_per_s": primk / (ms_step * 1e-3),

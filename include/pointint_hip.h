@@ -164,6 +164,28 @@ int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* ge
 int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int gram_mode,
              double* V);
 
+/* ---- serving ------------------------------------------------------------------
+ * No counterpart in the reference (its drivers are synchronous, one call at a
+ * time, with per-call device allocation).  A server keeps the k-grid and the
+ * geometries resident and answers volume-matrix queries -- a scene of n
+ * resident triangle geometries, each under its own 12 affine DoFs -- by
+ * continuous batching on its own HIP streams.  Queries are admitted against
+ * hbm_budget bytes (the reservation of a query lasts from admission to
+ * completion); pii_server_submit returns PII_ERR_BUDGET at once for a query
+ * that does not fit.  pii_server_wait blocks up to timeout_s seconds (< 0:
+ * no limit; PII_ERR_BUSY on timeout), copies V (n*n) and the latency
+ * (completion - submission, ms), and releases the query.
+ * pii_server_stats: submitted, admitted, refused, completed, batches,
+ * objects, reserved bytes, budget bytes. */
+typedef struct pii_server pii_server;
+int pii_server_create(pii_context* ctx, const pii_kgrid* kg, int gram_mode, double hbm_budget,
+                      int max_batch_objects, pii_server** out);
+int pii_server_destroy(pii_server* srv);
+int pii_server_submit(pii_server* srv, int n, const pii_geometry* const* geoms,
+                      const double* affine_dofs, int64_t* id);
+int pii_server_wait(pii_server* srv, int64_t id, double timeout_s, double* V, double* latency_ms);
+int pii_server_stats(pii_server* srv, double* out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,18 +16,6 @@
 #include "pii_internal.hpp"
 #include "pii_capi_internal.hpp"
 
-struct pii_context {
-  pii::Context c;
-};
-struct pii_kgrid {
-  pii::KGridDev k;
-  pii_context* ctx = nullptr;
-};
-struct pii_geometry {
-  pii::GeometryDev g;
-  pii_context* ctx = nullptr;
-};
-
 namespace pii {
 
 void DevBuf::release() {
@@ -64,24 +52,6 @@ void Context::phase_end(int p) {
   if (!profiling) return;
   PII_HIP(hipEventRecord(ev[p][1], stream));
   ev_used[p] = true;
-}
-
-template <class F>
-int guarded(F&& f) {
-  clear_error();
-  try {
-    f();
-    return PII_OK;
-  } catch (const Error& e) {
-    set_error(e.msg);
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    set_error("host allocation failed");
-    return PII_ERR_NOMEM;
-  } catch (const std::exception& e) {
-    set_error(e.what());
-    return PII_ERR_INVALID;
-  }
 }
 
 static void require(bool cond, const std::string& msg) {

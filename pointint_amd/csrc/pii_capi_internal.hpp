@@ -3,14 +3,48 @@
 #pragma once
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <vector>
 
 #include "device_common.hpp"
 #include "pii_internal.hpp"
 
+// Handles behind the opaque C types of pointint_hip.h.
+struct pii_context {
+  pii::Context c;
+};
+struct pii_kgrid {
+  pii::KGridDev k;
+  pii_context* ctx = nullptr;
+};
+struct pii_geometry {
+  pii::GeometryDev g;
+  pii_context* ctx = nullptr;
+};
+
 namespace pii {
 
 size_t align_up(size_t x, size_t a);
+
+// Runs f, mapping exceptions to status codes and the thread-local message.
+template <class F>
+int guarded(F&& f) {
+  clear_error();
+  try {
+    f();
+    return PII_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return PII_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return PII_ERR_INVALID;
+  }
+}
 
 // One call's objects on the device: DoFs uploaded, descriptors built, the
 // pre-pass (world vertices + element records) launched on the context stream.

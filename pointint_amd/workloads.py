@@ -82,30 +82,25 @@ def uv_sphere(n_tri_target: int):
     """UV tessellated unit sphere with about n_tri_target outward triangles."""
     nlat = max(3, int(round(math.sqrt(n_tri_target / 4.0))) + 1)
     nlon = max(3, int(round(n_tri_target / (2.0 * (nlat - 1)))))
-    verts = [[0.0, 0.0, 1.0]]
-    for i in range(1, nlat):
-        th = math.pi * i / nlat
-        for j in range(nlon):
-            ph = 2 * math.pi * j / nlon
-            verts.append([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
-    verts.append([0.0, 0.0, -1.0])
-    V = np.array(verts)
-    south = len(verts) - 1
+    th = math.pi * np.arange(1, nlat) / nlat
+    ph = 2 * math.pi * np.arange(nlon) / nlon
+    ring = np.stack([np.outer(np.sin(th), np.cos(ph)), np.outer(np.sin(th), np.sin(ph)),
+                     np.outer(np.cos(th), np.ones(nlon))], axis=-1).reshape(-1, 3)
+    V = np.concatenate([[[0.0, 0.0, 1.0]], ring, [[0.0, 0.0, -1.0]]])
+    south = V.shape[0] - 1
+    j = np.arange(nlon)
+    jn = (j + 1) % nlon
 
-    def idx(i, j):
-        return 1 + (i - 1) * nlon + (j % nlon)
+    def idx(i, jj):
+        return 1 + (i - 1) * nlon + jj
 
-    F = []
-    for j in range(nlon):
-        F.append([0, idx(1, j), idx(1, j + 1)])
-    for i in range(1, nlat - 1):
-        for j in range(nlon):
-            a, b, c, d = idx(i, j), idx(i, j + 1), idx(i + 1, j), idx(i + 1, j + 1)
-            F.append([a, c, b])
-            F.append([b, c, d])
-    for j in range(nlon):
-        F.append([south, idx(nlat - 1, j + 1), idx(nlat - 1, j)])
-    F = np.array(F, dtype=np.int32)
+    caps_n = np.stack([np.zeros(nlon, dtype=np.int64), idx(1, j), idx(1, jn)], axis=1)
+    i = np.arange(1, nlat - 1)[:, None]
+    a, b, c, d = idx(i, j[None, :]), idx(i, jn[None, :]), idx(i + 1, j[None, :]), idx(i + 1, jn[None, :])
+    body = np.concatenate([np.stack([a, c, b], axis=-1).reshape(-1, 3),
+                           np.stack([b, c, d], axis=-1).reshape(-1, 3)])
+    caps_s = np.stack([np.full(nlon, south), idx(nlat - 1, jn), idx(nlat - 1, j)], axis=1)
+    F = np.concatenate([caps_n, body, caps_s]).astype(np.int32)
     cen = V[F].mean(axis=1)
     nrm = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
     flip = (nrm * cen).sum(axis=1) < 0
