@@ -16,6 +16,10 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
 OUT = os.path.join(HERE, "libpointint_hip.so")
+CXX_DIR = os.path.join(HERE, "cxx")
+OUT_CXX = os.path.join(HERE, "libpointint_cxx.so")
+OUT_EXAMPLE = os.path.join(HERE, "example_cube_pair")
+ROCM_INCLUDE = "/opt/rocm/include"
 BUILD_DIR = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -56,9 +60,32 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_cxx()
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT}, {OUT_CXX}, {OUT_EXAMPLE}")
     return OUT
+
+
+def build_cxx() -> None:
+    """The reference-named C++ API (host code on the C ABI) and its example,
+    compiled with g++ and linked against libpointint_hip.so."""
+    cxx = os.environ.get("CXX", "g++")
+    common = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE,
+              "-I", ROCM_INCLUDE, "-Wall"]
+    tmp = OUT_CXX + ".tmp"
+    cmd = [cxx, *common, "-shared", "-o", tmp, os.path.join(CXX_DIR, "pointint_cxx.cpp"),
+           "-L", HERE, "-lpointint_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ API build failed: {' '.join(cmd)}\n{r.stderr}")
+    os.replace(tmp, OUT_CXX)
+    tmp = OUT_EXAMPLE + ".tmp"
+    cmd = [cxx, *common, "-o", tmp, os.path.join(CXX_DIR, "example_cube_pair.cpp"),
+           "-L", HERE, "-lpointint_cxx", "-lpointint_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ example build failed: {' '.join(cmd)}\n{r.stderr}")
+    os.replace(tmp, OUT_EXAMPLE)
 
 
 if __name__ == "__main__":

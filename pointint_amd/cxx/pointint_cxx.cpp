@@ -1,0 +1,456 @@
+// C++ API with the reference's names (include/pointint/pointint.hpp),
+// implemented on the C ABI of libpointint_hip.so.  Host code only: no HIP
+// calls here, every device operation goes through pii_* entry points.
+//
+// Error behaviour follows the reference drivers
+// (cpp/src/compute_intersection_volume_multi_object.cu:250-314): a message
+// on std::cerr and a zero (or empty) result, no exceptions.
+#include "pointint/pointint.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <mutex>
+#include <stdexcept>
+
+#include "pointint_hip.h"
+
+namespace PoIntInt {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+int g_ff = PII_FF_FAST;
+int g_gram = PII_GRAM_FP16X3;
+
+// One context per process on device 0, created on first use.
+pii_context* context() {
+  static std::once_flag once;
+  static pii_context* ctx = nullptr;
+  std::call_once(once, [] {
+    if (pii_context_create(0, &ctx) != PII_OK) {
+      std::cerr << "PoIntInt: HIP context creation failed: " << pii_last_error() << std::endl;
+      ctx = nullptr;
+    }
+  });
+  return ctx;
+}
+
+bool report(int rc, const char* where) {
+  if (rc == PII_OK) return true;
+  std::cerr << "PoIntInt::" << where << ": " << pii_last_error() << " (status " << rc << ")"
+            << std::endl;
+  return false;
+}
+
+// Device-resident copies made for one call (the reference allocates per call too).
+struct CallObjects {
+  std::vector<pii_geometry*> geoms;
+  pii_kgrid* kg = nullptr;
+  ~CallObjects() {
+    for (pii_geometry* g : geoms) pii_geometry_destroy(g);
+    if (kg) pii_kgrid_destroy(kg);
+  }
+};
+
+pii_geometry* upload(pii_context* ctx, const Geometry& g) {
+  pii_geometry* out = nullptr;
+  int rc = PII_OK;
+  if (g.type == GEOM_TRIANGLE) {
+    std::vector<double> tri(12 * g.tris.size());
+    std::vector<int32_t> vid(3 * g.tris.size());
+    for (size_t i = 0; i < g.tris.size(); ++i) {
+      const TriPacked& t = g.tris[i];
+      const double v[12] = {t.a.x, t.a.y, t.a.z, t.e1.x, t.e1.y, t.e1.z,
+                            t.e2.x, t.e2.y, t.e2.z, t.S.x, t.S.y, t.S.z};
+      std::memcpy(&tri[12 * i], v, sizeof(v));
+      vid[3 * i] = t.vid0;
+      vid[3 * i + 1] = t.vid1;
+      vid[3 * i + 2] = t.vid2;
+    }
+    rc = pii_geometry_create_tris(ctx, tri.data(), vid.data(), (int)g.tris.size(), &out);
+  } else if (g.type == GEOM_DISK) {
+    std::vector<double> el(8 * g.disks.size());
+    for (size_t i = 0; i < g.disks.size(); ++i) {
+      const DiskPacked& d = g.disks[i];
+      const double v[8] = {d.c.x, d.c.y, d.c.z, d.n.x, d.n.y, d.n.z, d.rho, d.area};
+      std::memcpy(&el[8 * i], v, sizeof(v));
+    }
+    rc = pii_geometry_create_disks(ctx, el.data(), (int)g.disks.size(), &out);
+  } else {
+    std::vector<double> el(8 * g.gaussians.size());
+    for (size_t i = 0; i < g.gaussians.size(); ++i) {
+      const GaussianPacked& d = g.gaussians[i];
+      const double v[8] = {d.c.x, d.c.y, d.c.z, d.n.x, d.n.y, d.n.z, d.sigma, d.w};
+      std::memcpy(&el[8 * i], v, sizeof(v));
+    }
+    rc = pii_geometry_create_gaussians(ctx, el.data(), (int)g.gaussians.size(), &out);
+  }
+  if (!report(rc, "upload geometry")) return nullptr;
+  return out;
+}
+
+pii_kgrid* upload(pii_context* ctx, const KGrid& kg) {
+  const int Q = (int)kg.kmag.size();
+  std::vector<double> dirs(3 * (size_t)Q);
+  for (int q = 0; q < Q; ++q)
+    for (int c = 0; c < 3; ++c) dirs[3 * (size_t)q + c] = kg.dirs[q][c];
+  pii_kgrid* out = nullptr;
+  if (!report(pii_kgrid_create(ctx, Q, dirs.data(), kg.kmag.data(), kg.w.data(), &out),
+              "upload k-grid"))
+    return nullptr;
+  return out;
+}
+
+double3 d3(double x, double y, double z) {
+  double3 v;
+  v.x = x;
+  v.y = y;
+  v.z = z;
+  return v;
+}
+
+}  // namespace
+
+int Geometry::num_elements() const {
+  switch (type) {
+    case GEOM_TRIANGLE: return (int)tris.size();
+    case GEOM_DISK: return (int)disks.size();
+    case GEOM_GAUSSIAN: return (int)gaussians.size();
+  }
+  return 0;
+}
+
+void set_precision(int ff_precision, int gram_mode) {
+  g_ff = ff_precision;
+  g_gram = gram_mode;
+}
+
+// ---- packing: cpp/src/geometry/packing.cpp:10-124 ----
+std::vector<TriPacked> pack_tris(const MatrixXd& V, const MatrixXi& F) {
+  std::vector<TriPacked> T(F.rows());
+  for (int i = 0; i < F.rows(); ++i) {
+    const int ia = F(i, 0), ib = F(i, 1), ic = F(i, 2);
+    const double a[3] = {V(ia, 0), V(ia, 1), V(ia, 2)};
+    const double e1[3] = {V(ib, 0) - a[0], V(ib, 1) - a[1], V(ib, 2) - a[2]};
+    const double e2[3] = {V(ic, 0) - a[0], V(ic, 1) - a[1], V(ic, 2) - a[2]};
+    T[i].a = d3(a[0], a[1], a[2]);
+    T[i].e1 = d3(e1[0], e1[1], e1[2]);
+    T[i].e2 = d3(e2[0], e2[1], e2[2]);
+    T[i].S = d3(0.5 * (e1[1] * e2[2] - e1[2] * e2[1]), 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]),
+                0.5 * (e1[0] * e2[1] - e1[1] * e2[0]));
+    T[i].vid0 = ia;
+    T[i].vid1 = ib;
+    T[i].vid2 = ic;
+  }
+  return T;
+}
+
+static void unit(const MatrixXd& N, int i, double* n) {
+  const double l = std::sqrt(N(i, 0) * N(i, 0) + N(i, 1) * N(i, 1) + N(i, 2) * N(i, 2));
+  for (int c = 0; c < 3; ++c) n[c] = l > 0 ? N(i, c) / l : N(i, c);
+}
+
+std::vector<DiskPacked> pack_disks(const MatrixXd& P, const MatrixXd& N, const VectorXd& r,
+                                   bool is_radius) {
+  std::vector<DiskPacked> D(P.rows());
+  for (int i = 0; i < P.rows(); ++i) {
+    double n[3];
+    unit(N, i, n);
+    D[i].c = d3(P(i, 0), P(i, 1), P(i, 2));
+    D[i].n = d3(n[0], n[1], n[2]);
+    if (is_radius) {
+      D[i].rho = r[i];
+      D[i].area = kPi * r[i] * r[i];
+    } else {
+      D[i].area = r[i];
+      D[i].rho = std::sqrt(r[i] / kPi);
+    }
+  }
+  return D;
+}
+
+std::vector<GaussianPacked> pack_gaussians(const MatrixXd& P, const MatrixXd& N,
+                                           const VectorXd& sigmas, const VectorXd& weights) {
+  std::vector<GaussianPacked> G(P.rows());
+  for (int i = 0; i < P.rows(); ++i) {
+    double n[3];
+    unit(N, i, n);
+    G[i].c = d3(P(i, 0), P(i, 1), P(i, 2));
+    G[i].n = d3(n[0], n[1], n[2]);
+    G[i].sigma = sigmas[i];
+    G[i].w = weights[i];
+  }
+  return G;
+}
+
+Geometry make_triangle_mesh(const MatrixXd& V, const MatrixXi& F) {
+  Geometry g(GEOM_TRIANGLE);
+  g.tris = pack_tris(V, F);
+  return g;
+}
+
+Geometry make_point_cloud(const MatrixXd& P, const MatrixXd& N, const VectorXd& r,
+                          bool is_radius) {
+  Geometry g(GEOM_DISK);
+  g.disks = pack_disks(P, N, r, is_radius);
+  return g;
+}
+
+Geometry make_gaussian_splat(const MatrixXd& P, const MatrixXd& N, const VectorXd& sigmas,
+                             const VectorXd& weights) {
+  Geometry g(GEOM_GAUSSIAN);
+  g.gaussians = pack_gaussians(P, N, sigmas, weights);
+  return g;
+}
+
+// cpp/src/geometry/geometry_helpers.cpp:16-38
+void create_unit_cube_mesh(MatrixXd& V, MatrixXi& F) {
+  static const double v[24] = {-0.5, -0.5, -0.5, 0.5, -0.5, -0.5, 0.5, 0.5, -0.5, -0.5, 0.5, -0.5,
+                               -0.5, -0.5, 0.5,  0.5, -0.5, 0.5,  0.5, 0.5, 0.5,  -0.5, 0.5, 0.5};
+  static const int f[36] = {0, 2, 1, 0, 3, 2, 4, 5, 6, 4, 6, 7, 0, 1, 5, 0, 5, 4,
+                            2, 7, 6, 2, 3, 7, 0, 4, 7, 0, 7, 3, 1, 6, 5, 1, 2, 6};
+  V = MatrixXd(8, 3);
+  F = MatrixXi(12, 3);
+  std::memcpy(V.data.data(), v, sizeof(v));
+  std::memcpy(F.data.data(), f, sizeof(f));
+}
+
+// cpp/src/geometry/geometry_helpers.cpp:40-81
+void create_sphere_pointcloud(MatrixXd& P, MatrixXd& N, VectorXd& radii, int n) {
+  P = MatrixXd(n, 3);
+  N = MatrixXd(n, 3);
+  radii.assign(n, std::sqrt((4.0 * kPi / n) / kPi));
+  const double ga = kPi * (3.0 - std::sqrt(5.0));
+  for (int i = 0; i < n; ++i) {
+    const double y = 1.0 - (2.0 * i) / (n - 1.0);
+    const double ry = std::sqrt(1.0 - y * y);
+    const double x = ry * std::cos(ga * i), z = ry * std::sin(ga * i);
+    P(i, 0) = N(i, 0) = x;
+    P(i, 1) = N(i, 1) = y;
+    P(i, 2) = N(i, 2) = z;
+  }
+}
+
+// ---- quadrature ----
+std::pair<std::vector<double>, std::vector<double>> gauss_legendre_interval(int n, double a,
+                                                                            double b) {
+  std::vector<double> x(n), w(n);
+  if (n > 0) report(pii_gauss_legendre(n, a, b, x.data(), w.data()), "gauss_legendre_interval");
+  return {x, w};
+}
+
+LebedevGrid load_lebedev_txt(const std::string& path) {
+  int M = 0;
+  if (pii_lebedev_load(path.c_str(), nullptr, nullptr, 0, &M) != PII_OK)
+    throw std::runtime_error("Failed to open Lebedev file: " + path);
+  std::vector<double> d(3 * (size_t)M), w(M);
+  pii_lebedev_load(path.c_str(), d.data(), w.data(), M, &M);
+  LebedevGrid L;
+  L.dirs.resize(M);
+  for (int j = 0; j < M; ++j) L.dirs[j] = {d[3 * j], d[3 * j + 1], d[3 * j + 2]};
+  L.weights = w;
+  return L;
+}
+
+KGrid build_kgrid(const std::vector<std::array<double, 3>>& leb_dirs,
+                  const std::vector<double>& leb_w, int Nrad) {
+  const int M = (int)leb_w.size();
+  std::vector<double> ld(3 * (size_t)M);
+  for (int j = 0; j < M; ++j)
+    for (int c = 0; c < 3; ++c) ld[3 * j + c] = leb_dirs[j][c];
+  const size_t Q = (size_t)M * Nrad;
+  std::vector<double> dirs(3 * Q), kmag(Q), w(Q);
+  KGrid kg;
+  if (!report(pii_kgrid_build(ld.data(), leb_w.data(), M, Nrad, dirs.data(), kmag.data(),
+                              w.data()), "build_kgrid"))
+    return kg;
+  kg.dirs.resize(Q);
+  for (size_t q = 0; q < Q; ++q) kg.dirs[q] = {dirs[3 * q], dirs[3 * q + 1], dirs[3 * q + 2]};
+  kg.kmag = kmag;
+  kg.w = w;
+  return kg;
+}
+
+// ---- drivers ----
+IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
+    const std::vector<Geometry>& geoms, const std::vector<std::shared_ptr<DoFParameterization>>& dofs,
+    const std::vector<VectorXd>& dof_vectors, const KGrid& kgrid, ComputationFlags flags,
+    int /*blockSize*/, bool enable_profiling) {
+  IntersectionVolumeMatrixResult res;
+  const int n = (int)geoms.size();
+  res.volume_matrix = MatrixXd::Zero(n, n);
+  if (n == 0) return res;
+  if ((int)dofs.size() != n || (int)dof_vectors.size() != n) {
+    std::cerr << "Error: Mismatch between number of geometries (" << n
+              << ") and number of DoF parameterizations (" << dofs.size()
+              << ") or DoF vectors (" << dof_vectors.size() << ")" << std::endl;
+    return res;
+  }
+  pii_context* ctx = context();
+  if (!ctx) return res;
+  CallObjects co;
+  co.kg = upload(ctx, kgrid);
+  if (!co.kg) return res;
+  std::vector<int> kinds(n), nd(n);
+  std::vector<const double*> dp(n);
+  for (int i = 0; i < n; ++i) {
+    pii_geometry* g = upload(ctx, geoms[i]);
+    if (!g) return res;
+    co.geoms.push_back(g);
+    kinds[i] = dofs[i] ? dofs[i]->dof_kind() : PII_DOF_AFFINE;
+    nd[i] = (int)dof_vectors[i].size();
+    dp[i] = dof_vectors[i].data();
+  }
+  if (enable_profiling) pii_context_set_profiling(ctx, 1);
+  std::vector<const pii_geometry*> gc(co.geoms.begin(), co.geoms.end());
+  if (!report(pii_volume_matrix(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(), co.kg,
+                                g_ff, g_gram, res.volume_matrix.data.data()),
+              "compute_intersection_volume_matrix_cuda")) {
+    res.volume_matrix = MatrixXd::Zero(n, n);
+    return res;
+  }
+  if (enable_profiling) {
+    double ms[4];
+    pii_context_timings(ctx, ms, 4);
+    std::cout << "\n=== MI355X Multi-Object Volume Computation Profiler ===\n"
+              << "Number of objects: " << n << "\nK-grid nodes: " << kgrid.kmag.size()
+              << "\n  Pre-pass:    " << ms[0] << " ms\n  Form factor: " << ms[1]
+              << " ms\n  Gram:        " << ms[2] << " ms" << std::endl;
+    pii_context_set_profiling(ctx, 0);
+  }
+  if (needs_gradient(flags)) {
+    // grad_matrix[i][j] = dV_ij/dtheta_i from one VJP per unordered pair.
+    res.grad_matrix.assign(n, std::vector<VectorXd>(n));
+    std::vector<double> G((size_t)n * n);
+    std::vector<VectorXd> grads(n);
+    std::vector<double*> gp(n);
+    for (int i = 0; i < n; ++i) {
+      grads[i].assign(nd[i], 0.0);
+      gp[i] = grads[i].data();
+    }
+    for (int i = 0; i < n; ++i) {
+      for (int j = i; j < n; ++j) {
+        std::fill(G.begin(), G.end(), 0.0);
+        G[(size_t)i * n + j] = 1.0;
+        if (!report(pii_volume_matrix_vjp(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(),
+                                          co.kg, G.data(), gp.data(), nullptr),
+                    "gradient")) {
+          res.grad_matrix.clear();
+          return res;
+        }
+        if (i == j) {
+          res.grad_matrix[i][i] = grads[i];
+          for (double& x : res.grad_matrix[i][i]) x *= 0.5;
+        } else {
+          res.grad_matrix[i][j] = grads[i];
+          res.grad_matrix[j][i] = grads[j];
+        }
+      }
+    }
+  }
+  if (needs_hessian(flags))
+    std::cerr << "PoIntInt: Gauss-Newton Hessians are not implemented in this build" << std::endl;
+  return res;
+}
+
+IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
+    const std::vector<Geometry>& geometries, const KGrid& kgrid, int blockSize,
+    bool enable_profiling) {
+  const int n = (int)geometries.size();
+  std::vector<std::shared_ptr<DoFParameterization>> dofs(n, std::make_shared<AffineDoF>());
+  const VectorXd id = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+  std::vector<VectorXd> dv(n, id);
+  return compute_intersection_volume_matrix_cuda(geometries, dofs, dv, kgrid,
+                                                 ComputationFlags::VOLUME_ONLY, blockSize,
+                                                 enable_profiling);
+}
+
+IntersectionVolumeResult compute_intersection_volume_cuda(
+    const Geometry& g1, const Geometry& g2, const std::shared_ptr<DoFParameterization>& dof1,
+    const std::shared_ptr<DoFParameterization>& dof2, const VectorXd& dofs1,
+    const VectorXd& dofs2, const KGrid& kgrid, ComputationFlags flags, int blockSize,
+    bool enable_profiling) {
+  IntersectionVolumeResult r;
+  auto m = compute_intersection_volume_matrix_cuda({g1, g2}, {dof1, dof2}, {dofs1, dofs2}, kgrid,
+                                                   flags, blockSize, enable_profiling);
+  if (m.volume_matrix.rows() == 2) r.volume = m.volume_matrix(0, 1);
+  if (needs_gradient(flags) && m.grad_matrix.size() == 2) {
+    r.grad_geom1 = m.grad_matrix[0][1];
+    r.grad_geom2 = m.grad_matrix[1][0];
+  }
+  return r;
+}
+
+double compute_intersection_volume_cuda(const Geometry& g1, const Geometry& g2,
+                                        const KGrid& kgrid, int blockSize,
+                                        bool enable_profiling) {
+  auto m = compute_intersection_volume_matrix_cuda({g1, g2}, kgrid, blockSize, enable_profiling);
+  return m.volume_matrix.rows() == 2 ? m.volume_matrix(0, 1) : 0.0;
+}
+
+std::complex<double> compute_Ak_cuda(const Geometry& geom, const Vector3d& k,
+                                     const std::shared_ptr<DoFParameterization>& dof,
+                                     const VectorXd& dofs, int /*blockSize*/) {
+  pii_context* ctx = context();
+  if (!ctx) return 0.0;
+  CallObjects co;
+  const double km = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  const double dir[3] = {km > 0 ? k[0] / km : 1.0, km > 0 ? k[1] / km : 0.0,
+                         km > 0 ? k[2] / km : 0.0};
+  const double w = 1.0;
+  if (!report(pii_kgrid_create(ctx, 1, dir, &km, &w, &co.kg), "compute_Ak_cuda")) return 0.0;
+  pii_geometry* g = upload(ctx, geom);
+  if (!g) return 0.0;
+  co.geoms.push_back(g);
+  double out[2] = {0, 0};
+  if (!report(pii_form_factor(ctx, g, dof ? dof->dof_kind() : PII_DOF_AFFINE, dofs.data(),
+                              (int)dofs.size(), co.kg, PII_FF_FP64, out),
+              "compute_Ak_cuda"))
+    return 0.0;
+  return {out[0], out[1]};
+}
+
+// cpp/src/compute_volume.cpp:61-73 with cpp/src/dof/affine_dof.cpp:324-384 (identity DoFs)
+double compute_volume_cpu(const Geometry& g, bool) {
+  double v = 0.0;
+  if (g.type == GEOM_TRIANGLE) {
+    for (const TriPacked& t : g.tris) v += t.a.x * t.S.x + t.a.y * t.S.y + t.a.z * t.S.z;
+  } else if (g.type == GEOM_DISK) {
+    for (const DiskPacked& d : g.disks)
+      v += (d.c.x * d.n.x + d.c.y * d.n.y + d.c.z * d.n.z) * d.area;
+  } else {
+    for (const GaussianPacked& d : g.gaussians)
+      v += (d.c.x * d.n.x + d.c.y * d.n.y + d.c.z * d.n.z) * d.w;
+  }
+  return v / 3.0;
+}
+
+// cpp/src/analytical_intersection.cpp:15-60
+double box_box_intersection_volume(const Vector3d& c1, const Vector3d& h1, const Vector3d& c2,
+                                   const Vector3d& h2) {
+  double v = 1.0;
+  for (int i = 0; i < 3; ++i) {
+    const double lo = std::max(c1[i] - h1[i], c2[i] - h2[i]);
+    const double hi = std::min(c1[i] + h1[i], c2[i] + h2[i]);
+    v *= std::max(0.0, hi - lo);
+  }
+  return v;
+}
+
+double sphere_sphere_intersection_volume(const Vector3d& c1, double r1, const Vector3d& c2,
+                                         double r2) {
+  const double dx = c2[0] - c1[0], dy = c2[1] - c1[1], dz = c2[2] - c1[2];
+  const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+  if (d >= r1 + r2) return 0.0;
+  if (d <= std::abs(r1 - r2)) {
+    const double rm = std::min(r1, r2);
+    return 4.0 * kPi / 3.0 * rm * rm * rm;
+  }
+  const double t1 = (r1 + r2 - d) * (r1 + r2 - d);
+  const double t2 = d * d + 2.0 * d * (r1 + r2) - 3.0 * (r1 * r1 + r2 * r2) + 6.0 * r1 * r2;
+  return kPi * t1 * t2 / (12.0 * d);
+}
+
+}  // namespace PoIntInt
