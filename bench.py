@@ -73,6 +73,24 @@ def cpu_baseline(wl, kg, target_seconds=15.0):
     return t_full, len(idx), t_s, O.threads()
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One device-to-device copy of nbytes (hipMemcpy through the HIP runtime
+    the library uses), so that a counter pass reads FETCH_SIZE / WRITE_SIZE of
+    a kernel of known traffic next to the form-factor kernel's."""
+    import ctypes as C
+    P.Context.default()  # loads the library and its HIP runtime
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy calibration: {nbytes} bytes device-to-device", file=sys.stderr)
+
+
 def make_geoms(wl):
     out = []
     for o in wl.objects:
@@ -212,7 +230,11 @@ def main():
     ap.add_argument("--pool", type=int, default=4096)
     ap.add_argument("--budget-gb", type=float, default=2.0)
     ap.add_argument("--max-batch", type=int, default=1024)
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="GiB copied device-to-device before the run (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(int(args.copy_calibration * (1 << 30)))
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path "

@@ -86,6 +86,28 @@ __device__ __forceinline__ float2 full_phasor(double u) {
   const double t = u * 0.15915494309189535;  // 1/(2 pi)
   return phasor_rev(t - rint(t));
 }
+__device__ __forceinline__ float2 full_phasor_hw(double u) {
+  const double t = u * 0.15915494309189535;  // 1/(2 pi)
+  return phasor_rev_hw(t - rint(t));
+}
+
+// J1(x)/x = sum_m (-1)^m y^m / (2 m! (m+1)!), y = x^2/4, Horner in FP32 with
+// terms m <= 10: truncation < 1e-10 for x <= 4; rounding ~1e-7 absolute
+// (the value itself is at most 1/2).
+__device__ __forceinline__ float j1ox_small(float x) {
+  const float y = 0.25f * x * x;
+  float s = 3.451844740057556e-15f;           // 1/(2 10! 11!)
+  s = s * y - 3.7970292140633116e-13f;         // 1/(2 9! 10!)
+  s = s * y + 3.417326292656981e-11f;          // 1/(2 8! 9!)
+  s = s * y - 2.460474930713026e-09f;          // 1/(2 7! 8!)
+  s = s * y + 1.3778659611992944e-07f;         // 1/(2 6! 7!)
+  s = s * y - 5.787037037037037e-06f;          // 1/(2 5! 6!)
+  s = s * y + 1.7361111111111112e-04f;         // 1/(2 4! 5!)
+  s = s * y - 3.472222222222222e-03f;          // 1/(2 3! 4!)
+  s = s * y + 4.1666666666666664e-02f;         // 1/(2 2! 3!)
+  s = s * y - 0.25f;                           // 1/(2 1! 2!)
+  return s * y + 0.5f;
+}
 
 __device__ __forceinline__ float sinc_from(float h, float sin_h) {
   const float ah = fabsf(h);
@@ -156,18 +178,29 @@ __device__ __forceinline__ float2 tri_term(double al, double be, float g2, float
   return tri_main(al, be, g2, Q0, Q1, Q2);
 }
 
-// J1(x)/x for x >= 0 in FP64: series below 12, Hankel asymptotics above
-// (6 terms each of P and Q: relative error < 1e-9 at x = 12).
+// Coefficients (-1)^m / (2 m! (m+1)!) of J1(x)/x in powers of y = x^2/4.
+struct J1Series {
+  double c[25];
+  constexpr J1Series() : c() {
+    double t = 0.5;
+    for (int m = 0; m < 25; ++m) {
+      c[m] = t;
+      t = -t / ((double)(m + 1) * (double)(m + 2));
+    }
+  }
+};
+__device__ constexpr J1Series kJ1Series{};
+
+// J1(x)/x for x >= 0 in FP64: the power series (Horner, 25 terms, truncation
+// < 1e-13 for x < 12) below 12, Hankel asymptotics above (6 terms each of P
+// and Q: relative error < 1e-9 at x = 12).
 __device__ __forceinline__ double j1ox_fast(double x) {
   if (x < 12.0) {
-    const double q = 0.25 * x * x;
-    double term = 0.5, sum = 0.5;
-    for (int m = 1; m < 40; ++m) {
-      term *= -q / (double)(m * (m + 1));
-      sum += term;
-      if (fabs(term) < 1e-12 * fabs(sum)) break;
-    }
-    return sum;
+    const double y = 0.25 * x * x;
+    double s = kJ1Series.c[24];
+#pragma unroll
+    for (int m = 23; m >= 0; --m) s = fma(s, y, kJ1Series.c[m]);
+    return s;
   }
   const double inv = 1.0 / x;
   const double i2 = inv * inv;
@@ -279,21 +312,39 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
     const double kp2 = kpx * kpx + kpy * kpy + kpz * kpz;
     const double inv_km = 1.0 / km;
     const bool disk = (d.type == PII_GEOM_DISK);
+    // |k'| <= kmax ||A||_F for every lane: bounds rho |k'_perp| per disk.
+    double kmax = km;
+    for (int off = 32; off > 0; off >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, off));
+    double an2 = 0.0;
+    for (int i = 3; i < 12; ++i) an2 += p[i] * p[i];
+    const float kbound = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(
+        (float)(kmax * sqrt(an2) * (1.0 + 1e-6)))));
     float px = 0.f, py = 0.f;
     int cnt = 0;
     const auto* rec = as_const(d.rec);
     for (int e = wave; e < d.n; e += kWaves) {
       const auto* r = rec + (size_t)kRecSurfel * e;
       const double kdn = kx * r[3] + ky * r[4] + kz * r[5];
-      double S;
+      const double r2 = kp2 - kdn * kdn;
+      float S;
       if (disk) {
-        const double rr = r[6] * sqrt(fmax(0.0, kp2 - kdn * kdn));
-        S = (rr < 1e-10) ? r[7] : 2.0 * r[7] * j1ox_fast(rr);
+        const float rho = (float)r[6];
+        const float x = rho * sqrtf((float)fmax(0.0, r2));
+        float j;
+        if (rho * kbound < 4.0f) {
+          j = j1ox_small(x);          // every lane has x < 4
+        } else if (x < 4.0f) {
+          j = j1ox_small(x);
+        } else {
+          j = (float)j1ox_fast((double)x);
+        }
+        S = 2.0f * (float)r[7] * j;
       } else {
-        S = r[7] * (double)__expf((float)(-0.5 * r[6] * r[6] * (kp2 - kdn * kdn)));
+        // no clamp of r2, as on the reference's CPU path (affine_dof.cpp:152)
+        S = (float)r[7] * __expf((float)(-0.5 * r[6] * r[6] * r2));
       }
-      const float Ap = (float)(S * kdn * inv_km);
-      const float2 ph = full_phasor(kx * r[0] + ky * r[1] + kz * r[2]);
+      const float Ap = S * (float)(kdn * inv_km);
+      const float2 ph = full_phasor_hw(kx * r[0] + ky * r[1] + kz * r[2]);
       px += Ap * ph.x;
       py += Ap * ph.y;
       if (++cnt == 64) {
