@@ -29,11 +29,17 @@ sys.path.insert(0, ROOT)
 import pointint_amd as P  # noqa: E402
 from pointint_amd import workloads as W  # noqa: E402
 
-# Algorithmic VALU work per triangle x k-node of the form factor: 30 FMA-class
-# operations (alpha, beta, gamma and phase dot products; complex Phi, phase
-# and accumulate) = 60 flop, plus 3 sin/cos pairs and 3 reciprocals counted
-# as one flop each.
+# Algorithmic VALU work per triangle x k-node of the per-face kernel: 30
+# FMA-class operations (alpha, beta, gamma and phase dot products; complex Phi,
+# phase and accumulate) = 60 flop, plus 3 sin/cos pairs and 3 reciprocals
+# counted as one flop each.
 FLOP_PER_TRI_K = 66.0
+# Moment series (order 48): per face x direction, 4 dot products (20 flop),
+# H_0 (1) and per order one multiply and three FMAs (7 flop x 48); per object
+# x node, two Horner sums (2 x 48 flop) and the phase product (12).
+SERIES_ORDER = 48
+FLOP_PER_FACE_DIR = 21.0 + 7.0 * SERIES_ORDER
+FLOP_PER_SERIES_NODE = 2.0 * SERIES_ORDER + 12.0
 PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 (= non-packed FP32) peak
 PEAK_MFMA_F16_TFLOPS = 2500.0
 
@@ -267,16 +273,18 @@ def main():
     for _ in range(args.warmup):
         step(args.ff, args.gram)
     ctx.set_profiling(True)
-    ff_ms, gram_ms, pre_ms = [], [], []
+    ff_ms, gram_ms, pre_ms, ser_ms, face_ms = [], [], [], [], []
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         V = step(args.ff, args.gram)
         t = ctx.timings()
         ff_ms.append(t["form_factor"]); gram_ms.append(t["gram"]); pre_ms.append(t["prepass"])
+        ser_ms.append(t["ff_series"]); face_ms.append(t["ff_per_face"])
     ctx.synchronize()
     t1 = time.perf_counter()
     ctx.set_profiling(False)
+    work = ctx.work()
     ms_step = (t1 - t0) * 1e3 / args.steps
     print(f"[bench] {args.ff}/{args.gram}: {ms_step:.2f} ms/step, FF {np.mean(ff_ms):.2f} ms, "
           f"Gram {np.mean(gram_ms):.3f} ms", file=sys.stderr, flush=True)
@@ -304,16 +312,31 @@ def main():
         print(f"[bench] fp64 reference: {ref['fp64_ms_per_step']:.2f} ms/step, "
               f"err {ref['max_err_vs_fp64_scaled']:.2e}", file=sys.stderr, flush=True)
 
-    # Roofline of the dominant kernel (form factor).  Units one launch
-    # processes: triangles x k-nodes it evaluates (the fast path evaluates one
-    # node of each antipodal pair).
-    q_eval = kg.Qh if (args.ff == "fast" and kg.symmetric) else kg.Q
-    ff_s = float(np.mean(ff_ms)) * 1e-3
-    ach = wl.prim_count() * q_eval * FLOP_PER_TRI_K / ff_s / 1e12
+    # Roofline of the dominant form-factor kernel, from its HIP-event time and
+    # the work counters of the last call (units evaluated by that launch).
+    kernels = {}
+    if args.ff == "fast":
+        kernels["k_ff_fast (per-face)"] = {
+            "ms": float(np.mean(face_ms)), "units": work["tri_node"],
+            "unit": "triangle x node", "flop_per_unit": FLOP_PER_TRI_K}
+        if work["series_face_dir"] > 0:
+            kernels["k_series_tri (moment series)"] = {
+                "ms": float(np.mean(ser_ms)), "units": work["series_face_dir"],
+                "unit": "face x direction (+ object x node evaluations)",
+                "flop_per_unit": FLOP_PER_FACE_DIR,
+                "extra_flop": work["series_node"] * FLOP_PER_SERIES_NODE}
+    else:
+        kernels["k_ff_fp64"] = {"ms": float(np.mean(ff_ms)), "units": wl.prim_count() * kg.Q,
+                                "unit": "triangle x node", "flop_per_unit": FLOP_PER_TRI_K}
+    for k in kernels.values():
+        k["tflops"] = ((k["units"] * k["flop_per_unit"] + k.get("extra_flop", 0.0))
+                       / (k["ms"] * 1e-3) / 1e12) if k["ms"] > 0 else 0.0
+    dom = max(kernels, key=lambda n: kernels[n]["ms"])
+    ach = kernels[dom]["tflops"]
     roofline = {"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_VALU_TFLOPS, "traffic": None,
-                "kernel": "form factor", "kernel_ms": float(np.mean(ff_ms)),
-                "units_per_launch": wl.prim_count() * q_eval}
+                "kernel": dom, "kernel_ms": kernels[dom]["ms"],
+                "units_per_launch": kernels[dom]["units"], "form_factor_kernels": kernels}
 
     cpu = None
     if not args.no_cpu:
@@ -340,7 +363,8 @@ def main():
         "dtype": "f32+f64" if args.ff == "fast" else "f64",
         "data": "synthetic (seeded)",
         "config": {"workload": ("256 icospheres L3 (1,280 tris), Lebedev 1202 x L=64 "
-                                "(Q=76,928), FP32 form factors, BF16x3 Gram")
+                                "(Q=76,928), fast form factors (FP64 moment series for "
+                                "|k|R <= 12, per-face FP32/FP64 above), BF16x3 Gram")
                                if args.workload == "icospheres" else
                                ("1,024 mixed objects (512 star meshes T in [80, 20480], "
                                 "512 disk clouds in [1k, 50k]), Lebedev 2702 x L=96 "

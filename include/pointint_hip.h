@@ -95,10 +95,25 @@ int pii_context_synchronize(pii_context* ctx);
 /* Phase timing with HIP events on the context's stream (the reference's
  * enable_profiling flag, e.g.
  * cpp/src/compute_intersection_volume_multi_object.cu:626-695, which used host
- * clocks).  Phases: 0 pre-pass, 1 form factor, 2 Gram, 3 adjoint; ms[p] holds
- * the last call's duration of phase p (0 when it did not run). */
+ * clocks).  Phases: 0 pre-pass, 1 form factor, 2 Gram, 3 adjoint, and within
+ * phase 1 on the fast path: 4 moment-series kernel, 5 per-face kernel; ms[p]
+ * holds the last call's duration of phase p (0 when it did not run). */
 int pii_context_set_profiling(pii_context* ctx, int enable);
 int pii_context_timings(pii_context* ctx, double* ms, int n);
+/* Work done by the last call's form-factor kernels (for roofline accounting;
+ * no reference counterpart): units[0] triangle x node evaluations of the
+ * per-face kernel, [1] face x direction passes of the moment series, [2]
+ * object x node evaluations of the series, [3] disk/Gaussian x node
+ * evaluations.  Entries beyond 3 are set to 0. */
+int pii_context_work(pii_context* ctx, double* units, int n);
+/* Low-|k| moment series of the fast path (default on).  With it, triangle
+ * objects on a shell-structured k-grid (every shell repeats one direction
+ * set, as pii_kgrid_build makes) take their form factor at the leading shells
+ * with |k| R <= 12 (R the object's radius about its bounding-box centre) from
+ * a 48-term moment expansion, and the per-face evaluation only above.  No
+ * reference counterpart (the reference evaluates every face at every node);
+ * switching it off gives the per-face path everywhere. */
+int pii_context_set_series(pii_context* ctx, int enable);
 
 /* ---- k-grid resident in HBM ----------------------------------------------- */
 int pii_kgrid_create(pii_context* ctx, int Q, const double* dirs, const double* kmag,

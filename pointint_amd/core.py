@@ -59,11 +59,24 @@ class Context:
     def set_profiling(self, on: bool) -> None:
         N.check(N.load().pii_context_set_profiling(self._h, 1 if on else 0))
 
+    def set_series(self, on: bool) -> None:
+        """Moment series for the low-|k| shells of triangle objects on the fast
+        path (default on); off gives the per-face evaluation at every node."""
+        N.check(N.load().pii_context_set_series(self._h, 1 if on else 0))
+
     def timings(self) -> dict:
         """Last call's phase durations (ms) from HIP events on the context stream."""
-        ms = np.zeros(4)
-        N.check(N.load().pii_context_timings(self._h, N.ptr(ms), 4))
-        return {"prepass": ms[0], "form_factor": ms[1], "gram": ms[2], "adjoint": ms[3]}
+        ms = np.zeros(6)
+        N.check(N.load().pii_context_timings(self._h, N.ptr(ms), 6))
+        return {"prepass": ms[0], "form_factor": ms[1], "gram": ms[2], "adjoint": ms[3],
+                "ff_series": ms[4], "ff_per_face": ms[5]}
+
+    def work(self) -> dict:
+        """Work of the last call's form-factor kernels (evaluation counts)."""
+        u = np.zeros(4)
+        N.check(N.load().pii_context_work(self._h, N.ptr(u), 4))
+        return {"tri_node": u[0], "series_face_dir": u[1], "series_node": u[2],
+                "surfel_node": u[3]}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -2,6 +2,7 @@
 // geometries, and the form-factor / volume-matrix entry points.  Host code
 // only; kernels live in the .hip translation units.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -64,6 +65,36 @@ static void upload(DevBuf& b, const T* src, size_t count, hipStream_t s) {
   if (count) PII_HIP(hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+// Shell structure of the representative nodes: the first run of equal |k|
+// gives Mh; every later block of Mh nodes must repeat those directions
+// exactly at one magnitude of its own.
+void kgrid_shells(KGridDev& k) {
+  k.nshell = 0;
+  k.Mh = 0;
+  k.h_shell_k.clear();
+  const int Qh = k.Qh;
+  if (Qh <= 0) return;
+  auto kq = [&](int r) { return k.h_kmag[k.h_rep[r]]; };
+  auto dq = [&](int r, int c) { return k.h_dirs[3 * (size_t)k.h_rep[r] + c]; };
+  int Mh = 1;
+  while (Mh < Qh && kq(Mh) == kq(0)) ++Mh;
+  if (Qh % Mh != 0) return;
+  const int ns = Qh / Mh;
+  std::vector<double> shell_k(ns);
+  for (int sh = 0; sh < ns; ++sh) {
+    shell_k[sh] = kq(sh * Mh);
+    for (int j = 0; j < Mh; ++j) {
+      const int r = sh * Mh + j;
+      if (kq(r) != shell_k[sh]) return;
+      for (int c = 0; c < 3; ++c)
+        if (dq(r, c) != dq(j, c)) return;
+    }
+  }
+  k.nshell = ns;
+  k.Mh = Mh;
+  k.h_shell_k = std::move(shell_k);
+}
+
 // ---------------------------------------------------------------------------
 // Batch set-up shared by every multi-object entry point.
 // ---------------------------------------------------------------------------
@@ -83,9 +114,81 @@ void validate_object(const GeometryDev& g, int dof_kind, int ndofs) {
   }
 }
 
+// Largest singular value of a row-major 3x3 matrix: square root of the top
+// eigenvalue of A^T A (closed form for symmetric 3x3), rounded up slightly so
+// that it bounds |A x| / |x| despite rounding.
+static double spectral_norm3(const double* A) {
+  double B[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      B[i][j] = A[i] * A[j] + A[3 + i] * A[3 + j] + A[6 + i] * A[6 + j];
+  double fro2 = 0.0;
+  for (int i = 0; i < 9; ++i) fro2 += A[i] * A[i];
+  const double p1 = B[0][1] * B[0][1] + B[0][2] * B[0][2] + B[1][2] * B[1][2];
+  const double q = (B[0][0] + B[1][1] + B[2][2]) / 3.0;
+  double lam;
+  if (p1 <= 1e-300 * fro2 * fro2) {
+    lam = std::max(B[0][0], std::max(B[1][1], B[2][2]));
+  } else {
+    const double p2 = (B[0][0] - q) * (B[0][0] - q) + (B[1][1] - q) * (B[1][1] - q) +
+                      (B[2][2] - q) * (B[2][2] - q) + 2.0 * p1;
+    const double p = std::sqrt(p2 / 6.0);
+    double C[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) C[i][j] = (B[i][j] - (i == j ? q : 0.0)) / p;
+    const double det = C[0][0] * (C[1][1] * C[2][2] - C[1][2] * C[2][1]) -
+                       C[0][1] * (C[1][0] * C[2][2] - C[1][2] * C[2][0]) +
+                       C[0][2] * (C[1][0] * C[2][1] - C[1][1] * C[2][0]);
+    const double r = std::max(-1.0, std::min(1.0, det / 2.0));
+    lam = q + 2.0 * p * std::cos(std::acos(r) / 3.0);
+  }
+  const double sigma = std::sqrt(std::max(lam, 0.0)) * (1.0 + 1e-9) + 1e-300;
+  return std::min(sigma, std::sqrt(fro2) * (1.0 + 1e-12));
+}
+
+// Centre and radius (max |v - c|) of a triangle object's world vertices.
+static void world_bounds(const GeometryDev& g, int kind, const double* dofs, double c[3],
+                         double& R) {
+  if (kind == PII_DOF_AFFINE) {
+    // W = A V + t: centre A c_ref + t, radius <= |A|_2 R_ref
+    for (int r = 0; r < 3; ++r)
+      c[r] = dofs[3 + 3 * r] * g.ref_c[0] + dofs[4 + 3 * r] * g.ref_c[1] +
+             dofs[5 + 3 * r] * g.ref_c[2] + dofs[r];
+    R = spectral_norm3(dofs + 3) * g.ref_R;
+    return;
+  }
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int v = 0; v < g.nv; ++v) {
+    const int dv = g.h_dofvid[v];
+    if (dv < 0) continue;  // only in zero-area faces
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], dofs[3 * (size_t)dv + k]);
+      hi[k] = std::max(hi[k], dofs[3 * (size_t)dv + k]);
+    }
+  }
+  if (!(lo[0] <= hi[0])) {
+    c[0] = c[1] = c[2] = 0.0;
+    R = 0.0;
+    return;
+  }
+  for (int k = 0; k < 3; ++k) c[k] = 0.5 * (lo[k] + hi[k]);
+  double r2 = 0.0;
+  for (int v = 0; v < g.nv; ++v) {
+    const int dv = g.h_dofvid[v];
+    if (dv < 0) continue;
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double t = dofs[3 * (size_t)dv + k] - c[k];
+      d2 += t * t;
+    }
+    r2 = std::max(r2, d2);
+  }
+  R = std::sqrt(r2) * (1.0 + 1e-12);
+}
+
 void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
                   const GeometryDev* const* geoms, const int* kinds, const double* const* dofs,
-                  const int* ndofs, bool fast) {
+                  const int* ndofs, bool fast, const KGridDev* series_kg) {
   N = n;
   // Largest objects first: the dispatcher then starts the longest work early.
   order.resize(N);
@@ -123,6 +226,36 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   }
   w.world.ensure(std::max<size_t>(total_w, 1) * sizeof(double));
   w.faces.ensure(std::max<size_t>(total_rec, 1) * sizeof(double));
+  // Moment series plan: leading shells with |k| R <= kSeriesReach, when at
+  // least kSeriesMinShells of them (below that the per-face kernel is cheaper
+  // than one pass over the faces per direction).
+  constexpr int kSeriesMinShells = 4;
+  std::vector<int> nser(N, 0);
+  std::vector<std::array<double, 3>> ctr(N, {0.0, 0.0, 0.0});
+  std::vector<size_t> moff(N, 0);
+  size_t total_m = 0;
+  Mh = 0;
+  if (fast && series_kg && series_kg->nshell > 0) {
+    Mh = series_kg->Mh;
+    for (int slot = 0; slot < N; ++slot) {
+      const int i = order[slot];
+      const GeometryDev& g = *geoms[i];
+      if (g.type != PII_GEOM_TRIANGLE || g.n == 0) continue;
+      double R;
+      world_bounds(g, kinds[i], dofs[i], ctr[slot].data(), R);
+      int ns = 0;
+      while (ns < series_kg->nshell && series_kg->h_shell_k[ns] >= 0.0 &&
+             series_kg->h_shell_k[ns] * R <= kSeriesReach)
+        ++ns;
+      if (ns < kSeriesMinShells) continue;
+      nser[slot] = ns;
+      moff[slot] = total_m;
+      total_m += align_up((size_t)g.n * kRecMoment, 32);
+    }
+  }
+  if (total_m) w.mrec.ensure(total_m * sizeof(double));
+  nseries = 0;
+  for (int slot = 0; slot < N; ++slot) nseries += nser[slot] > 0;
   h_objs.resize(N);
   for (int slot = 0; slot < N; ++slot) {
     const int i = order[slot];
@@ -146,6 +279,15 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     d.dofs = w.dofs.as<double>() + dof_off[i];
     d.world = w.world.as<double>() + wv_off[slot];
     d.rec = w.faces.as<double>() + rec_off[slot];
+    d.nser = nser[slot];
+    d.qbeg = nser[slot] * Mh;
+    for (int k = 0; k < 3; ++k) d.ctr[k] = ctr[slot][k];
+    d.mrec = nser[slot] ? w.mrec.as<double>() + moff[slot] : nullptr;
+  }
+  qbeg_min = 0;
+  if (N > 0) {
+    qbeg_min = h_objs[0].qbeg;
+    for (int slot = 1; slot < N; ++slot) qbeg_min = std::min(qbeg_min, h_objs[slot].qbeg);
   }
   // Descriptor table + prefix sums in one upload.
   const size_t obj_bytes = align_up(sizeof(ObjDesc) * N, 256);
@@ -160,7 +302,10 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   d_eprefix = reinterpret_cast<const int*>(static_cast<char*>(w.misc.p) + obj_bytes + pre_bytes);
   total_v = h_vprefix[N];
   total_e = h_eprefix[N];
-  if (prof) prof->phase_begin(0);
+  if (prof) {
+    prof->begin_call();
+    prof->phase_begin(0);
+  }
   launch_prepass(s, d_objs, d_vprefix, total_v, d_eprefix, total_e, N, fast ? 1 : 0);
   if (prof) prof->phase_end(0);
   PII_HIP(hipGetLastError());
@@ -276,6 +421,14 @@ int pii_context_set_profiling(pii_context* ctx, int enable) {
   });
 }
 
+int pii_context_set_series(pii_context* ctx, int enable) {
+  return guarded([&] {
+    require(ctx != nullptr, "NULL context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    ctx->c.series = enable != 0;
+  });
+}
+
 int pii_context_timings(pii_context* ctx, double* ms, int n) {
   return guarded([&] {
     require(ctx && ms && n > 0, "pii_context_timings: bad arguments");
@@ -293,6 +446,14 @@ int pii_context_timings(pii_context* ctx, double* ms, int n) {
   });
 }
 
+int pii_context_work(pii_context* ctx, double* units, int n) {
+  return guarded([&] {
+    require(ctx && units && n > 0, "pii_context_work: bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    for (int i = 0; i < n; ++i) units[i] = i < Context::kWork ? ctx->c.work[i] : 0.0;
+  });
+}
+
 int pii_kgrid_create(pii_context* ctx, int Q, const double* dirs, const double* kmag,
                      const double* w, pii_kgrid** out) {
   return guarded([&] {
@@ -307,7 +468,9 @@ int pii_kgrid_create(pii_context* ctx, int Q, const double* dirs, const double* 
     k.h_kmag.assign(kmag, kmag + Q);
     k.h_w.assign(w, w + Q);
     k.symmetric = kgrid_pairs(Q, dirs, kmag, w, k.h_anti, k.h_rep);
+    kgrid_order_reps(Q, dirs, kmag, k.h_anti, k.h_rep);
     k.Qh = (int)k.h_rep.size();
+    kgrid_shells(k);
     auto soa = [&](const std::vector<int>& idx, bool sum_pair) {
       const size_t n = idx.size();
       std::vector<double> a(8 * n);
@@ -354,6 +517,26 @@ int pii_kgrid_info(const pii_kgrid* kg, int* Q, int* Qh, int* symmetric) {
 
 static void finish_triangles(pii_context* ctx, GeometryDev& g) {
   hipStream_t s = ctx->c.stream;
+  // bounding-box centre and radius of the reference vertices (moment series)
+  if (g.nv > 0) {
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int v = 0; v < g.nv; ++v)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], g.h_verts[3 * (size_t)v + k]);
+        hi[k] = std::max(hi[k], g.h_verts[3 * (size_t)v + k]);
+      }
+    for (int k = 0; k < 3; ++k) g.ref_c[k] = 0.5 * (lo[k] + hi[k]);
+    double r2 = 0.0;
+    for (int v = 0; v < g.nv; ++v) {
+      double d2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double t = g.h_verts[3 * (size_t)v + k] - g.ref_c[k];
+        d2 += t * t;
+      }
+      r2 = std::max(r2, d2);
+    }
+    g.ref_R = std::sqrt(r2) * (1.0 + 1e-12);
+  }
   upload(g.verts, g.h_verts.data(), g.h_verts.size(), s);
   upload(g.faces, g.h_faces.data(), g.h_faces.size(), s);
   upload(g.dofvid, g.h_dofvid.data(), g.h_dofvid.size(), s);
@@ -495,7 +678,7 @@ int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const
                              ctx->c.stream));
       PII_HIP(hipStreamSynchronize(ctx->c.stream));
     } else if (precision == PII_FF_FAST) {
-      b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, true);
+      b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, true, ctx->c.series_grid(kg->k));
       form_factor_fast_full(ctx->c, b, kg->k, out);
     } else {
       throw Error(PII_ERR_INVALID, "unknown form-factor precision");
@@ -521,7 +704,7 @@ int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
       b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, false);
       volume_matrix_fp64(ctx->c, b, kg->k, V);
     } else if (ff_precision == PII_FF_FAST) {
-      b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true);
+      b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
       volume_matrix_fast(ctx->c, b, kg->k, gram_mode, V);
     } else {
       throw Error(PII_ERR_INVALID, "unsupported precision / Gram mode combination");
@@ -543,7 +726,7 @@ int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* ge
     std::vector<const GeometryDev*> gs(n);
     for (int i = 0; i < n; ++i) gs[i] = &geoms[i]->g;
     Batch b;
-    b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true);
+    b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
     volume_matrix_vjp(ctx->c, b, gs.data(), kg->k, dL_dV, grads, V);
   });
 }

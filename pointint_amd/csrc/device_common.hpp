@@ -17,7 +17,12 @@ struct ObjDesc {
   int npatch;          // triangle patches
   int dof_kind;        // PII_DOF_*
   int col;             // output column
-  int pad0, pad1;
+  // Low-|k| moment series (triangles on a shell-structured k-grid): the first
+  // nser shells come from the series, representative nodes q >= qbeg = nser*Mh
+  // from the per-face kernel.
+  int nser, qbeg, pad0, pad1;
+  double ctr[3];       // expansion centre of the series (world frame)
+  double pad_d;
   const double* verts;        // nv*3 reference-frame vertices
   const int32_t* faces;       // n*3
   const int32_t* dofvid;      // nv
@@ -29,7 +34,9 @@ struct ObjDesc {
   const double* dofs;         // device DoFs (12 affine or 3V mesh)
   double* world;              // nv*3 world vertices (written by the pre-pass)
   double* rec;                // element records (written by the pre-pass)
+  double* mrec;               // n*12 moment records {v0-c, v1-c, v2-c, S} (nser > 0)
 };
+
 
 // Element record layouts written by the pre-pass.
 //   plain triangle : 12 doubles {a, e1, e2, S}            (world frame)
@@ -38,6 +45,13 @@ struct ObjDesc {
 constexpr int kRecPlainTri = 12;
 constexpr int kRecFastTri = 8;
 constexpr int kRecSurfel = 8;
+constexpr int kRecMoment = 12;
+
+// Moment series of the triangle form factor (see moments.hip): order P and the
+// largest |k| R it is used for, R the object's radius about the centre.  With
+// P = 48 and |k| R <= 12 the tail is below 1.6e-10 of the total area.
+constexpr int kSeriesOrder = 48;
+constexpr double kSeriesReach = 12.0;
 
 // Read-only data that a kernel indexes with wave-uniform indices (element
 // records, world vertices, patch tables) is read through the constant address

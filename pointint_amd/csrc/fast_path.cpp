@@ -6,6 +6,37 @@
 
 namespace pii {
 
+void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
+                              int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                              unsigned* d_colmax, Context* prof) {
+  const double* krep = kg.rep.as<double>();
+  if (prof) {
+    for (const ObjDesc& d : b.h_objs) {
+      if (d.type == PII_GEOM_TRIANGLE) {
+        prof->work[0] += (double)d.n * (kg.Qh - d.qbeg);
+        if (d.nser > 0) {
+          prof->work[1] += (double)d.n * b.Mh;
+          prof->work[2] += (double)d.qbeg;
+        }
+      } else {
+        prof->work[3] += (double)d.n * kg.Qh;
+      }
+    }
+  }
+  if (b.nseries > 0) {
+    if (prof) prof->phase_begin(4);
+    launch_series_tri(s, b.d_objs, b.N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
+                      d_colmax);
+    if (prof) prof->phase_end(4);
+    PII_HIP(hipGetLastError());
+  }
+  if (prof) prof->phase_begin(5);
+  launch_ff_fast(s, b.d_objs, b.N, krep, kg.Qh, b.qbeg_min, out_mode, d_A, ldA, d_X, ldX,
+                 d_colmax);
+  if (prof) prof->phase_end(5);
+  PII_HIP(hipGetLastError());
+}
+
 void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out_host) {
   hipStream_t s = c.stream;
   const int N = b.N, Qh = kg.Qh, Q = kg.Q;
@@ -13,8 +44,7 @@ void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, doubl
   c.w.X.ensure(sizeof(double2) * (size_t)ldA * N);
   c.w.J.ensure(sizeof(double2) * (size_t)Q * N);
   c.phase_begin(1);
-  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 0, c.w.X.as<double2>(), ldA, nullptr,
-                 0, nullptr);
+  launch_form_factors_fast(s, b, kg, 0, c.w.X.as<double2>(), ldA, nullptr, 0, nullptr, &c);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   launch_expand(s, c.w.X.as<double2>(), ldA, kg.rep_index.as<int>(), kg.anti_index.as<int>(), Qh,
@@ -35,8 +65,7 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
   unsigned* colmax = c.w.colmax.as<unsigned>();
   PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s));
   c.phase_begin(1);
-  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 1, nullptr, 0, c.w.X.as<float2>(), ldX,
-                 colmax);
+  launch_form_factors_fast(s, b, kg, 1, nullptr, 0, c.w.X.as<float2>(), ldX, colmax, &c);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   c.w.V.ensure(sizeof(double) * (size_t)N * N);
@@ -77,8 +106,7 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
   unsigned* colmax = c.w.colmax.as<unsigned>();
   PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s));
   c.phase_begin(1);
-  launch_ff_fast(s, b.d_objs, N, kg.rep.as<double>(), Qh, 1, nullptr, 0, c.w.X.as<float2>(), ldX,
-                 colmax);
+  launch_form_factors_fast(s, b, kg, 1, nullptr, 0, c.w.X.as<float2>(), ldX, colmax, &c);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   if (V_host) {

@@ -109,16 +109,16 @@ __device__ __forceinline__ float j1ox_small(float x) {
   return s * y + 0.5f;
 }
 
+// sinc(h) from h and sin(h): polynomial below |h| = kSincPoly, sin(h)/h above
+// (both evaluated, then selected, so lanes do not diverge).
 __device__ __forceinline__ float sinc_from(float h, float sin_h) {
-  const float ah = fabsf(h);
-  if (ah < kSincPoly) {
-    const float h2 = h * h;
-    float p = h2 * (1.0f / 362880.0f) - (1.0f / 5040.0f);
-    p = p * h2 + (1.0f / 120.0f);
-    p = p * h2 - (1.0f / 6.0f);
-    return p * h2 + 1.0f;
-  }
-  return sin_h * __builtin_amdgcn_rcpf(h);
+  const float h2 = h * h;
+  float p = h2 * (1.0f / 362880.0f) - (1.0f / 5040.0f);
+  p = p * h2 + (1.0f / 120.0f);
+  p = p * h2 - (1.0f / 6.0f);
+  p = p * h2 + 1.0f;
+  const float r = sin_h * __builtin_amdgcn_rcpf(h);
+  return fabsf(h) < kSincPoly ? p : r;
 }
 
 // Phi(a, b) = sum_{n<NT} c_n h_n(a, b), c_n = 2 i^n / (n+2)!, in FP64, times
@@ -144,28 +144,35 @@ __device__ __forceinline__ float2 tri_taylor(double al, double be, float g2, flo
 }
 
 // Main branch: middle node m and the other two (p, q); see the file comment.
+// The node choice is made with selects.  Half spans and the full span are
+// formed in FP32 from the FP32-rounded alpha, beta: their absolute error,
+// 6e-8 |k| diam, changes the result by a relative 6e-8 at most, because the
+// span D itself is at least that large here.
 __device__ __forceinline__ float2 tri_main(double al, double be, float g2, float2 Q0, float2 Q1,
                                            float2 Q2) {
+  const float a = (float)al, b = (float)be;
+  // relative phases u0 = 0, u1 = a, u2 = b; m is the middle one
   const bool m0 = (al * be <= 0.0);
-  const bool m1 = !m0 && (fabs(al) < fabs(be));
-  float2 Qm, Qp, Qq;
-  double hp, hq, D;
-  if (m0) {
-    Qm = Q0; Qp = Q1; Qq = Q2; hp = -0.5 * al; hq = 0.5 * be; D = be - al;
-  } else if (m1) {
-    Qm = Q1; Qp = Q0; Qq = Q2; hp = 0.5 * al; hq = 0.5 * (be - al); D = be;
-  } else {
-    Qm = Q2; Qp = Q0; Qq = Q1; hp = 0.5 * be; hq = 0.5 * (al - be); D = al;
-  }
+  const bool m1 = (!m0) & (fabsf(a) < fabsf(b));
+  const bool m01 = m0 | m1;
+  const float um = m0 ? 0.0f : (m1 ? a : b);
+  const float up = m0 ? a : 0.0f;
+  const float uq = m01 ? b : a;
+  const float2 Qm = make_float2(m0 ? Q0.x : (m1 ? Q1.x : Q2.x), m0 ? Q0.y : (m1 ? Q1.y : Q2.y));
+  const float2 Qp = make_float2(m0 ? Q1.x : Q0.x, m0 ? Q1.y : Q0.y);
+  const float2 Qq = make_float2(m01 ? Q2.x : Q1.x, m01 ? Q2.y : Q1.y);
+  const float hp = 0.5f * (um - up);
+  const float hq = 0.5f * (uq - um);
+  const float D = uq - up;
   const float sin_p = Qm.y * Qp.x - Qm.x * Qp.y;   // Im(Qm conj Qp) = sin(hp)
   const float sin_q = Qq.y * Qm.x - Qq.x * Qm.y;   // Im(Qq conj Qm) = sin(hq)
-  const float sp = sinc_from((float)hp, sin_p);
-  const float sq = sinc_from((float)hq, sin_q);
+  const float sp = sinc_from(hp, sin_p);
+  const float sq = sinc_from(hq, sin_q);
   const float nx = Qq.x * sq - Qp.x * sp;
   const float ny = Qq.y * sq - Qp.y * sp;
   const float tx = Qm.x * nx - Qm.y * ny;
   const float ty = Qm.x * ny + Qm.y * nx;
-  const float f = g2 * __builtin_amdgcn_rcpf((float)D);
+  const float f = g2 * __builtin_amdgcn_rcpf(D);
   // -i * (tx + i ty) = ty - i tx
   return make_float2(ty * f, -tx * f);
 }
@@ -230,7 +237,15 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
   // wave index as an SGPR value: loop indices derived from it stay uniform,
   // so element records are fetched with scalar loads
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int q = blockIdx.x * kLanes + lane;
+  // This object's nodes start at qbeg (the moment series wrote those below).
+  // Both exits are uniform over the workgroup.
+  const int q0 = d.qbeg + blockIdx.x * kLanes;
+  if (q0 >= (out_mode == 0 ? Qh : ldX)) return;
+  const int q = q0 + lane;
+  if (q0 >= Qh) {  // padding columns of X only
+    if (wave == 0 && q < ldX) Xout[(size_t)d.col * ldX + q] = make_float2(0.f, 0.f);
+    return;
+  }
   const int qc = q < Qh ? q : Qh - 1;
   const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
   const float hx = (float)krep[3 * (size_t)Qh + qc], hy = (float)krep[4 * (size_t)Qh + qc],
@@ -395,11 +410,12 @@ __global__ void __launch_bounds__(256) k_expand(const double2* __restrict__ Arep
 }  // namespace
 
 void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
-                    int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                    int qbeg_min, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                     unsigned* d_colmax) {
-  if (N <= 0 || Qh <= 0) return;
-  hipLaunchKernelGGL(k_ff_fast, dim3((Qh + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves), 0, s,
-                     d_objs, d_krep, Qh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+  const int span = (out_mode == 0 ? Qh : ldX) - qbeg_min;
+  if (N <= 0 || Qh <= 0 || span <= 0) return;
+  hipLaunchKernelGGL(k_ff_fast, dim3((span + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves), 0,
+                     s, d_objs, d_krep, Qh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
 }
 
 void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,

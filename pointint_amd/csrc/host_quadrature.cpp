@@ -3,10 +3,12 @@
 // builder used by the shared-vertex form-factor kernel.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "pii_internal.hpp"
@@ -168,6 +170,61 @@ bool kgrid_pairs(int Q, const double* dirs, const double* kmag, const double* w,
   for (int q = 0; q < Q; ++q)
     if (q < anti[q]) rep.push_back(q);
   return true;
+}
+
+// Morton key of a unit direction on the octahedral map (full sphere; the
+// lower hemisphere folds onto the square's corners), 16 bits per axis.
+static uint32_t direction_key(const double* d) {
+  const double n = std::abs(d[0]) + std::abs(d[1]) + std::abs(d[2]);
+  double u = n > 0 ? d[0] / n : 0.0, v = n > 0 ? d[1] / n : 0.0;
+  if (d[2] < 0) {
+    const double fu = (1.0 - std::abs(v)) * (u >= 0 ? 1.0 : -1.0);
+    const double fv = (1.0 - std::abs(u)) * (v >= 0 ? 1.0 : -1.0);
+    u = fu;
+    v = fv;
+  }
+  auto q16 = [](double x) {
+    const double t = std::min(1.0, std::max(0.0, 0.5 * (x + 1.0)));
+    return (uint32_t)std::min(65535.0, std::floor(t * 65536.0));
+  };
+  uint32_t a = q16(u), b = q16(v), key = 0;
+  for (int bit = 0; bit < 16; ++bit) {
+    key |= ((a >> bit) & 1u) << (2 * bit);
+    key |= ((b >> bit) & 1u) << (2 * bit + 1);
+  }
+  return key;
+}
+
+// Representatives of a paired grid: the member of each pair on the upper
+// hemisphere (z > 0, then y > 0, then x > 0), ordered inside each shell along
+// the Morton curve of the direction, so that the 64 nodes of a wave lie close
+// together on the sphere.  Unpaired grids keep every node, ordered the same
+// way.  The order depends only on the directions, so shells with the same
+// direction set get the same order.
+void kgrid_order_reps(int Q, const double* dirs, const double* kmag, const std::vector<int>& anti,
+                      std::vector<int>& rep) {
+  auto upper = [&](int q) {
+    const double* d = dirs + 3 * (size_t)q;
+    if (d[2] != 0.0) return d[2] > 0.0;
+    if (d[1] != 0.0) return d[1] > 0.0;
+    return d[0] >= 0.0;
+  };
+  for (int& q : rep)
+    if (anti[q] >= 0 && !upper(q)) q = anti[q];
+  size_t beg = 0;
+  while (beg < rep.size()) {
+    size_t end = beg + 1;
+    while (end < rep.size() && kmag[rep[end]] == kmag[rep[beg]]) ++end;
+    std::vector<std::pair<uint32_t, int>> keyed(end - beg);
+    for (size_t i = beg; i < end; ++i)
+      keyed[i - beg] = {direction_key(dirs + 3 * (size_t)rep[i]), rep[i]};
+    // equal keys fall back to the node index (shell-major, so the same order
+    // in every shell of a product grid)
+    std::sort(keyed.begin(), keyed.end());
+    for (size_t i = beg; i < end; ++i) rep[i] = keyed[i - beg].second;
+    beg = end;
+  }
+  (void)Q;
 }
 
 // Greedy patch builder.  Faces are visited in breadth-first order over

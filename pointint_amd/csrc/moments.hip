@@ -1,0 +1,177 @@
+// Low-|k| triangle form factors from a moment series.
+//
+// The reference sums, per face f with area vector S_f, corners v_f0..v_f2 and
+// k = kappa d (cpp/src/dof/triangle_mesh_dof.cpp:36-71):
+//
+//   A(k) = sum_f (d.S_f) e^{i k.v_f0} Phi(k.e1, k.e2)
+//        = sum_f (d.S_f) / area_f * int_f e^{i k.x} dS.
+//
+// Expanding e^{i k.x} about a centre c and using, for a linear function z on
+// a triangle, int_f z^n dS = area_f * 2 n! / (n+2)! * h_n(z0, z1, z2) (h_n the
+// complete homogeneous symmetric polynomial of the corner values):
+//
+//   A(kappa d) = e^{i kappa d.c} sum_{n>=0} (i kappa)^n 2/(n+2)! H_n(d),
+//   H_n(d)     = sum_f (d.S_f) h_n(w_f0, w_f1, w_f2),   w_fj = d.(v_fj - c).
+//
+// H_n depends on the direction only, so one pass over the faces per direction
+// serves every shell of the grid with kappa R <= kSeriesReach (R bounds
+// |v - c|).  Each term is bounded by Area (kappa R)^n / n!, so the tail after
+// P = kSeriesOrder terms is below Area X^(P+1)/(P+1)! / (1 - X/(P+2)) at
+// X = kappa R: 1.6e-10 Area for P = 48, X = 12.  The largest term, about
+// Area e^X / sqrt(2 pi X), keeps FP64 rounding below 1e-11 Area.
+//
+// h_n by the recurrences (all FP64, 4 operations per order and face)
+//   b_n = w1^n,  h_n(w0, w1) = w0 h_{n-1}(w0, w1) + b_n,
+//   h_n(w0, w1, w2) = w2 h_{n-1}(w0, w1, w2) + h_n(w0, w1).
+#include "device_common.hpp"
+#include "pii_internal.hpp"
+
+namespace pii {
+
+namespace {
+
+constexpr int P = kSeriesOrder;
+constexpr int kLanes = 64;
+constexpr int kWaves = 4;
+
+// c_n = (-1)^floor(n/2) 2/(n+2)!: the real factor of (i)^n 2/(n+2)! (the i
+// of odd n is applied after the Horner sums).
+struct SeriesCoef {
+  double c[P + 1];
+  constexpr SeriesCoef() : c() {
+    double f = 1.0;  // 2/(n+2)!, from n = 0: 2/2! = 1
+    for (int n = 0; n <= P; ++n) {
+      const double sign = ((n / 2) % 2 == 0) ? 1.0 : -1.0;
+      c[n] = sign * f;
+      f = f / (double)(n + 3);
+    }
+  }
+};
+__device__ constexpr SeriesCoef kSeries{};
+
+// One workgroup per (object, 64 directions): the four waves take every fourth
+// face, their H_n are summed through LDS, then the waves evaluate the series
+// at every fourth shell.  out_mode 0: A (FP64) to Aout[col*ldA + q]; 1:
+// X = sqrt(w) A (FP32) to Xout[col*ldX + q] and the column max |X| to
+// colmax[col].
+__global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
+    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
+    int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
+    unsigned* __restrict__ colmax) {
+  __shared__ double red[2][P + 1][kLanes];
+  const ObjDesc& d = objs[blockIdx.y];
+  const int nser = d.nser;
+  if (nser <= 0) return;  // uniform over the workgroup
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j = blockIdx.x * kLanes + lane;
+  const int jc = j < Mh ? j : Mh - 1;
+  const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
+               dz = krep[5 * (size_t)Qh + jc];
+  double H[P + 1];
+#pragma unroll
+  for (int n = 0; n <= P; ++n) H[n] = 0.0;
+  const auto* rec = as_const(d.mrec);
+  const int nf = d.n;
+  // Projections of a face record: corners relative to the centre and the
+  // area vector onto this lane's direction.
+  auto project = [&](const auto* r, double& w0, double& w1, double& w2, double& s) {
+    w0 = dx * r[0] + dy * r[1] + dz * r[2];
+    w1 = dx * r[3] + dy * r[4] + dz * r[5];
+    w2 = dx * r[6] + dy * r[7] + dz * r[8];
+    s = dx * r[9] + dy * r[10] + dz * r[11];
+  };
+  double w0, w1, w2, s;
+  project(rec + (size_t)kRecMoment * (wave < nf ? wave : nf - 1), w0, w1, w2, s);
+  for (int f = wave; f < nf; f += kWaves) {
+    const double c0 = w0, c1 = w1, c2 = w2, cs = s;
+    // the next face's record loads are issued before the sums of this face
+    // and consumed after them (index clamped to the object)
+    const int fn = f + kWaves < nf ? f + kWaves : nf - 1;
+    const auto* rn = rec + (size_t)kRecMoment * fn;
+    double t[kRecMoment];
+#pragma unroll
+    for (int i = 0; i < kRecMoment; ++i) t[i] = rn[i];
+    // keep the loads issued here: the scheduler would otherwise sink them to
+    // their use, exposing their latency
+    __builtin_amdgcn_sched_barrier(0);
+    double b = 1.0, hab = 1.0, habc = 1.0;
+    H[0] += cs;
+#pragma unroll
+    for (int n = 1; n <= P; ++n) {
+      b *= c1;
+      hab = fma(c0, hab, b);
+      habc = fma(c2, habc, hab);
+      H[n] = fma(cs, habc, H[n]);
+    }
+    project(t, w0, w1, w2, s);
+  }
+  // (H0 + H2) + (H1 + H3), in a fixed order
+  if (wave >= 2) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) red[wave - 2][n][lane] = H[n];
+  }
+  __syncthreads();
+  if (wave < 2) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) H[n] += red[wave][n][lane];
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) red[0][n][lane] = H[n];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) red[1][n][lane] = (H[n] + red[0][n][lane]) * kSeries.c[n];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n <= P; ++n) H[n] = red[1][n][lane];
+  const double dc = dx * d.ctr[0] + dy * d.ctr[1] + dz * d.ctr[2];
+  float m = 0.0f;
+  for (int s = wave; s < nser; s += kWaves) {
+    const int q = s * Mh + jc;
+    const double kap = krep[6 * (size_t)Qh + q];
+    const double k2 = kap * kap;
+    // even part E = sum a_{2t} kappa^{2t}, odd part O = sum a_{2t+1} kappa^{2t}
+    constexpr int ne = P / 2, no = (P - 1) / 2;
+    double E = H[2 * ne], O = H[2 * no + 1];
+#pragma unroll
+    for (int t = ne - 1; t >= 0; --t) E = fma(E, k2, H[2 * t]);
+#pragma unroll
+    for (int t = no - 1; t >= 0; --t) O = fma(O, k2, H[2 * t + 1]);
+    O *= kap;
+    double sn, cs;
+    sincos(kap * dc, &sn, &cs);
+    double ax = cs * E - sn * O, ay = sn * E + cs * O;
+    if (kap < 1e-10) { ax = 0.0; ay = 0.0; }  // A(0) = 0 (triangle_mesh_dof.cpp:28)
+    if (out_mode == 0) {
+      if (j < Mh) Aout[(size_t)d.col * ldA + q] = make_double2(ax, ay);
+    } else {
+      const double sw = sqrt(krep[7 * (size_t)Qh + q]);
+      const float2 x = make_float2((float)(ax * sw), (float)(ay * sw));
+      if (j < Mh) {
+        Xout[(size_t)d.col * ldX + q] = x;
+        m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+      }
+    }
+  }
+  if (out_mode == 1) {
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if (lane == 0) atomicMax(colmax + d.col, __float_as_uint(m));
+  }
+}
+
+}  // namespace
+
+void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
+                       int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                       unsigned* d_colmax) {
+  if (N <= 0 || Mh <= 0) return;
+  hipLaunchKernelGGL(k_series_tri, dim3((Mh + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves), 0,
+                     s, d_objs, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+}
+
+}  // namespace pii

@@ -59,14 +59,28 @@ struct Batch {
   const int* d_vprefix = nullptr;
   const int* d_eprefix = nullptr;
   int total_v = 0, total_e = 0;
-  // prof (may be null) records the pre-pass phase with HIP events.
+  // Moment series plan (fast path, shell-structured grid): objects with a
+  // series, directions per shell, and the smallest first node of the
+  // per-face kernel over all objects.
+  int nseries = 0, Mh = 0, qbeg_min = 0;
+  // prof (may be null) records the pre-pass phase with HIP events.  series_kg
+  // (fast path only, may be null) enables the moment series on that grid.
   void build(Workspace& w, hipStream_t s, Context* prof, int n, const GeometryDev* const* geoms,
-             const int* kinds, const double* const* dofs, const int* ndofs, bool fast);
+             const int* kinds, const double* const* dofs, const int* ndofs, bool fast,
+             const KGridDev* series_kg = nullptr);
   void build(Context& c, int n, const GeometryDev* const* geoms, const int* kinds,
-             const double* const* dofs, const int* ndofs, bool fast) {
-    build(c.w, c.stream, &c, n, geoms, kinds, dofs, ndofs, fast);
+             const double* const* dofs, const int* ndofs, bool fast,
+             const KGridDev* series_kg = nullptr) {
+    build(c.w, c.stream, &c, n, geoms, kinds, dofs, ndofs, fast, series_kg);
   }
 };
+
+// Fast form factors of a batch: the moment series for the leading shells of
+// triangle objects, the per-face kernel for the remaining nodes.  prof (may be
+// null) records the two kernels' phases and the work counters.
+void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
+                              int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                              unsigned* d_colmax, Context* prof);
 
 // Fast path (FP32 transcendentals, FP64 phases and Taylor branch).
 void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out_host);

@@ -13,6 +13,8 @@ void kgrid_build(const double* ldirs, const double* lw, int M, int nrad, double*
                  double* kmag, double* w);
 bool kgrid_pairs(int Q, const double* dirs, const double* kmag, const double* w,
                  std::vector<int>& anti, std::vector<int>& rep);
+void kgrid_order_reps(int Q, const double* dirs, const double* kmag, const std::vector<int>& anti,
+                      std::vector<int>& rep);
 void build_patches(int nv, int nf, const int32_t* F, std::vector<int>& hdr,
                    std::vector<int>& pverts, std::vector<uint32_t>& pfaces,
                    std::vector<uint32_t>& ploc);

@@ -83,7 +83,14 @@ struct KGridDev {
   DevBuf rep;                   // 8*Qh doubles
   DevBuf rep_index;             // Qh int32: rep -> full index
   DevBuf anti_index;            // Q int32: full -> antipode
+  // Shell structure of the representative grid (0 when absent): Qh = nshell*Mh
+  // and node s*Mh + j has magnitude h_shell_k[s] and the direction of node j.
+  int nshell = 0, Mh = 0;
+  std::vector<double> h_shell_k;
 };
+
+// Detects the shell structure of a k-grid's representative nodes.
+void kgrid_shells(KGridDev& k);
 
 // ---------------------------------------------------------------------------
 // Geometry resident in HBM.
@@ -102,6 +109,8 @@ struct GeometryDev {
   std::vector<double> h_verts;       // nv*3 reference-frame vertex positions
   std::vector<int32_t> h_faces;      // n*3 indices into h_verts
   std::vector<int32_t> h_dofvid;     // nv: geometry vertex -> DoF vertex (or -1)
+  double ref_c[3] = {0, 0, 0};       // bounding-box centre of the reference vertices
+  double ref_R = 0.0;                // max |v - ref_c| over the reference vertices
   int ndof_verts = 0;                // DoF vertex count for TRIMESH DoFs
   DevBuf verts;                      // nv*3 FP64 (AoS)
   DevBuf faces;                      // n*3 int32
@@ -119,7 +128,7 @@ struct GeometryDev {
 
 // Per-call device workspaces (grow-only, reused across calls on one stream).
 struct Workspace {
-  DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw;
+  DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw, mrec;
 };
 
 struct Context {
@@ -127,14 +136,31 @@ struct Context {
   hipStream_t stream = nullptr;
   std::mutex mu;                     // serialises calls on one context
   Workspace w;
+  // Moment series for low-|k| shells of triangle objects on the fast path.
+  bool series = true;
+  const KGridDev* series_grid(const KGridDev& k) const {
+    return series ? &k : nullptr;
+  }
   // Phase timing with HIP events on the context stream (enable_profiling of
   // the reference drivers, e.g. compute_intersection_volume_multi_object.cu:626-695).
   bool profiling = false;
-  static constexpr int kPhases = 4;   // 0 pre-pass, 1 form factor, 2 Gram, 3 adjoint
+  // 0 pre-pass, 1 form factor, 2 Gram, 3 adjoint; inside 1: 4 moment series
+  // kernel, 5 per-face kernel.
+  static constexpr int kPhases = 6;
   hipEvent_t ev[kPhases][2] = {};
   bool ev_used[kPhases] = {};
   void phase_begin(int p);
   void phase_end(int p);
+  // Work of the last call's form-factor kernels: [0] triangle x node
+  // evaluations of the per-face kernel, [1] face x direction passes of the
+  // moment series, [2] object x node series evaluations, [3] disk/Gaussian x
+  // node evaluations.
+  static constexpr int kWork = 4;
+  double work[kWork] = {};
+  void begin_call() {
+    for (bool& u : ev_used) u = false;
+    for (double& x : work) x = 0.0;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -165,9 +191,16 @@ void launch_gram_fp64_x(hipStream_t s, const float2* d_X, int Qh, int N, int ldX
 // Fast form factor over the representative nodes.  out_mode 0: A (FP64) to
 // d_A[col*ldA + q]; out_mode 1: X = sqrt(w) A (FP32) to d_X[col*ldX + q] for
 // q < ldX (zeros beyond Qh) and the column max |X| to d_colmax[col].
+// Each object starts at its own representative node ObjDesc::qbeg (nodes
+// below come from the moment series); qbeg_min is the smallest over objects.
 void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
-                    int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                    int qbeg_min, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                     unsigned* d_colmax);
+// Moment series of triangle objects at their first nser shells (objects with
+// nser = 0 are skipped); outputs as launch_ff_fast.
+void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
+                       int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                       unsigned* d_colmax);
 // Full-grid values from representative values: A(-k) = -conj(A(k)).
 void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,
                    const int* d_anti, int Qh, int Q, int N, double2* d_Afull);

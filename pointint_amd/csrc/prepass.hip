@@ -39,7 +39,8 @@ __global__ void __launch_bounds__(256) k_world_vertices(const ObjDesc* __restric
   d.world[3 * v + 2] = w2;
 }
 
-// Element records.  fast != 0 writes the patch-ordered triangle layout.
+// Element records.  fast != 0 writes the patch-ordered triangle layout and,
+// for objects with a moment series, the moment records.
 __global__ void __launch_bounds__(256) k_records(const ObjDesc* __restrict__ objs,
                                                  const int* __restrict__ eprefix, int N,
                                                  int total, int fast) {
@@ -84,6 +85,15 @@ __global__ void __launch_bounds__(256) k_records(const ObjDesc* __restrict__ obj
       int code = 0;
       if (diam > 0.0) code = min(127, max(-64, (int)ceil(log2(diam * (1.0 + 1e-12))))) + 64;
       reinterpret_cast<uint32_t*>(rf)[3] = (d.patch_floc[e] & 0xffffffu) | ((uint32_t)code << 24);
+      if (d.nser > 0) {
+        // corners relative to the series centre, and the area vector
+        double* m = d.mrec + (size_t)kRecMoment * e;
+        const double c0 = d.ctr[0], c1 = d.ctr[1], c2 = d.ctr[2];
+        m[0] = ax - c0; m[1] = ay - c1; m[2] = az - c2;
+        m[3] = W[3 * i1] - c0; m[4] = W[3 * i1 + 1] - c1; m[5] = W[3 * i1 + 2] - c2;
+        m[6] = W[3 * i2] - c0; m[7] = W[3 * i2 + 1] - c1; m[8] = W[3 * i2 + 2] - c2;
+        m[9] = sx; m[10] = sy; m[11] = sz;
+      }
     }
   } else {
     // cpp/src/dof/affine_dof.cpp:115-118: c' = A c + t; k'.n = (A^T k).n = k.(A n)

@@ -70,6 +70,7 @@ struct Server {
   int device = 0;
   const KGridDev* kg = nullptr;
   int gram_mode = PII_GRAM_FP16X3;
+  bool series = true;                 // moment series (the creating context's setting)
   size_t budget = 0;
   size_t reserved = 0;
   int max_batch_objects = 1024;
@@ -94,6 +95,7 @@ struct Server {
     b += (size_t)n * n * sizeof(double) * (nsplit + 1);      // partials + V
     for (const GeometryDev* g : gs) {
       b += (size_t)g->n * sizeof(double) * kRecFastTri;       // element records
+      if (series) b += (size_t)g->n * sizeof(double) * kRecMoment;  // moment records
       b += (size_t)g->nv * 3 * sizeof(double);                // world vertices
     }
     return b + 4096;
@@ -114,15 +116,16 @@ struct Server {
     }
     const int N = (int)gs.size();
     s.batch = Batch();
-    s.batch.build(s.w, s.stream, nullptr, N, gs.data(), kinds.data(), dp.data(), nd.data(), true);
+    s.batch.build(s.w, s.stream, nullptr, N, gs.data(), kinds.data(), dp.data(), nd.data(), true,
+                  series ? kg : nullptr);
     const int Qh = kg->Qh;
     const int ldX = (int)align_up((size_t)Qh, 64);
     s.w.X.ensure(sizeof(float2) * (size_t)ldX * N);
     s.w.colmax.ensure(sizeof(unsigned) * (size_t)N);
     unsigned* colmax = s.w.colmax.as<unsigned>();
     PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s.stream));
-    launch_ff_fast(s.stream, s.batch.d_objs, N, kg->rep.as<double>(), Qh, 1, nullptr, 0,
-                   s.w.X.as<float2>(), ldX, colmax);
+    launch_form_factors_fast(s.stream, s.batch, *kg, 1, nullptr, 0, s.w.X.as<float2>(), ldX,
+                             colmax, nullptr);
     PII_HIP(hipGetLastError());
     // One Gram per query over its column range.
     size_t vtot = 0, maxpart = 0;
@@ -260,6 +263,7 @@ int pii_server_create(pii_context* ctx, const pii_kgrid* kg, int gram_mode, doub
     Server& S = srv->s;
     S.device = ctx->c.device;
     S.kg = &kg->k;
+    S.series = ctx->c.series;
     S.gram_mode = gram_mode;
     S.budget = (size_t)hbm_budget;
     S.max_batch_objects = max_batch_objects;
