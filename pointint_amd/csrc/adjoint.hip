@@ -163,7 +163,8 @@ __global__ void __launch_bounds__(128) k_adjoint_tri(const ObjDesc* __restrict__
   if (p >= d.npatch) return;
   const int vbeg = d.patch_hdr[4 * p], vcnt = d.patch_hdr[4 * p + 1];
   const int fbeg = d.patch_hdr[4 * p + 2], fcnt = d.patch_hdr[4 * p + 3];
-  const int kbeg = blockIdx.y * kchunk;
+  // nodes below qbeg come from the moment series (adjoint_series.hip)
+  const int kbeg = d.qbeg + blockIdx.y * kchunk;
   const int kend = min(Qh, kbeg + kchunk);
   if (kbeg >= kend) return;
   const int t = threadIdx.x;
@@ -317,10 +318,11 @@ void launch_mix_columns(hipStream_t s, const float2* d_X, int N, int ldX, int Qh
 }
 
 void launch_adjoint_tri(hipStream_t s, const ObjDesc* d_objs, int nobj, int max_patches,
-                        const double* d_krep, int Qh, const float2* d_Bw, int ldB, int ksplit,
-                        double* d_gworld, const int* d_goff) {
-  if (nobj <= 0 || max_patches <= 0 || Qh <= 0) return;
-  int kchunk = (Qh + ksplit - 1) / ksplit;
+                        const double* d_krep, int Qh, int qbeg_min, const float2* d_Bw, int ldB,
+                        int ksplit, double* d_gworld, const int* d_goff) {
+  const int span = Qh - qbeg_min;
+  if (nobj <= 0 || max_patches <= 0 || span <= 0) return;
+  int kchunk = (span + ksplit - 1) / ksplit;
   kchunk = (kchunk + kKB - 1) / kKB * kKB;
   hipLaunchKernelGGL(k_adjoint_tri, dim3(max_patches, ksplit, nobj), dim3(128), 0, s, d_objs,
                      d_krep, Qh, d_Bw, ldB, kchunk, d_gworld, d_goff);

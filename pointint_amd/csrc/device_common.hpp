@@ -52,6 +52,10 @@ constexpr int kRecMoment = 12;
 // P = 48 and |k| R <= 12 the tail is below 1.6e-10 of the total area.
 constexpr int kSeriesOrder = 48;
 constexpr double kSeriesReach = 12.0;
+// Per (object, direction) record of the series adjoint: Gamma_0..P, then
+// (m+3) Gamma_{m+1} for m < P; padded to a multiple of 8 doubles.
+constexpr int kGammaStride = 104;
+static_assert(kGammaStride >= 2 * kSeriesOrder + 1, "Gamma record too short");
 
 // Read-only data that a kernel indexes with wave-uniform indices (element
 // records, world vertices, patch tables) is read through the constant address

@@ -140,11 +140,19 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
   PII_HIP(hipMemsetAsync(d_gw, 0, sizeof(double) * std::max(goff[N], 1), s));
   c.phase_begin(3);
   launch_mix_columns(s, c.w.X.as<float2>(), N, ldX, Qh, c.w.C.as<double>(), c.w.B.as<float2>());
+  if (b.nseries > 0) {
+    int max_faces = 0;
+    for (const ObjDesc& d : b.h_objs)
+      if (d.nser > 0) max_faces = std::max(max_faces, d.n);
+    c.w.gam.ensure(sizeof(double) * (size_t)N * b.Mh * kGammaStride);
+    launch_series_adjoint_tri(s, b.d_objs, N, max_faces, kg.rep.as<double>(), Qh, b.Mh,
+                              c.w.B.as<float2>(), ldX, c.w.gam.as<double>(), d_gw, d_goff);
+  }
   // enough workgroups to fill the chip: split the k-range when objects are few
   const int wgs = std::max(1, max_patches * N);
   const int ksplit = std::max(1, std::min(64, (4096 + wgs - 1) / wgs));
-  launch_adjoint_tri(s, b.d_objs, N, max_patches, kg.rep.as<double>(), Qh, c.w.B.as<float2>(),
-                     ldX, ksplit, d_gw, d_goff);
+  launch_adjoint_tri(s, b.d_objs, N, max_patches, kg.rep.as<double>(), Qh, b.qbeg_min,
+                     c.w.B.as<float2>(), ldX, ksplit, d_gw, d_goff);
   c.phase_end(3);
   PII_HIP(hipGetLastError());
   std::vector<double> gw(std::max(goff[N], 1));

@@ -128,7 +128,7 @@ struct GeometryDev {
 
 // Per-call device workspaces (grow-only, reused across calls on one stream).
 struct Workspace {
-  DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw, mrec;
+  DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw, mrec, gam;
 };
 
 struct Context {
@@ -215,8 +215,14 @@ void launch_gram_split(hipStream_t s, int fp16, const float* d_X, int N, int ldK
 // d_gworld + d_goff[slot] (FP64, zeroed by the caller, atomically accumulated).
 void launch_mix_columns(hipStream_t s, const float2* d_X, int N, int ldX, int Qh,
                         const double* d_C, float2* d_Y);
+// Nodes q >= ObjDesc::qbeg of each object (qbeg_min: smallest over objects).
 void launch_adjoint_tri(hipStream_t s, const ObjDesc* d_objs, int nobj, int max_patches,
-                        const double* d_krep, int Qh, const float2* d_Bw, int ldB, int ksplit,
-                        double* d_gworld, const int* d_goff);
+                        const double* d_krep, int Qh, int qbeg_min, const float2* d_Bw, int ldB,
+                        int ksplit, double* d_gworld, const int* d_goff);
+// Adjoint of the moment series (nodes below qbeg of objects with nser > 0);
+// d_gam: N * Mh * kGammaStride doubles of scratch.
+void launch_series_adjoint_tri(hipStream_t s, const ObjDesc* d_objs, int N, int max_faces,
+                               const double* d_krep, int Qh, int Mh, const float2* d_Bw, int ldB,
+                               double* d_gam, double* d_gworld, const int* d_goff);
 
 }  // namespace pii

@@ -115,3 +115,52 @@ def test_series_volume_matrix(oracle, grid):
                 assert e < 1e-5, (on, gram, e)
         finally:
             ctx.set_series(True)
+
+
+def _oracle_vjp(oracle, kg, ogeoms, xs, G, kind):
+    J = np.stack([oracle.compute_A_grid(g, kg.dirs, kg.kmag, x, kind) for g, x in zip(ogeoms, xs)],
+                 axis=1)
+    C = G + G.T
+    return [oracle.vjp(ogeoms[i], xs[i], kind, kg.dirs, kg.kmag, kg.w, J @ C[:, i])
+            for i in range(len(ogeoms))]
+
+
+@pytest.mark.parametrize("kind", ["trimesh", "affine"])
+def test_series_adjoint(oracle, kind):
+    """Vertex / affine gradients of L = sum G_ij V_ij with the series adjoint
+    (leading shells) and with the per-face adjoint everywhere, against the
+    oracle's VJP of the reference gradient: ||g - g64|| / ||g64|| < 1e-5."""
+    rng = np.random.default_rng(31)
+    d, w = oracle.load_lebedev(P.lebedev_path(9))
+    dirs, kmag, wq = oracle.build_kgrid(d, w, 24)
+    kg = P.KGrid(dirs, kmag, wq)
+    Vi, Fi = W.icosphere(2)
+    verts = [W.star_perturb(Vi, rng, 0.1) * rng.uniform(0.3, 0.45) + rng.uniform(0, 0.6, size=3)
+             for _ in range(4)]
+    geoms = [P.make_triangle_mesh(v, Fi) for v in verts]
+    ogeoms = [oracle.geom_mesh(v, Fi) for v in verts]
+    if kind == "trimesh":
+        xs = [v.reshape(-1) for v in verts]
+        okind = oracle.DOF_MESH
+    else:
+        xs = [np.concatenate([rng.normal(size=3) * 0.1,
+                              (W.random_rotation(rng) @ np.diag(rng.uniform(0.8, 1.2, 3))).ravel()])
+              for _ in verts]
+        okind = oracle.DOF_AFFINE
+    G = rng.normal(size=(4, 4))
+    ref = _oracle_vjp(oracle, kg, ogeoms, xs, G, okind)
+    ctx = P.Context.default()
+    out = {}
+    for on in (True, False):
+        ctx.set_series(on)
+        try:
+            out[on], _ = P.volume_matrix_vjp(geoms, kg, G, xs, [kind] * 4)
+        finally:
+            ctx.set_series(True)
+    nser = (kmag[None, :] * np.array([radius(v) for v in verts])[:, None] <= REACH).sum(1)
+    assert nser.min() >= 4 * len(d) and nser.max() < len(kmag), nser  # both paths run
+    for on in (True, False):
+        for g, r in zip(out[on], ref):
+            e = np.linalg.norm(g - r) / np.linalg.norm(r)
+            print(f"{kind} series={on}: rel err {e:.2e}")
+            assert e < 1e-5, (kind, on, e)
