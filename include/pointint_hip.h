@@ -104,15 +104,18 @@ int pii_context_timings(pii_context* ctx, double* ms, int n);
  * no reference counterpart): units[0] triangle x node evaluations of the
  * per-face kernel, [1] face x direction passes of the moment series, [2]
  * object x node evaluations of the series, [3] disk/Gaussian x node
- * evaluations.  Entries beyond 3 are set to 0. */
+ * evaluations of the per-element kernel, [4] disk/Gaussian x direction passes
+ * of the moment series.  Entries beyond 4 are set to 0. */
 int pii_context_work(pii_context* ctx, double* units, int n);
-/* Low-|k| moment series of the fast path (default on).  With it, triangle
- * objects on a shell-structured k-grid (every shell repeats one direction
- * set, as pii_kgrid_build makes) take their form factor at the leading shells
- * with |k| R <= 12 (R the object's radius about its bounding-box centre) from
- * a 48-term moment expansion, and the per-face evaluation only above.  No
- * reference counterpart (the reference evaluates every face at every node);
- * switching it off gives the per-face path everywhere. */
+/* Low-|k| moment series of the fast path (default on).  With it, objects on a
+ * shell-structured k-grid (every shell repeats one direction set, as
+ * pii_kgrid_build makes) take their form factor at the leading shells with
+ * |k| R <= 12 (R the object's radius about its bounding-box centre; for disks
+ * and Gaussians also a bound on rho |k| or sigma |k|) from a 48-term moment
+ * expansion in FP64, and the per-element evaluation only above; the adjoint
+ * follows the same split.  No reference counterpart (the reference evaluates
+ * every element at every node); switching it off gives the per-element path
+ * everywhere. */
 int pii_context_set_series(pii_context* ctx, int enable);
 
 /* ---- k-grid resident in HBM ----------------------------------------------- */

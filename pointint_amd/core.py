@@ -73,10 +73,10 @@ class Context:
 
     def work(self) -> dict:
         """Work of the last call's form-factor kernels (evaluation counts)."""
-        u = np.zeros(4)
-        N.check(N.load().pii_context_work(self._h, N.ptr(u), 4))
+        u = np.zeros(5)
+        N.check(N.load().pii_context_work(self._h, N.ptr(u), 5))
         return {"tri_node": u[0], "series_face_dir": u[1], "series_node": u[2],
-                "surfel_node": u[3]}
+                "surfel_node": u[3], "series_surfel_dir": u[4]}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -227,10 +227,13 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   w.world.ensure(std::max<size_t>(total_w, 1) * sizeof(double));
   w.faces.ensure(std::max<size_t>(total_rec, 1) * sizeof(double));
   // Moment series plan: leading shells with |k| R <= kSeriesReach, when at
-  // least kSeriesMinShells of them (below that the per-face kernel is cheaper
-  // than one pass over the faces per direction).
-  constexpr int kSeriesMinShells = 4;
+  // least kSeriesMinShells of them (below that the per-element kernels are
+  // cheaper than one pass over the elements per direction).  Disks and
+  // Gaussians also keep |k|^2 |A|_F^2 rho^2 / 4 <= 3 (sigma^2 / 2 <= 0.49),
+  // where ten terms of the J1 (exponential) factor reach 1e-11.
+  constexpr int kSeriesMinShells = 4, kSurfelMinShells = 8;
   std::vector<int> nser(N, 0);
+  std::vector<double> kser(N, 0.0);
   std::vector<std::array<double, 3>> ctr(N, {0.0, 0.0, 0.0});
   std::vector<size_t> moff(N, 0);
   size_t total_m = 0;
@@ -240,17 +243,32 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     for (int slot = 0; slot < N; ++slot) {
       const int i = order[slot];
       const GeometryDev& g = *geoms[i];
-      if (g.type != PII_GEOM_TRIANGLE || g.n == 0) continue;
+      if (g.n == 0) continue;
+      const bool tri = g.type == PII_GEOM_TRIANGLE;
       double R;
       world_bounds(g, kinds[i], dofs[i], ctr[slot].data(), R);
+      double cap2 = HUGE_VAL;  // bound on |k|^2 from the surfel factor
+      if (!tri) {
+        double an2 = 0.0;
+        for (int t = 3; t < 12; ++t) an2 += dofs[i][t] * dofs[i][t];
+        const double f = (g.type == PII_GEOM_DISK ? 0.25 : 0.5) * g.ref_smax * g.ref_smax * an2;
+        if (f > 0.0) cap2 = (g.type == PII_GEOM_DISK ? 3.0 : 0.49) / f;
+      }
       int ns = 0;
-      while (ns < series_kg->nshell && series_kg->h_shell_k[ns] >= 0.0 &&
-             series_kg->h_shell_k[ns] * R <= kSeriesReach)
+      double km = 0.0;
+      while (ns < series_kg->nshell) {
+        const double k = series_kg->h_shell_k[ns];
+        if (!(k >= 0.0 && k * R <= kSeriesReach && k * k <= cap2)) break;
+        km = std::max(km, k);
         ++ns;
-      if (ns < kSeriesMinShells) continue;
+      }
+      if (ns < (tri ? kSeriesMinShells : kSurfelMinShells)) continue;
       nser[slot] = ns;
-      moff[slot] = total_m;
-      total_m += align_up((size_t)g.n * kRecMoment, 32);
+      kser[slot] = km;
+      if (tri) {
+        moff[slot] = total_m;
+        total_m += align_up((size_t)g.n * kRecMoment, 32);
+      }
     }
   }
   if (total_m) w.mrec.ensure(total_m * sizeof(double));
@@ -281,8 +299,10 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     d.rec = w.faces.as<double>() + rec_off[slot];
     d.nser = nser[slot];
     d.qbeg = nser[slot] * Mh;
+    d.kser = kser[slot];
     for (int k = 0; k < 3; ++k) d.ctr[k] = ctr[slot][k];
-    d.mrec = nser[slot] ? w.mrec.as<double>() + moff[slot] : nullptr;
+    d.mrec = (nser[slot] && g.type == PII_GEOM_TRIANGLE) ? w.mrec.as<double>() + moff[slot]
+                                                          : nullptr;
   }
   qbeg_min = 0;
   if (N > 0) {
@@ -627,6 +647,27 @@ static int create_surfels(pii_context* ctx, int type, const double* el, int n, p
     d.type = type;
     d.n = n;
     d.h_el.assign(el, el + 8 * (size_t)n);
+    if (n > 0) {
+      double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+      for (int e = 0; e < n; ++e) {
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = std::min(lo[k], el[8 * (size_t)e + k]);
+          hi[k] = std::max(hi[k], el[8 * (size_t)e + k]);
+        }
+        d.ref_smax = std::max(d.ref_smax, std::abs(el[8 * (size_t)e + 6]));
+      }
+      for (int k = 0; k < 3; ++k) d.ref_c[k] = 0.5 * (lo[k] + hi[k]);
+      double r2 = 0.0;
+      for (int e = 0; e < n; ++e) {
+        double d2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const double t = el[8 * (size_t)e + k] - d.ref_c[k];
+          d2 += t * t;
+        }
+        r2 = std::max(r2, d2);
+      }
+      d.ref_R = std::sqrt(r2) * (1.0 + 1e-12);
+    }
     upload(d.el, d.h_el.data(), d.h_el.size(), ctx->c.stream);
     PII_HIP(hipStreamSynchronize(ctx->c.stream));
     *out = g.release();

@@ -22,7 +22,7 @@ struct ObjDesc {
   // from the per-face kernel.
   int nser, qbeg, pad0, pad1;
   double ctr[3];       // expansion centre of the series (world frame)
-  double pad_d;
+  double kser;         // largest |k| of the object's series shells
   const double* verts;        // nv*3 reference-frame vertices
   const int32_t* faces;       // n*3
   const int32_t* dofvid;      // nv

@@ -10,23 +10,25 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
                               int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                               unsigned* d_colmax, Context* prof) {
   const double* krep = kg.rep.as<double>();
-  if (prof) {
-    for (const ObjDesc& d : b.h_objs) {
-      if (d.type == PII_GEOM_TRIANGLE) {
-        prof->work[0] += (double)d.n * (kg.Qh - d.qbeg);
-        if (d.nser > 0) {
-          prof->work[1] += (double)d.n * b.Mh;
-          prof->work[2] += (double)d.qbeg;
-        }
-      } else {
-        prof->work[3] += (double)d.n * kg.Qh;
-      }
+  bool tri_series = false, surfel_series = false;
+  for (const ObjDesc& d : b.h_objs) {
+    const bool tri = d.type == PII_GEOM_TRIANGLE;
+    if (d.nser > 0) (tri ? tri_series : surfel_series) = true;
+    if (!prof) continue;
+    prof->work[tri ? 0 : 3] += (double)d.n * (kg.Qh - d.qbeg);
+    if (d.nser > 0) {
+      prof->work[tri ? 1 : 4] += (double)d.n * b.Mh;
+      prof->work[2] += (double)d.qbeg;
     }
   }
-  if (b.nseries > 0) {
+  if (tri_series || surfel_series) {
     if (prof) prof->phase_begin(4);
-    launch_series_tri(s, b.d_objs, b.N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
-                      d_colmax);
+    if (tri_series)
+      launch_series_tri(s, b.d_objs, b.N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
+                        d_colmax);
+    if (surfel_series)
+      launch_series_surfel(s, b.d_objs, b.N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
+                           d_colmax);
     if (prof) prof->phase_end(4);
     PII_HIP(hipGetLastError());
   }

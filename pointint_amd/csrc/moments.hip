@@ -49,6 +49,79 @@ struct SeriesCoef {
 };
 __device__ constexpr SeriesCoef kSeries{};
 
+// Sums the four waves' H_n through LDS (in a fixed order: (H0 + H2) + (H1 +
+// H3)), applies the real factor of i^n (and, for triangles, 2/(n+2)!), then
+// each wave evaluates the series at every fourth of the object's series shells
+// for its 64 directions.  out_mode 0: A (FP64) to Aout[col*ldA + q]; 1:
+// X = sqrt(w) A (FP32) to Xout[col*ldX + q] and the column max |X| to
+// colmax[col].
+template <bool kTriCoef>
+__device__ __forceinline__ void reduce_and_evaluate(
+    double (&H)[P + 1], double (&red)[2][P + 1][kLanes], int wave, int lane, int j, int jc,
+    double dx, double dy, double dz, const ObjDesc& d, const double* __restrict__ krep, int Qh,
+    int Mh, int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout,
+    int ldX, unsigned* __restrict__ colmax) {
+  // (H0 + H2) + (H1 + H3), in a fixed order
+  if (wave >= 2) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) red[wave - 2][n][lane] = H[n];
+  }
+  __syncthreads();
+  if (wave < 2) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) H[n] += red[wave][n][lane];
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) red[0][n][lane] = H[n];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n)
+      red[1][n][lane] = (H[n] + red[0][n][lane]) *
+                        (kTriCoef ? kSeries.c[n] : (((n / 2) % 2 == 0) ? 1.0 : -1.0));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n <= P; ++n) H[n] = red[1][n][lane];
+  const int nser = d.nser;
+  const double dc = dx * d.ctr[0] + dy * d.ctr[1] + dz * d.ctr[2];
+  float m = 0.0f;
+  for (int s = wave; s < nser; s += kWaves) {
+    const int q = s * Mh + jc;
+    const double kap = krep[6 * (size_t)Qh + q];
+    const double k2 = kap * kap;
+    // even part E = sum a_{2t} kappa^{2t}, odd part O = sum a_{2t+1} kappa^{2t}
+    constexpr int ne = P / 2, no = (P - 1) / 2;
+    double E = H[2 * ne], O = H[2 * no + 1];
+#pragma unroll
+    for (int t = ne - 1; t >= 0; --t) E = fma(E, k2, H[2 * t]);
+#pragma unroll
+    for (int t = no - 1; t >= 0; --t) O = fma(O, k2, H[2 * t + 1]);
+    O *= kap;
+    double sn, cs;
+    sincos(kap * dc, &sn, &cs);
+    double ax = cs * E - sn * O, ay = sn * E + cs * O;
+    if (kap < 1e-10) { ax = 0.0; ay = 0.0; }  // A(0) = 0 (triangle_mesh_dof.cpp:28)
+    if (out_mode == 0) {
+      if (j < Mh) Aout[(size_t)d.col * ldA + q] = make_double2(ax, ay);
+    } else {
+      const double sw = sqrt(krep[7 * (size_t)Qh + q]);
+      const float2 x = make_float2((float)(ax * sw), (float)(ay * sw));
+      if (j < Mh) {
+        Xout[(size_t)d.col * ldX + q] = x;
+        m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+      }
+    }
+  }
+  if (out_mode == 1) {
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if (lane == 0) atomicMax(colmax + d.col, __float_as_uint(m));
+  }
+}
+
 // One workgroup per (object, 64 directions): the four waves take every fourth
 // face, their H_n are summed through LDS, then the waves evaluate the series
 // at every fourth shell.  out_mode 0: A (FP64) to Aout[col*ldA + q]; 1:
@@ -60,8 +133,7 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
     unsigned* __restrict__ colmax) {
   __shared__ double red[2][P + 1][kLanes];
   const ObjDesc& d = objs[blockIdx.y];
-  const int nser = d.nser;
-  if (nser <= 0) return;  // uniform over the workgroup
+  if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE) return;  // uniform over the workgroup
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int j = blockIdx.x * kLanes + lane;
@@ -106,65 +178,118 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
     }
     project(t, w0, w1, w2, s);
   }
-  // (H0 + H2) + (H1 + H3), in a fixed order
-  if (wave >= 2) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) red[wave - 2][n][lane] = H[n];
+  reduce_and_evaluate<true>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
+                            Aout, ldA, Xout, ldX, colmax);
+}
+
+// Disks and Gaussians (affine DoFs; cpp/src/dof/affine_dof.cpp:103-167).  For
+// k = kappa d, with mu = d.(A n), sg = |A^T d|^2 - mu^2 and w = d.(c' - c):
+//   disk:      A_e = area mu J1(x)/x * 2 e^{i kappa d.c'},  x = rho kappa sqrt(sg)
+//   Gaussian:  A_e = wt mu exp(-sigma^2 kappa^2 sg / 2) e^{i kappa d.c'}
+// and with J1(x)/x = sum_m (-1)^m (x/2)^(2m) / (2 m! (m+1)!), (i kappa)^(2m) =
+// (-1)^m kappa^(2m), the coefficient of (i kappa)^n is
+//   H_n = sum_e a_e sum_{2m+j=n} b_m(e) w^j / j!,
+//   disk: a_e = area mu, b_m = u^m / (m! (m+1)!), u = rho^2 max(sg, 0) / 4;
+//   Gaussian: a_e = wt mu, b_m = v^m / m!,  v = sigma^2 sg / 2 (no clamp, as
+//   the reference CPU path).
+// m runs to M_e <= kSurfelTerms, chosen per element (uniform over the wave)
+// from the bound sg <= |A|_F^2 at the object's largest series |k|, so that
+// the first omitted term is below 1e-12.
+constexpr int kSurfelTerms = 10;
+
+struct InvInt {
+  double c[P + 1];
+  constexpr InvInt() : c() {
+    c[0] = 1.0;
+    for (int j = 1; j <= P; ++j) c[j] = 1.0 / (double)j;
   }
-  __syncthreads();
-  if (wave < 2) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) H[n] += red[wave][n][lane];
-  }
-  __syncthreads();
-  if (wave == 1) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) red[0][n][lane] = H[n];
-  }
-  __syncthreads();
-  if (wave == 0) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) red[1][n][lane] = (H[n] + red[0][n][lane]) * kSeries.c[n];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int n = 0; n <= P; ++n) H[n] = red[1][n][lane];
+};
+__device__ constexpr InvInt kInvInt{};
+
+template <bool kDisk>
+__global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
+    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
+    int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
+    unsigned* __restrict__ colmax) {
+  __shared__ double red[2][P + 1][kLanes];
+  const ObjDesc& d = objs[blockIdx.y];
+  if (d.nser <= 0 || d.type != (kDisk ? PII_GEOM_DISK : PII_GEOM_GAUSSIAN)) return;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j = blockIdx.x * kLanes + lane;
+  const int jc = j < Mh ? j : Mh - 1;
+  const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
+               dz = krep[5 * (size_t)Qh + jc];
+  const auto* p = as_const(d.dofs);
+  // |A^T d|^2 (A row-major in dofs[3:12])
+  const double atx = p[3] * dx + p[6] * dy + p[9] * dz;
+  const double aty = p[4] * dx + p[7] * dy + p[10] * dz;
+  const double atz = p[5] * dx + p[8] * dy + p[11] * dz;
+  const double al = atx * atx + aty * aty + atz * atz;
   const double dc = dx * d.ctr[0] + dy * d.ctr[1] + dz * d.ctr[2];
-  float m = 0.0f;
-  for (int s = wave; s < nser; s += kWaves) {
-    const int q = s * Mh + jc;
-    const double kap = krep[6 * (size_t)Qh + q];
-    const double k2 = kap * kap;
-    // even part E = sum a_{2t} kappa^{2t}, odd part O = sum a_{2t+1} kappa^{2t}
-    constexpr int ne = P / 2, no = (P - 1) / 2;
-    double E = H[2 * ne], O = H[2 * no + 1];
+  // uniform bound of |k|^2 sg over this object's series shells
+  double an2 = 0.0;
+  for (int i = 3; i < 12; ++i) an2 += p[i] * p[i];
+  const double kb2 = an2 * d.kser * d.kser;
+  double H[P + 1];
 #pragma unroll
-    for (int t = ne - 1; t >= 0; --t) E = fma(E, k2, H[2 * t]);
-#pragma unroll
-    for (int t = no - 1; t >= 0; --t) O = fma(O, k2, H[2 * t + 1]);
-    O *= kap;
-    double sn, cs;
-    sincos(kap * dc, &sn, &cs);
-    double ax = cs * E - sn * O, ay = sn * E + cs * O;
-    if (kap < 1e-10) { ax = 0.0; ay = 0.0; }  // A(0) = 0 (triangle_mesh_dof.cpp:28)
-    if (out_mode == 0) {
-      if (j < Mh) Aout[(size_t)d.col * ldA + q] = make_double2(ax, ay);
+  for (int n = 0; n <= P; ++n) H[n] = 0.0;
+  const auto* rec = as_const(d.rec);
+  for (int e = wave; e < d.n; e += kWaves) {
+    const auto* r = rec + (size_t)kRecSurfel * e;
+    const double mu = dx * r[3] + dy * r[4] + dz * r[5];
+    const double w = dx * r[0] + dy * r[1] + dz * r[2] - dc;
+    const double sg = al - mu * mu;
+    const double s2 = r[6] * r[6];
+    const double a = r[7] * mu;
+    double u, ub;
+    if (kDisk) {
+      u = 0.25 * s2 * fmax(sg, 0.0);
+      ub = 0.25 * s2 * kb2;
     } else {
-      const double sw = sqrt(krep[7 * (size_t)Qh + q]);
-      const float2 x = make_float2((float)(ax * sw), (float)(ay * sw));
-      if (j < Mh) {
-        Xout[(size_t)d.col * ldX + q] = x;
-        m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+      u = 0.5 * s2 * sg;
+      ub = 0.5 * s2 * kb2;
+    }
+    // terms of the J1 / exponential factor for this element (uniform)
+    int M = 0;
+    {
+      double t = 1.0;
+      while (M < kSurfelTerms) {
+        t *= kDisk ? ub / ((double)(M + 1) * (double)(M + 2)) : ub / (double)(M + 1);
+        if (t < 1e-12) break;
+        ++M;
+      }
+    }
+    double E[P + 1];
+    E[0] = 1.0;
+#pragma unroll
+    for (int jj = 1; jj <= P; ++jj) E[jj] = E[jj - 1] * w * kInvInt.c[jj];
+    double b = a;
+#pragma unroll
+    for (int m = 0; m <= kSurfelTerms; ++m) {
+      if (m <= M) {
+#pragma unroll
+        for (int jj = 0; jj + 2 * m <= P; ++jj) H[2 * m + jj] = fma(b, E[jj], H[2 * m + jj]);
+        b *= kDisk ? u / ((double)(m + 1) * (double)(m + 2)) : u / (double)(m + 1);
       }
     }
   }
-  if (out_mode == 1) {
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if (lane == 0) atomicMax(colmax + d.col, __float_as_uint(m));
-  }
+  reduce_and_evaluate<false>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
+                             Aout, ldA, Xout, ldX, colmax);
 }
 
 }  // namespace
+
+void launch_series_surfel(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
+                          int Qh, int Mh, int out_mode, double2* d_A, int ldA, float2* d_X,
+                          int ldX, unsigned* d_colmax) {
+  if (N <= 0 || Mh <= 0) return;
+  const dim3 grid((Mh + kLanes - 1) / kLanes, N);
+  hipLaunchKernelGGL(k_series_surfel<true>, grid, dim3(kLanes * kWaves), 0, s, d_objs, d_krep, Qh,
+                     Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+  hipLaunchKernelGGL(k_series_surfel<false>, grid, dim3(kLanes * kWaves), 0, s, d_objs, d_krep,
+                     Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+}
 
 void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                        int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,

@@ -109,8 +109,12 @@ struct GeometryDev {
   std::vector<double> h_verts;       // nv*3 reference-frame vertex positions
   std::vector<int32_t> h_faces;      // n*3 indices into h_verts
   std::vector<int32_t> h_dofvid;     // nv: geometry vertex -> DoF vertex (or -1)
-  double ref_c[3] = {0, 0, 0};       // bounding-box centre of the reference vertices
-  double ref_R = 0.0;                // max |v - ref_c| over the reference vertices
+  // Bounding-box centre and radius (max |x - ref_c|) of the reference vertices
+  // (triangles) or element centres (disks / Gaussians), and the largest
+  // rho / sigma of disks / Gaussians: the moment series' reach.
+  double ref_c[3] = {0, 0, 0};
+  double ref_R = 0.0;
+  double ref_smax = 0.0;
   int ndof_verts = 0;                // DoF vertex count for TRIMESH DoFs
   DevBuf verts;                      // nv*3 FP64 (AoS)
   DevBuf faces;                      // n*3 int32
@@ -154,8 +158,9 @@ struct Context {
   // Work of the last call's form-factor kernels: [0] triangle x node
   // evaluations of the per-face kernel, [1] face x direction passes of the
   // moment series, [2] object x node series evaluations, [3] disk/Gaussian x
-  // node evaluations.
-  static constexpr int kWork = 4;
+  // node evaluations of the per-element kernel, [4] disk/Gaussian x direction
+  // passes of the moment series.
+  static constexpr int kWork = 5;
   double work[kWork] = {};
   void begin_call() {
     for (bool& u : ev_used) u = false;
@@ -201,6 +206,10 @@ void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d
 void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                        int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                        unsigned* d_colmax);
+// The same for disk and Gaussian objects.
+void launch_series_surfel(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
+                          int Qh, int Mh, int out_mode, double2* d_A, int ldA, float2* d_X,
+                          int ldX, unsigned* d_colmax);
 // Full-grid values from representative values: A(-k) = -conj(A(k)).
 void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,
                    const int* d_anti, int Qh, int Q, int N, double2* d_Afull);
