@@ -40,6 +40,14 @@ FLOP_PER_TRI_K = 66.0
 SERIES_ORDER = 48
 FLOP_PER_FACE_DIR = 21.0 + 7.0 * SERIES_ORDER
 FLOP_PER_SERIES_NODE = 2.0 * SERIES_ORDER + 12.0
+# Per disk x k-node of the per-element kernel: about 25 FMA-class operations
+# (50 flop), J1(x)/x (about 15 FMA, 30 flop), one sincos, one sqrt, one
+# reciprocal.  Per disk x direction of the series: 2 dot products and the
+# factors (15 flop), the w^j/j! sequence (2 x 48) and, for a nominal 6 terms
+# of the J1 factor, sum_{m<6} (49 - 2m) FMAs (528 flop).
+FLOP_PER_DISK_K = 83.0
+FLOP_PER_SURFEL_DIR = 15.0 + 2.0 * SERIES_ORDER + 2.0 * sum(SERIES_ORDER + 1 - 2 * m
+                                                           for m in range(6))
 PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 (= non-packed FP32) peak
 PEAK_MFMA_F16_TFLOPS = 2500.0
 
@@ -316,21 +324,24 @@ def main():
     # the work counters of the last call (units evaluated by that launch).
     kernels = {}
     if args.ff == "fast":
-        kernels["k_ff_fast (per-face)"] = {
-            "ms": float(np.mean(face_ms)), "units": work["tri_node"],
-            "unit": "triangle x node", "flop_per_unit": FLOP_PER_TRI_K}
-        if work["series_face_dir"] > 0:
-            kernels["k_series_tri (moment series)"] = {
-                "ms": float(np.mean(ser_ms)), "units": work["series_face_dir"],
-                "unit": "face x direction (+ object x node evaluations)",
-                "flop_per_unit": FLOP_PER_FACE_DIR,
-                "extra_flop": work["series_node"] * FLOP_PER_SERIES_NODE}
+        kernels["k_ff_fast (per-element)"] = {
+            "ms": float(np.mean(face_ms)), "units": work["tri_node"] + work["surfel_node"],
+            "unit": "element x node",
+            "flop": work["tri_node"] * FLOP_PER_TRI_K + work["surfel_node"] * FLOP_PER_DISK_K}
+        if work["series_face_dir"] + work["series_surfel_dir"] > 0:
+            kernels["k_series_tri / k_series_surfel (moment series)"] = {
+                "ms": float(np.mean(ser_ms)),
+                "units": work["series_face_dir"] + work["series_surfel_dir"],
+                "unit": "element x direction (+ object x node evaluations)",
+                "flop": (work["series_face_dir"] * FLOP_PER_FACE_DIR
+                         + work["series_surfel_dir"] * FLOP_PER_SURFEL_DIR
+                         + work["series_node"] * FLOP_PER_SERIES_NODE)}
     else:
         kernels["k_ff_fp64"] = {"ms": float(np.mean(ff_ms)), "units": wl.prim_count() * kg.Q,
-                                "unit": "triangle x node", "flop_per_unit": FLOP_PER_TRI_K}
+                                "unit": "element x node",
+                                "flop": wl.prim_count() * kg.Q * FLOP_PER_TRI_K}
     for k in kernels.values():
-        k["tflops"] = ((k["units"] * k["flop_per_unit"] + k.get("extra_flop", 0.0))
-                       / (k["ms"] * 1e-3) / 1e12) if k["ms"] > 0 else 0.0
+        k["tflops"] = k["flop"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
     dom = max(kernels, key=lambda n: kernels[n]["ms"])
     ach = kernels[dom]["tflops"]
     roofline = {"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",

@@ -293,6 +293,25 @@ def compute_form_factor(geom: Geometry, kgrid: KGrid, dofs=None, dof_kind="affin
     return out[0::2] + 1j * out[1::2]
 
 
+def _batch_args(geoms, dofs, dof_kinds):
+    """Pointer arrays of a batched call (numpy-built: per-object ctypes work
+    would cost more than the GPU step for a few hundred objects).  Returns the
+    arrays and the DoF vectors they point into (kept alive by the caller)."""
+    n = len(geoms)
+    gh = np.fromiter((g.handle.value for g in geoms), dtype=np.uintp, count=n)
+    if dofs is None and dof_kinds is None:
+        dl = [IDENTITY_AFFINE]
+        kh = np.full(n, _DOF["affine"], dtype=np.int32)
+        dh = np.full(n, IDENTITY_AFFINE.ctypes.data, dtype=np.uintp)
+        nh = np.full(n, 12, dtype=np.int32)
+        return gh, kh, dh, nh, dl
+    kinds, dl = _dof_args(geoms, dofs, dof_kinds)
+    kh = np.asarray(kinds, dtype=np.int32)
+    dh = np.fromiter((d.ctypes.data for d in dl), dtype=np.uintp, count=n)
+    nh = np.fromiter((d.shape[0] for d in dl), dtype=np.int32, count=n)
+    return gh, kh, dh, nh, dl
+
+
 def compute_volume_matrix(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None, dof_kinds=None,
                           ff: str = "fast", gram: str = "fp16x3") -> np.ndarray:
     """V (n x n) = Re(J^H D J)/(8 pi^3) on the GPU."""
@@ -300,14 +319,11 @@ def compute_volume_matrix(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None, do
     if n == 0:
         return np.zeros((0, 0))
     ctx = geoms[0].ctx
-    kinds, dl = _dof_args(geoms, dofs, dof_kinds)
-    gh = (C.c_void_p * n)(*[g.handle.value for g in geoms])
-    kh = (C.c_int * n)(*kinds)
-    dh = (C.c_void_p * n)(*[d.ctypes.data for d in dl])
-    nh = (C.c_int * n)(*[d.shape[0] for d in dl])
-    V = np.zeros((n, n))
-    N.check(N.load().pii_volume_matrix(ctx.handle, n, gh, kh, dh, nh, kgrid.handle, _FF[ff],
-                                       _GRAM[gram], N.ptr(V)))
+    gh, kh, dh, nh, _keep = _batch_args(geoms, dofs, dof_kinds)
+    V = np.empty((n, n))
+    N.check(N.load().pii_volume_matrix(ctx.handle, n, gh.ctypes.data, kh.ctypes.data,
+                                       dh.ctypes.data, nh.ctypes.data, kgrid.handle, _FF[ff],
+                                       _GRAM[gram], V.ctypes.data))
     return V
 
 

@@ -67,6 +67,28 @@ __device__ __forceinline__ const PII_CONST_AS T* as_const(const T* p) {
   return (const PII_CONST_AS T*)(p);
 }
 
+// Workgroup -> (block, object) for a grid of gridDim.x blocks per object and
+// gridDim.y objects that keeps all blocks of one object on one XCD, so that
+// the object's element records stay in that XCD's L2.  The command processor
+// hands workgroups to the 8 XCDs round-robin by linear id (x fastest), so
+// workgroup L runs on XCD L % 8; its (L / 8)-th workgroup there gets object
+// 8 (r / nb) + xcd and block r % nb.  Objects past the last full group of 8
+// keep the plain mapping.  A bijection either way; only locality depends on
+// the dispatch order.
+struct BlockObj {
+  int blk, obj;
+};
+__device__ __forceinline__ BlockObj xcd_block_map() {
+  const int nb = gridDim.x, N = gridDim.y;
+  const int L = blockIdx.y * nb + blockIdx.x;
+  const int N8 = N & ~7;
+  if (L < N8 * nb) {
+    const int r = L >> 3;
+    return {r % nb, (r / nb) * 8 + (L & 7)};
+  }
+  return {(int)blockIdx.x, (int)blockIdx.y};
+}
+
 struct cplx {
   double x, y;
 };

@@ -232,14 +232,15 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
   static_assert(sizeof(double2) * kWaves * kLanes <= sizeof(float2) * kPatchVerts * kLanes,
                 "reduction buffer must fit in the phasor table");
   double2 (*red)[kLanes] = reinterpret_cast<double2 (*)[kLanes]>(&tab[0][0]);
-  const ObjDesc& d = objs[blockIdx.y];
+  const BlockObj bo = xcd_block_map();
+  const ObjDesc& d = objs[bo.obj];
   const int lane = threadIdx.x & (kLanes - 1);
   // wave index as an SGPR value: loop indices derived from it stay uniform,
   // so element records are fetched with scalar loads
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // This object's nodes start at qbeg (the moment series wrote those below).
   // Both exits are uniform over the workgroup.
-  const int q0 = d.qbeg + blockIdx.x * kLanes;
+  const int q0 = d.qbeg + bo.blk * kLanes;
   if (q0 >= (out_mode == 0 ? Qh : ldX)) return;
   const int q = q0 + lane;
   if (q0 >= Qh) {  // padding columns of X only

@@ -132,11 +132,12 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
     int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
     unsigned* __restrict__ colmax) {
   __shared__ double red[2][P + 1][kLanes];
-  const ObjDesc& d = objs[blockIdx.y];
+  const BlockObj bo = xcd_block_map();
+  const ObjDesc& d = objs[bo.obj];
   if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE) return;  // uniform over the workgroup
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int j = blockIdx.x * kLanes + lane;
+  const int j = bo.blk * kLanes + lane;
   const int jc = j < Mh ? j : Mh - 1;
   const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
                dz = krep[5 * (size_t)Qh + jc];
@@ -212,11 +213,12 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
     int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
     unsigned* __restrict__ colmax) {
   __shared__ double red[2][P + 1][kLanes];
-  const ObjDesc& d = objs[blockIdx.y];
+  const BlockObj bo = xcd_block_map();
+  const ObjDesc& d = objs[bo.obj];
   if (d.nser <= 0 || d.type != (kDisk ? PII_GEOM_DISK : PII_GEOM_GAUSSIAN)) return;
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int j = blockIdx.x * kLanes + lane;
+  const int j = bo.blk * kLanes + lane;
   const int jc = j < Mh ? j : Mh - 1;
   const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
                dz = krep[5 * (size_t)Qh + jc];

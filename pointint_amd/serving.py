@@ -39,9 +39,9 @@ class Server:
         """Queue a scene; returns its id, or None when refused by the budget."""
         n = len(geoms)
         d = N.f64(dofs).reshape(n, 12)
-        gh = (C.c_void_p * n)(*[g.handle.value for g in geoms])
+        gh = np.fromiter((g.handle.value for g in geoms), dtype=np.uintp, count=n)
         qid = C.c_int64()
-        rc = N.load().pii_server_submit(self._h, n, gh, N.ptr(d), C.byref(qid))
+        rc = N.load().pii_server_submit(self._h, n, gh.ctypes.data, N.ptr(d), C.byref(qid))
         if rc == N.PII_ERR_BUDGET:
             return None
         N.check(rc)
@@ -49,7 +49,7 @@ class Server:
 
     def wait(self, qid: int, n: int, timeout: float = -1.0):
         """(V (n x n), latency ms) of a finished query; releases it."""
-        V = np.zeros((n, n))
+        V = np.empty((n, n))
         lat = C.c_double()
         N.check(N.load().pii_server_wait(self._h, C.c_int64(qid), float(timeout), N.ptr(V),
                                          C.byref(lat)))
