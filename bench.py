@@ -49,6 +49,11 @@ FLOP_PER_DISK_K = 83.0
 FLOP_PER_SURFEL_DIR = 15.0 + 2.0 * SERIES_ORDER + 2.0 * sum(SERIES_ORDER + 1 - 2 * m
                                                            for m in range(6))
 PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 (= non-packed FP32) peak
+# HBM bytes per launch of the dominant kernel on the default workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB device copy
+# (fetch x2.00, write x1.00); profiles/ff_fast_counters_icospheres256_L64x1202.md
+# holds the counts and commands.  Counters cannot be read inside this run.
+MEASURED_TRAFFIC = {("icospheres", "k_ff_fast (per-element)"): 61.93e6}
 PEAK_MFMA_F16_TFLOPS = 2500.0
 
 
@@ -156,22 +161,30 @@ def run_diff(args):
                   for g, x in zip(geoms, xs)], axis=1)
     rng = np.random.default_rng(7)
     h = 1e-4
-    errs = []
+    errs, errs_r = [], []
     tot = J.sum(axis=1)
+
+    def central(i, v, c, hh, B):
+        vals = []
+        for sgn in (1, -1):
+            x = xs[i].copy()
+            x[3 * v + c] += sgn * hh
+            Ai = P.compute_form_factor(geoms[i], kg, x, "trimesh", precision="fp64")
+            vals.append((kg.w * (Ai * np.conj(B)).real).sum() / (8 * np.pi ** 3))
+        return (vals[0] - vals[1]) / (2 * hh)
+
     for _ in range(args.fd_vertices):
         i = int(rng.integers(n))
         v = int(rng.integers(xs[i].shape[0] // 3))
         c = int(rng.integers(3))
         B = tot - J[:, i]
-        vals = []
-        for sgn in (1, -1):
-            x = xs[i].copy()
-            x[3 * v + c] += sgn * h
-            Ai = P.compute_form_factor(geoms[i], kg, x, "trimesh", precision="fp64")
-            vals.append((kg.w * (Ai * np.conj(B)).real).sum() / (8 * np.pi ** 3))
-        fd = (vals[0] - vals[1]) / (2 * h)
+        fd = central(i, v, c, h, B)
+        # Richardson: (4 D(h/2) - D(h)) / 3 removes the h^2 term, which the
+        # high-|k| shells (|k| up to ~1e3 on this grid) make large at h = 1e-4
+        fd_r = (4.0 * central(i, v, c, 0.5 * h, B) - fd) / 3.0
         scale = np.abs(grads[i]).max()
         errs.append(abs(grads[i][3 * v + c] - fd) / scale)
+        errs_r.append(abs(grads[i][3 * v + c] - fd_r) / scale)
     out = {
         "metric": "adjoint: dLoss/dvertex evaluations (forward V + VJP) per second",
         "value": 1e3 / ms, "unit": "gradient evals/s", "n_gpus": 1, "steps": args.steps,
@@ -182,7 +195,9 @@ def run_diff(args):
         "phase_ms": tm, "overlapping_pair_fraction": frac_overlap,
         "fd_check": {"vertices": args.fd_vertices, "h": h,
                      "max_err_rel_to_max_grad": float(max(errs)),
-                     "median_err_rel_to_max_grad": float(np.median(errs))},
+                     "median_err_rel_to_max_grad": float(np.median(errs)),
+                     "richardson_max_err_rel_to_max_grad": float(max(errs_r)),
+                     "richardson_median_err_rel_to_max_grad": float(np.median(errs_r))},
     }
     print(json.dumps(out))
 
@@ -234,7 +249,7 @@ def main():
     ap.add_argument("--gram", default="bf16x3", choices=["fp64", "bf16x3", "fp16x3"])
     ap.add_argument("--no-fp64-ref", action="store_true", help="skip the plain FP64 GPU run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--n-obj", type=int, default=256)
     ap.add_argument("--workload", default="icospheres",
                     choices=["icospheres", "mixed", "diff", "serving"])
@@ -344,10 +359,19 @@ def main():
         k["tflops"] = k["flop"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
     dom = max(kernels, key=lambda n: kernels[n]["ms"])
     ach = kernels[dom]["tflops"]
+    traffic = MEASURED_TRAFFIC.get((args.workload, dom)) if args.ff == "fast" else None
     roofline = {"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_VALU_TFLOPS, "traffic": None,
+                "frac": ach / PEAK_VALU_TFLOPS, "traffic": traffic,
                 "kernel": dom, "kernel_ms": kernels[dom]["ms"],
                 "units_per_launch": kernels[dom]["units"], "form_factor_kernels": kernels}
+    if args.ff == "fast" and args.workload == "icospheres":
+        # algorithmic bytes of the per-element launch: face records (64 B),
+        # world vertices (24 B), representative grid (64 B per node) in; X
+        # (8 B per object x node it evaluates) out
+        n_direct = work["tri_node"] / (wl.prim_count() / N)
+        roofline["algorithmic_bytes"] = (64.0 * wl.prim_count()
+                                         + 24.0 * sum(o.V.shape[0] for o in wl.objects)
+                                         + 64.0 * kg.Qh + 8.0 * n_direct)
 
     cpu = None
     if not args.no_cpu:
