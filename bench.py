@@ -34,17 +34,17 @@ from pointint_amd import workloads as W  # noqa: E402
 # phase and accumulate) = 60 flop, plus 3 sin/cos pairs and 3 reciprocals
 # counted as one flop each.
 FLOP_PER_TRI_K = 66.0
-# Moment series (order 48): per face x direction, 4 dot products (20 flop),
-# H_0 (1) and per order one multiply and three FMAs (7 flop x 48); per object
-# x node, two Horner sums (2 x 48 flop) and the phase product (12).
-SERIES_ORDER = 48
+# Moment series (order 56): per face x direction, 4 dot products (20 flop),
+# H_0 (1) and per order one multiply and three FMAs (7 flop x 56); per object
+# x node, two Horner sums (2 x 56 flop) and the phase product (12).
+SERIES_ORDER = 56
 FLOP_PER_FACE_DIR = 21.0 + 7.0 * SERIES_ORDER
 FLOP_PER_SERIES_NODE = 2.0 * SERIES_ORDER + 12.0
 # Per disk x k-node of the per-element kernel: about 25 FMA-class operations
 # (50 flop), J1(x)/x (about 15 FMA, 30 flop), one sincos, one sqrt, one
 # reciprocal.  Per disk x direction of the series: 2 dot products and the
-# factors (15 flop), the w^j/j! sequence (2 x 48) and, for a nominal 6 terms
-# of the J1 factor, sum_{m<6} (49 - 2m) FMAs (528 flop).
+# factors (15 flop), the w^j/j! sequence (2 x 56) and, for a nominal 6 terms
+# of the J1 factor, sum_{m<6} (57 - 2m) FMAs (624 flop).
 FLOP_PER_DISK_K = 83.0
 FLOP_PER_SURFEL_DIR = 15.0 + 2.0 * SERIES_ORDER + 2.0 * sum(SERIES_ORDER + 1 - 2 * m
                                                            for m in range(6))
@@ -399,7 +399,7 @@ def main():
         "data": "synthetic (seeded)",
         "config": {"workload": ("256 icospheres L3 (1,280 tris), Lebedev 1202 x L=64 "
                                 "(Q=76,928), fast form factors (FP64 moment series for "
-                                "|k|R <= 12, per-face FP32/FP64 above), BF16x3 Gram")
+                                "|k|R <= 15, per-face FP32/FP64 above), BF16x3 Gram")
                                if args.workload == "icospheres" else
                                ("1,024 mixed objects (512 star meshes T in [80, 20480], "
                                 "512 disk clouds in [1k, 50k]), Lebedev 2702 x L=96 "

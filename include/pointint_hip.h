@@ -110,8 +110,8 @@ int pii_context_work(pii_context* ctx, double* units, int n);
 /* Low-|k| moment series of the fast path (default on).  With it, objects on a
  * shell-structured k-grid (every shell repeats one direction set, as
  * pii_kgrid_build makes) take their form factor at the leading shells with
- * |k| R <= 12 (R the object's radius about its bounding-box centre; for disks
- * and Gaussians also a bound on rho |k| or sigma |k|) from a 48-term moment
+ * |k| R <= 15 (R the object's radius about its bounding-box centre; for disks
+ * and Gaussians also a bound on rho |k| or sigma |k|) from a 56-term moment
  * expansion in FP64, and the per-element evaluation only above; the adjoint
  * follows the same split.  No reference counterpart (the reference evaluates
  * every element at every node); switching it off gives the per-element path

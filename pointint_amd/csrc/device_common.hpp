@@ -49,12 +49,12 @@ constexpr int kRecMoment = 12;
 
 // Moment series of the triangle form factor (see moments.hip): order P and the
 // largest |k| R it is used for, R the object's radius about the centre.  With
-// P = 48 and |k| R <= 12 the tail is below 1.6e-10 of the total area.
-constexpr int kSeriesOrder = 48;
-constexpr double kSeriesReach = 12.0;
+// P = 56 and |k| R <= 15 the tail is below 3.6e-10 of the total area.
+constexpr int kSeriesOrder = 56;
+constexpr double kSeriesReach = 15.0;
 // Per (object, direction) record of the series adjoint: Gamma_0..P, then
 // (m+3) Gamma_{m+1} for m < P; padded to a multiple of 8 doubles.
-constexpr int kGammaStride = 104;
+constexpr int kGammaStride = (2 * kSeriesOrder + 1 + 7) / 8 * 8;
 static_assert(kGammaStride >= 2 * kSeriesOrder + 1, "Gamma record too short");
 
 // Read-only data that a kernel indexes with wave-uniform indices (element

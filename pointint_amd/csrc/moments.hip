@@ -17,8 +17,8 @@
 // serves every shell of the grid with kappa R <= kSeriesReach (R bounds
 // |v - c|).  Each term is bounded by Area (kappa R)^n / n!, so the tail after
 // P = kSeriesOrder terms is below Area X^(P+1)/(P+1)! / (1 - X/(P+2)) at
-// X = kappa R: 1.6e-10 Area for P = 48, X = 12.  The largest term, about
-// Area e^X / sqrt(2 pi X), keeps FP64 rounding below 1e-11 Area.
+// X = kappa R: 3.6e-10 Area for P = 56, X = 15.  The largest term, about
+// Area e^X / sqrt(2 pi X), keeps FP64 rounding below 4e-11 Area.
 //
 // h_n by the recurrences (all FP64, 4 operations per order and face)
 //   b_n = w1^n,  h_n(w0, w1) = w0 h_{n-1}(w0, w1) + b_n,

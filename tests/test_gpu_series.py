@@ -1,9 +1,9 @@
 """GPU parity of the fast path's low-|k| moment series against the FP64 oracle.
 
-The series replaces the per-face evaluation at the shells with |k| R <= 12
+The series replaces the per-face evaluation at the shells with |k| R <= 15
 (R: radius of the object's world vertices about their bounding-box centre;
 for affine DoFs the library bounds it by |A|_2 R_ref).  Its truncation and
-rounding errors are below 2e-10 of the object's total area, so on those nodes
+rounding errors are below 4e-10 of the object's total area, so on those nodes
 the fast path must agree with the FP64 oracle to 1e-9 x area (absolute), far
 tighter than the FP32 per-face path (about 1e-7 x area).  On the whole grid the
 fast-path gate applies: ||A - A64||_w / ||A64||_w < 1e-6.
@@ -16,7 +16,7 @@ from pointint_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-REACH = 12.0
+REACH = 15.0
 
 
 def wrel(a, b, w):
