@@ -8,7 +8,7 @@
 // (cpp/src/dof/triangle_mesh_dof.cpp:76-143, contracted as in
 // cpp/src/compute_intersection_volume_multi_object.cpp:186-233).
 //
-// Per face, with u_j = k.v_j, F = gamma... written as A_face = kappa * 2 g[u0, u1, u2]
+// Per face, with u_j = k.v_j and A_face = kappa * 2 g[u0, u1, u2]
 // (kappa = khat.S, g(z) = -e^{iz}, second divided difference), the
 // derivatives are
 //
@@ -17,21 +17,13 @@
 //   dkappa/dv0 = khat x (e2 - e1)/2, dkappa/dv1 = -khat x e2 / 2,
 //   dkappa/dv2 = khat x e1 / 2.
 //
-// With the nodes sorted a <= b <= c the third differences follow from five
-// second differences, each evaluated by the same cancellation-free
-// half-phasor formula as the forward kernel, or by Taylor series when the
-// span is below kSpanTaylor:
-//   G_a = (2g[a,b,c] - 2g[a,a,b]) / (c - a)
-//   G_b = (2g[b,b,c] - 2g[a,b,b]) / (c - a)
-//   G_c = (2g[b,c,c] - 2g[a,b,c]) / (c - a)
-// and for a total span below kSpanTaylor directly from
-//   2 g[0, a, b, d] = sum_n 2 i^(n+1) h_n(a, b, d) / (n+3)!   (times e^{i u0}).
+// For spans of at least kSpanTaylor, G_j comes from differentiating the
+// forward kernel's middle-node formula (tri_main_adj); below, from the Taylor
+// series 2 g[0, a, b, d] = sum_n 2 i^(n+1) h_n(a, b, d) / (n+3)! (times
+// e^{i u0}) in FP64.
 //
-// Mapping: one lane owns one face of a patch and loops over k-nodes in
-// batches of kKB; per batch the workgroup fills an LDS table of the patch
-// vertices' half phasors for those nodes.  Per face the lane accumulates
-// sum_k r_j k (j = 0..2) and sum_k s khat in FP64; the cross products with
-// the edges and the scatter to vertices happen once at the end.
+// This kernel covers the nodes from each object's qbeg (the high-|k| shells);
+// adjoint_series.hip covers the shells below.
 #include "device_common.hpp"
 #include "pii_internal.hpp"
 
@@ -39,7 +31,8 @@ namespace pii {
 
 namespace {
 
-constexpr int kKB = 32;              // k-nodes per LDS batch
+constexpr int kLanes = 64;
+constexpr int kWaves = 4;
 constexpr double kSpanTaylor = 1.0;
 constexpr float kSincPoly = 0.5f;
 
@@ -69,17 +62,6 @@ __device__ __forceinline__ float2 half_phasor_a(double u) {
   return phasor_rev_a(t - rint(t));
 }
 
-__device__ __forceinline__ float sinc_a(float h, float sin_h) {
-  if (fabsf(h) < kSincPoly) {
-    const float h2 = h * h;
-    float p = h2 * (1.0f / 362880.0f) - (1.0f / 5040.0f);
-    p = p * h2 + (1.0f / 120.0f);
-    p = p * h2 - (1.0f / 6.0f);
-    return p * h2 + 1.0f;
-  }
-  return sin_h * __builtin_amdgcn_rcpf(h);
-}
-
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -99,28 +81,6 @@ __device__ __forceinline__ double2 phi_taylor(double a, double b) {
   an *= a; hn = fma(b, hn, an); re -= hn * (1.0 / 239500800.0);
   an *= a; hn = fma(b, hn, an); im -= hn * (1.0 / 3113510400.0);
   return make_double2(re, im);
-}
-
-// 2 g[x0, x1, x2] for nodes given relative to a reference node r (offsets
-// d0 <= d1 <= d2, FP64) with half phasors Q0, Q1, Q2 of the absolute nodes and
-// the full phasor Pr of the reference node (for the Taylor branch, which
-// expands around the smallest node).
-__device__ __forceinline__ float2 dd2(double d0, double d1, double d2, float2 Q0, float2 Q1,
-                                      float2 Q2) {
-  const double span = d2 - d0;
-  if (span < kSpanTaylor) {
-    const double2 t = phi_taylor(d1 - d0, d2 - d0);
-    const float2 P0 = cmulf(Q0, Q0);  // e^{i x0}
-    return cmulf(P0, make_float2((float)t.x, (float)t.y));
-  }
-  // middle node x1:  2 g = -2i Q1 (Q2 sinc(hq) - Q0 sinc(hp)) / (x2 - x0)
-  const double hp = 0.5 * (d1 - d0), hq = 0.5 * (d2 - d1);
-  const float sp = sinc_a((float)hp, Q1.y * Q0.x - Q1.x * Q0.y);
-  const float sq = sinc_a((float)hq, Q2.y * Q1.x - Q2.x * Q1.y);
-  const float2 n = make_float2(Q2.x * sq - Q0.x * sp, Q2.y * sq - Q0.y * sp);
-  const float2 t = cmulf(Q1, n);
-  const float f = 2.0f * __builtin_amdgcn_rcpf((float)span);
-  return make_float2(t.y * f, -t.x * f);
 }
 
 // 2 g[0, a, b, d] by its Taylor series (relative to u0), for d in {0, a, b}:
@@ -152,135 +112,214 @@ __device__ __forceinline__ void dd3_taylor(double a, double b, double2& T0, doub
   T2 = make_double2(r2, i2);
 }
 
-__global__ void __launch_bounds__(128) k_adjoint_tri(const ObjDesc* __restrict__ objs,
+// Sum over the 64 lanes of a wave with DPP moves (VALU only, no LDS):
+// quad_perm [1,0,3,2] and [2,3,0,1] give quad sums, row_half_mirror and
+// row_mirror row sums, row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3)
+// carry them into lane 63, which holds the total.  Lanes a move does not
+// write receive 0.  Every lane of the wave must be active.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask, 0xf, false));
+}
+__device__ __forceinline__ float wave_total(float v) {
+  v += dpp_move<0xb1, 0xf>(v);
+  v += dpp_move<0x4e, 0xf>(v);
+  v += dpp_move<0x141, 0xf>(v);
+  v += dpp_move<0x140, 0xf>(v);
+  v += dpp_move<0x142, 0xa>(v);
+  v += dpp_move<0x143, 0xc>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// sinc'(h) = (cos h - sinc h) / h from cos h and sinc h; odd polynomial below
+// |h| = kSincPoly (truncation < 1e-9 there).  Both evaluated, then selected.
+__device__ __forceinline__ float dsinc_from(float h, float cos_h, float sinc_h) {
+  const float h2 = h * h;
+  float p = h2 * (1.0f / 45360.0f) - (1.0f / 840.0f);
+  p = p * h2 + (1.0f / 30.0f);
+  p = p * h2 - (1.0f / 3.0f);
+  p = p * h;
+  const float r = (cos_h - sinc_h) * __builtin_amdgcn_rcpf(h);
+  return fabsf(h) < kSincPoly ? p : r;
+}
+
+__device__ __forceinline__ float sinc_sel(float h, float sin_h) {
+  const float h2 = h * h;
+  float p = h2 * (1.0f / 362880.0f) - (1.0f / 5040.0f);
+  p = p * h2 + (1.0f / 120.0f);
+  p = p * h2 - (1.0f / 6.0f);
+  p = p * h2 + 1.0f;
+  const float r = sin_h * __builtin_amdgcn_rcpf(h);
+  return fabsf(h) < kSincPoly ? p : r;
+}
+
+// F = 2 g[u0, u1, u2] and G_j = dF/du_j for relative phases (0, a, b) and
+// half phasors Q0..Q2 of the absolute phases, span >= kSpanTaylor.  With the
+// middle node m and the outer p, q:  F = -2i T / D,  T = Q_m N,
+// N = Q_q S(h_q) - Q_p S(h_p),  D = u_q - u_p,  h the half gaps, S = sinc,
+// dQ/du = (i/2) Q, and
+//   G_q = -2i (Q_m dN/du_q / D - T / D^2),  dN/du_q =  Q_q (S'_q + i S_q) / 2,
+//   G_p = -2i (Q_m dN/du_p / D + T / D^2),  dN/du_p =  Q_p (S'_p - i S_p) / 2,
+//   G_m = i F - G_p - G_q   (translation: sum_j dF/du_j = i F).
+__device__ __forceinline__ void tri_main_adj(double al, double be, float2 Q0, float2 Q1,
+                                             float2 Q2, float2& F, float2& G0, float2& G1,
+                                             float2& G2) {
+  const float a = (float)al, b = (float)be;
+  const bool m0 = (al * be <= 0.0);
+  const bool m1 = (!m0) & (fabsf(a) < fabsf(b));
+  const bool m01 = m0 | m1;
+  const float um = m0 ? 0.0f : (m1 ? a : b);
+  const float up = m0 ? a : 0.0f;
+  const float uq = m01 ? b : a;
+  const float2 Qm = make_float2(m0 ? Q0.x : (m1 ? Q1.x : Q2.x), m0 ? Q0.y : (m1 ? Q1.y : Q2.y));
+  const float2 Qp = make_float2(m0 ? Q1.x : Q0.x, m0 ? Q1.y : Q0.y);
+  const float2 Qq = make_float2(m01 ? Q2.x : Q1.x, m01 ? Q2.y : Q1.y);
+  const float hp = 0.5f * (um - up), hq = 0.5f * (uq - um);
+  const float iD = __builtin_amdgcn_rcpf(uq - up);
+  // e^{i h_p} = Q_m conj Q_p, e^{i h_q} = Q_q conj Q_m
+  const float sin_p = Qm.y * Qp.x - Qm.x * Qp.y, cos_p = Qm.x * Qp.x + Qm.y * Qp.y;
+  const float sin_q = Qq.y * Qm.x - Qq.x * Qm.y, cos_q = Qq.x * Qm.x + Qq.y * Qm.y;
+  const float Sp = sinc_sel(hp, sin_p), Sq = sinc_sel(hq, sin_q);
+  const float dSp = dsinc_from(hp, cos_p, Sp), dSq = dsinc_from(hq, cos_q, Sq);
+  const float2 N = make_float2(Qq.x * Sq - Qp.x * Sp, Qq.y * Sq - Qp.y * Sp);
+  const float2 T = cmulf(Qm, N);
+  F = make_float2(2.0f * T.y * iD, -2.0f * T.x * iD);
+  const float2 dNq = cmulf(Qq, make_float2(0.5f * dSq, 0.5f * Sq));
+  const float2 dNp = cmulf(Qp, make_float2(0.5f * dSp, -0.5f * Sp));
+  const float2 Mq = cmulf(Qm, dNq), Mp = cmulf(Qm, dNp);
+  const float iD2 = iD * iD;
+  // -2i (x + i y) = 2y - 2i x
+  const float2 cq = make_float2(Mq.x * iD - T.x * iD2, Mq.y * iD - T.y * iD2);
+  const float2 cp = make_float2(Mp.x * iD + T.x * iD2, Mp.y * iD + T.y * iD2);
+  const float2 Gq = make_float2(2.0f * cq.y, -2.0f * cq.x);
+  const float2 Gp = make_float2(2.0f * cp.y, -2.0f * cp.x);
+  const float2 Gm = make_float2(-F.y - Gp.x - Gq.x, F.x - Gp.y - Gq.y);
+  G0 = make_float2(m0 ? Gm.x : Gp.x, m0 ? Gm.y : Gp.y);
+  G1 = make_float2(m0 ? Gp.x : (m1 ? Gm.x : Gq.x), m0 ? Gp.y : (m1 ? Gm.y : Gq.y));
+  G2 = make_float2(m01 ? Gq.x : Gm.x, m01 ? Gq.y : Gm.y);
+}
+
+// One workgroup per (patch, node range): one lane per node in blocks of 64
+// nodes; per block the waves fill an LDS table of the patch vertices' half
+// phasors, then stream the patch's faces (a quarter each, wave-uniform, scalar
+// loads).  Per face and node the lane forms s = Re(F w conj B) and
+// r_j = kappa Re(G_j w conj B); the wave sums r_j k and s khat over its 64
+// nodes and adds them to the face's LDS accumulator.  At the end the
+// accumulators become vertex gradients (cross products with the edges for
+// the dkappa/dv terms) added to the world-vertex gradient.
+__global__ void __launch_bounds__(256) k_adjoint_tri(const ObjDesc* __restrict__ objs,
                                                      const double* __restrict__ krep, int Qh,
                                                      const float2* __restrict__ Bw, int ldB,
                                                      int kchunk, double* __restrict__ gworld,
                                                      const int* __restrict__ goff) {
-  __shared__ float2 tab[kKB][kPatchVerts];
+  __shared__ float2 tab[kPatchVerts][kLanes];
+  __shared__ double acc[kPatchFaces][12];
   const ObjDesc& d = objs[blockIdx.z];
   const int p = blockIdx.x;
   if (p >= d.npatch) return;
-  const int vbeg = d.patch_hdr[4 * p], vcnt = d.patch_hdr[4 * p + 1];
-  const int fbeg = d.patch_hdr[4 * p + 2], fcnt = d.patch_hdr[4 * p + 3];
   // nodes below qbeg come from the moment series (adjoint_series.hip)
   const int kbeg = d.qbeg + blockIdx.y * kchunk;
   const int kend = min(Qh, kbeg + kchunk);
   if (kbeg >= kend) return;
-  const int t = threadIdx.x;
-  const bool active = t < fcnt;
-  const int fi = active ? t : 0;
-  const double* r = d.rec + (size_t)kRecFastTri * (fbeg + fi);
-  const double e1x = r[0], e1y = r[1], e1z = r[2], e2x = r[3], e2y = r[4], e2z = r[5];
-  const float* rf = reinterpret_cast<const float*>(r + 6);
-  const float S2x = rf[0], S2y = rf[1], S2z = rf[2];
-  const uint32_t loc = reinterpret_cast<const uint32_t*>(rf)[3];
-  const int l0 = loc & 0xff, l1 = (loc >> 8) & 0xff, l2 = (loc >> 16) & 0xff;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const auto* hdr = as_const(d.patch_hdr);
+  const int vbeg = hdr[4 * p], vcnt = hdr[4 * p + 1];
+  const int fbeg = hdr[4 * p + 2], fcnt = hdr[4 * p + 3];
+  for (int i = threadIdx.x; i < fcnt * 12; i += blockDim.x) (&acc[0][0])[i] = 0.0;
+  const auto* pverts = as_const(d.patch_verts);
+  const auto* world = as_const(d.world);
+  const auto* rec = as_const(d.rec);
   const float2* Bcol = Bw + (size_t)d.col * ldB;
 
-  double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  double Sk[3] = {0, 0, 0};
-
-  for (int k0 = kbeg; k0 < kend; k0 += kKB) {
-    const int nk = min(kKB, kend - k0);
-    __syncthreads();
-    for (int e = t; e < nk * vcnt; e += blockDim.x) {
-      const int kk = e / vcnt, s = e - kk * vcnt;
-      const int q = k0 + kk;
-      const int gv = d.patch_verts[vbeg + s];
-      const double* W = d.world + 3 * (size_t)gv;
-      const double u = krep[q] * W[0] + krep[(size_t)Qh + q] * W[1] + krep[2 * (size_t)Qh + q] * W[2];
-      tab[kk][s] = half_phasor_a(u);
+  for (int q0 = kbeg; q0 < kend; q0 += kLanes) {
+    const int q = q0 + lane;
+    const int qc = q < kend ? q : kend - 1;
+    const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
+    const float hx = (float)krep[3 * (size_t)Qh + qc], hy = (float)krep[4 * (size_t)Qh + qc],
+                hz = (float)krep[5 * (size_t)Qh + qc];
+    const double km = krep[6 * (size_t)Qh + qc];
+    // w conj(B) = sqrt(w) conj(sqrt(w) B); zero for lanes past the range and
+    // for k = 0 (A(0) = 0)
+    const float sw = (q < kend && km >= 1e-10) ? (float)sqrt(krep[7 * (size_t)Qh + qc]) : 0.0f;
+    const float2 bq = Bcol[qc];
+    const float ox = bq.x * sw, oy = -bq.y * sw;
+    __syncthreads();  // the previous block's table reads are done
+    for (int sv = wave; sv < vcnt; sv += kWaves) {
+      const int gv = pverts[vbeg + sv];
+      const double u = kx * world[3 * gv] + ky * world[3 * gv + 1] + kz * world[3 * gv + 2];
+      tab[sv][lane] = half_phasor_a(u);
     }
     __syncthreads();
-    if (!active) continue;
-    for (int kk = 0; kk < nk; ++kk) {
-      const int q = k0 + kk;
-      const double kx = krep[q], ky = krep[(size_t)Qh + q], kz = krep[2 * (size_t)Qh + q];
-      const float hx = (float)krep[3 * (size_t)Qh + q], hy = (float)krep[4 * (size_t)Qh + q],
-                  hz = (float)krep[5 * (size_t)Qh + q];
-      const double km = krep[6 * (size_t)Qh + q];
-      if (km < 1e-10) continue;
-      // conj(B) scaled by sqrt(w): Bw holds sqrt(w) B; times sqrt(w) again.
-      const float sw = (float)sqrt(krep[7 * (size_t)Qh + q]);
-      const float2 b = Bcol[q];
-      const float bx = b.x * sw, by = -b.y * sw;   // sqrt(w) * w^{1/2} conj(B) = w conj(B)
-      const double al = kx * e1x + ky * e1y + kz * e1z;
-      const double be = kx * e2x + ky * e2y + kz * e2z;
-      const float kap = 0.5f * (hx * S2x + hy * S2y + hz * S2z);
-      const float2 Q0 = tab[kk][l0], Q1 = tab[kk][l1], Q2 = tab[kk][l2];
-      // sort nodes {0 (v0), al (v1), be (v2)}
-      double x[3] = {0.0, al, be};
-      float2 Qs[3] = {Q0, Q1, Q2};
-      int id[3] = {0, 1, 2};
-#define PII_SWAP(i, j)                                               \
-  if (x[j] < x[i]) {                                                 \
-    double tx = x[i]; x[i] = x[j]; x[j] = tx;                        \
-    float2 tq = Qs[i]; Qs[i] = Qs[j]; Qs[j] = tq;                    \
-    int ti = id[i]; id[i] = id[j]; id[j] = ti;                       \
-  }
-      PII_SWAP(0, 1)
-      PII_SWAP(1, 2)
-      PII_SWAP(0, 1)
-#undef PII_SWAP
-      const double H = x[2] - x[0];
-      float2 F, Gs[3];  // Gs in sorted order (a, b, c)
+    for (int f = wave; f < fcnt; f += kWaves) {
+      const auto* r = rec + (size_t)kRecFastTri * (fbeg + f);
+      const double al = kx * r[0] + ky * r[1] + kz * r[2];
+      const double be = kx * r[3] + ky * r[4] + kz * r[5];
+      const auto* rf = reinterpret_cast<const PII_CONST_AS float*>(r + 6);
+      const float kap = 0.5f * (hx * rf[0] + hy * rf[1] + hz * rf[2]);
+      const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
+      const float2 Q0 = tab[loc & 0xff][lane], Q1 = tab[(loc >> 8) & 0xff][lane],
+                   Q2 = tab[(loc >> 16) & 0xff][lane];
+      const double H = fmax(0.0, fmax(al, be)) - fmin(0.0, fmin(al, be));
+      float2 F, G0, G1, G2;
       if (H < kSpanTaylor) {
         double2 T0, T1, T2;
         dd3_taylor(al, be, T0, T1, T2);
         const double2 tf = phi_taylor(al, be);
-        const float2 P0 = cmulf(Q0, Q0);
+        const float2 P0 = cmulf(Q0, Q0);  // e^{i u0}
         F = cmulf(P0, make_float2((float)tf.x, (float)tf.y));
-        const float2 G0 = cmulf(P0, make_float2((float)T0.x, (float)T0.y));
-        const float2 G1 = cmulf(P0, make_float2((float)T1.x, (float)T1.y));
-        const float2 G2 = cmulf(P0, make_float2((float)T2.x, (float)T2.y));
-        // back to sorted order
-        for (int s = 0; s < 3; ++s) Gs[s] = (id[s] == 0) ? G0 : (id[s] == 1) ? G1 : G2;
+        G0 = cmulf(P0, make_float2((float)T0.x, (float)T0.y));
+        G1 = cmulf(P0, make_float2((float)T1.x, (float)T1.y));
+        G2 = cmulf(P0, make_float2((float)T2.x, (float)T2.y));
       } else {
-        F = dd2(x[0], x[1], x[2], Qs[0], Qs[1], Qs[2]);
-        const float2 Faab = dd2(x[0], x[0], x[1], Qs[0], Qs[0], Qs[1]);
-        const float2 Fabb = dd2(x[0], x[1], x[1], Qs[0], Qs[1], Qs[1]);
-        const float2 Fbbc = dd2(x[1], x[1], x[2], Qs[1], Qs[1], Qs[2]);
-        const float2 Fbcc = dd2(x[1], x[2], x[2], Qs[1], Qs[2], Qs[2]);
-        const float ih = __builtin_amdgcn_rcpf((float)H);
-        Gs[0] = make_float2((F.x - Faab.x) * ih, (F.y - Faab.y) * ih);
-        Gs[1] = make_float2((Fbbc.x - Fabb.x) * ih, (Fbbc.y - Fabb.y) * ih);
-        Gs[2] = make_float2((Fbcc.x - F.x) * ih, (Fbcc.y - F.y) * ih);
+        tri_main_adj(al, be, Q0, Q1, Q2, F, G0, G1, G2);
       }
-      // s = Re(F w conj B); r_j = kappa Re(G_j w conj B)
-      const float s = F.x * bx - F.y * by;
-      Sk[0] += (double)(s * hx); Sk[1] += (double)(s * hy); Sk[2] += (double)(s * hz);
+      const float sf = F.x * ox - F.y * oy;
+      const float r0 = kap * (G0.x * ox - G0.y * oy);
+      const float r1 = kap * (G1.x * ox - G1.y * oy);
+      const float r2 = kap * (G2.x * ox - G2.y * oy);
+      // per-block sums over the wave's 64 nodes in FP32 (relative rounding
+      // 6e-8 of the block's terms), accumulated over blocks in FP64
+      const float fx = (float)kx, fy = (float)ky, fz = (float)kz;
+      const float v[12] = {r0 * fx, r0 * fy, r0 * fz, r1 * fx, r1 * fy, r1 * fz,
+                           r2 * fx, r2 * fy, r2 * fz,
+                           sf * hx, sf * hy, sf * hz};
+      float t[12];
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const float rj = kap * (Gs[m].x * bx - Gs[m].y * by);
-        const int j = id[m];
-        const double rd = (double)rj;
-        R[j][0] = fma(rd, kx, R[j][0]);
-        R[j][1] = fma(rd, ky, R[j][1]);
-        R[j][2] = fma(rd, kz, R[j][2]);
+      for (int i = 0; i < 12; ++i) t[i] = wave_total(v[i]);
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 12; ++i) acc[f][i] += (double)t[i];
       }
     }
   }
-  if (!active) return;
-  // dkappa terms: v0: Sk x (e2 - e1)/2, v1: -Sk x e2 / 2, v2: Sk x e1 / 2
-  const double dx = e2x - e1x, dy = e2y - e1y, dz = e2z - e1z;
-  double g[3][3];
-  g[0][0] = R[0][0] + 0.5 * (Sk[1] * dz - Sk[2] * dy);
-  g[0][1] = R[0][1] + 0.5 * (Sk[2] * dx - Sk[0] * dz);
-  g[0][2] = R[0][2] + 0.5 * (Sk[0] * dy - Sk[1] * dx);
-  g[1][0] = R[1][0] - 0.5 * (Sk[1] * e2z - Sk[2] * e2y);
-  g[1][1] = R[1][1] - 0.5 * (Sk[2] * e2x - Sk[0] * e2z);
-  g[1][2] = R[1][2] - 0.5 * (Sk[0] * e2y - Sk[1] * e2x);
-  g[2][0] = R[2][0] + 0.5 * (Sk[1] * e1z - Sk[2] * e1y);
-  g[2][1] = R[2][1] + 0.5 * (Sk[2] * e1x - Sk[0] * e1z);
-  g[2][2] = R[2][2] + 0.5 * (Sk[0] * e1y - Sk[1] * e1x);
-  const int f = (int)d.patch_faces[fbeg + t];
-  double* G = gworld + goff[blockIdx.z];
+  __syncthreads();
+  for (int t = threadIdx.x; t < fcnt; t += blockDim.x) {
+    const double* r = d.rec + (size_t)kRecFastTri * (fbeg + t);
+    const double e1x = r[0], e1y = r[1], e1z = r[2], e2x = r[3], e2y = r[4], e2z = r[5];
+    const double* R = acc[t];
+    const double* Sk = acc[t] + 9;
+    // dkappa terms: v0: Sk x (e2 - e1)/2, v1: -Sk x e2 / 2, v2: Sk x e1 / 2
+    const double dx = e2x - e1x, dy = e2y - e1y, dz = e2z - e1z;
+    double g[3][3];
+    g[0][0] = R[0] + 0.5 * (Sk[1] * dz - Sk[2] * dy);
+    g[0][1] = R[1] + 0.5 * (Sk[2] * dx - Sk[0] * dz);
+    g[0][2] = R[2] + 0.5 * (Sk[0] * dy - Sk[1] * dx);
+    g[1][0] = R[3] - 0.5 * (Sk[1] * e2z - Sk[2] * e2y);
+    g[1][1] = R[4] - 0.5 * (Sk[2] * e2x - Sk[0] * e2z);
+    g[1][2] = R[5] - 0.5 * (Sk[0] * e2y - Sk[1] * e2x);
+    g[2][0] = R[6] + 0.5 * (Sk[1] * e1z - Sk[2] * e1y);
+    g[2][1] = R[7] + 0.5 * (Sk[2] * e1x - Sk[0] * e1z);
+    g[2][2] = R[8] + 0.5 * (Sk[0] * e1y - Sk[1] * e1x);
+    const int f = (int)d.patch_faces[fbeg + t];
+    double* G = gworld + goff[blockIdx.z];
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const int v = d.faces[3 * f + j];
+    for (int j = 0; j < 3; ++j) {
+      const int v = d.faces[3 * f + j];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) atomicAdd(G + 3 * (size_t)v + c, g[j][c] * kInv8Pi3);
+      for (int c = 0; c < 3; ++c) atomicAdd(G + 3 * (size_t)v + c, g[j][c] * kInv8Pi3);
+    }
   }
 }
 
@@ -323,9 +362,9 @@ void launch_adjoint_tri(hipStream_t s, const ObjDesc* d_objs, int nobj, int max_
   const int span = Qh - qbeg_min;
   if (nobj <= 0 || max_patches <= 0 || span <= 0) return;
   int kchunk = (span + ksplit - 1) / ksplit;
-  kchunk = (kchunk + kKB - 1) / kKB * kKB;
-  hipLaunchKernelGGL(k_adjoint_tri, dim3(max_patches, ksplit, nobj), dim3(128), 0, s, d_objs,
-                     d_krep, Qh, d_Bw, ldB, kchunk, d_gworld, d_goff);
+  kchunk = (kchunk + kLanes - 1) / kLanes * kLanes;
+  hipLaunchKernelGGL(k_adjoint_tri, dim3(max_patches, ksplit, nobj), dim3(kLanes * kWaves), 0, s,
+                     d_objs, d_krep, Qh, d_Bw, ldB, kchunk, d_gworld, d_goff);
 }
 
 }  // namespace pii
