@@ -18,9 +18,9 @@
 //     Vector3d (std::array<double,3>) with the accessors the drivers use.
 //   * double3 comes from HIP's vector types; <cuda_runtime.h> is never used.
 //   * The "_cuda" names are kept for drop-in compatibility; they run on HIP.
-//   * HESSIAN is not implemented: the result's Hessian blocks stay empty and
-//     pii_last_error-style text is printed to std::cerr, as the reference
-//     reports errors.
+//   * HESSIAN (Gauss-Newton blocks) is computed for AffineDoF objects; other
+//     DoF kinds print the error to std::cerr and leave the blocks empty, as
+//     the reference reports errors.
 #pragma once
 
 #include <hip/hip_vector_types.h>
@@ -156,14 +156,16 @@ struct IntersectionVolumeMatrixResult {
   // grad_matrix[i][j] = dV_ij / dtheta_i; for i == j half the full derivative
   // (cpp/include/compute_intersection_volume_multi_object.hpp:22-26).
   std::vector<std::vector<VectorXd>> grad_matrix;
-  std::vector<std::vector<MatrixXd>> hessian_ii, hessian_jj, hessian_ij;  // not computed
+  // Gauss-Newton blocks (AffineDoF): hessian_ij[i][j] = (1/8 pi^3) sum_q w_q
+  // Re(dA_i/dtheta_i (dA_j/dtheta_j)^H); hessian_ii / hessian_jj are zero.
+  std::vector<std::vector<MatrixXd>> hessian_ii, hessian_jj, hessian_ij;
   int num_objects() const { return volume_matrix.rows(); }
 };
 
 struct IntersectionVolumeResult {
   double volume = 0.0;
   VectorXd grad_geom1, grad_geom2;
-  MatrixXd hessian_geom1, hessian_geom2, hessian_cross;  // not computed
+  MatrixXd hessian_geom1, hessian_geom2, hessian_cross;  // Gauss-Newton (AffineDoF)
 };
 
 // ---- drivers ----

@@ -163,6 +163,22 @@ int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
                       const int* dof_kinds, const double* const* dofs, const int* ndofs,
                       const pii_kgrid* kg, int ff_precision, int gram_mode, double* V);
 
+/* Gauss-Newton Hessian blocks of the volume matrix for AffineDoF objects
+ * (triangles, disks, Gaussians):
+ *   H_ij = (1/8 pi^3) sum_q w_q Re( dA_i/dtheta_i (dA_j/dtheta_j)^H ),
+ * every ordered pair (i, j), i == j included.  H is (12n x 12n) row-major;
+ * block (i, j) occupies rows 12i..12i+11 and columns 12j..12j+11, DoF order
+ * [t, A row-major].  gram_mode as for pii_volume_matrix.  V (may be NULL)
+ * receives the volume matrix of the same call.  Other DoF kinds return
+ * PII_ERR_UNSUPPORTED.
+ * Replaces the HESSIAN flag of compute_intersection_volume_matrix_cuda/cpu
+ * (cpp/src/compute_intersection_volume_multi_object.cpp:236-292), whose
+ * hessian_ij[i][j] is block (i, j) and whose hessian_ii / hessian_jj are zero
+ * in the Gauss-Newton form. */
+int pii_volume_matrix_hessian(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                              const int* dof_kinds, const double* const* dofs, const int* ndofs,
+                              const pii_kgrid* kg, int gram_mode, double* H, double* V);
+
 /* Adjoint: gradients of L = sum_ij dL_dV[i*n+j] V_ij with respect to every
  * object's DoFs, written to grads[i] (ndofs[i] doubles), without forming
  * dA/dtheta (vector-Jacobian product over the k-nodes).  V (optional, may be

@@ -24,4 +24,5 @@ from .core import (  # noqa: F401
     make_point_cloud,
     make_triangle_mesh,
     volume_matrix_vjp,
+    volume_matrix_hessian,
 )

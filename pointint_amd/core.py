@@ -347,6 +347,23 @@ def volume_matrix_vjp(geoms: Sequence[Geometry], kgrid: KGrid, dL_dV, dofs=None,
     return grads, V
 
 
+def volume_matrix_hessian(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None,
+                          gram: str = "fp16x3", return_volume: bool = False):
+    """Gauss-Newton Hessian blocks of V for AffineDoF objects: H (12n x 12n),
+    block (i, j) = (1/8 pi^3) sum_q w_q Re(dA_i/dtheta_i (dA_j/dtheta_j)^H);
+    with return_volume also V of the same call."""
+    n = len(geoms)
+    ctx = geoms[0].ctx
+    gh, kh, dh, nh, _keep = _batch_args(geoms, dofs, None if dofs is None else ["affine"] * n)
+    H = np.empty((12 * n, 12 * n))
+    V = np.empty((n, n)) if return_volume else None
+    N.check(N.load().pii_volume_matrix_hessian(ctx.handle, n, gh.ctypes.data, kh.ctypes.data,
+                                               dh.ctypes.data, nh.ctypes.data, kgrid.handle,
+                                               _GRAM[gram], H.ctypes.data,
+                                               V.ctypes.data if V is not None else None))
+    return (H, V) if return_volume else H
+
+
 def gram_from_J(J: np.ndarray, kgrid: KGrid, ctx: Optional[Context] = None) -> np.ndarray:
     """FP64 Gram of caller form factors J (Q x n complex)."""
     ctx = ctx or kgrid.ctx

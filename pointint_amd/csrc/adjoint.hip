@@ -26,6 +26,7 @@
 // adjoint_series.hip covers the shells below.
 #include "device_common.hpp"
 #include "pii_internal.hpp"
+#include "tri_derivs.hpp"
 
 namespace pii {
 
@@ -33,173 +34,6 @@ namespace {
 
 constexpr int kLanes = 64;
 constexpr int kWaves = 4;
-constexpr double kSpanTaylor = 1.0;
-constexpr float kSincPoly = 0.5f;
-
-__device__ __forceinline__ float2 phasor_rev_a(double r) {
-  const double j = rint(4.0 * r);
-  const float x = (float)((r - 0.25 * j) * 6.283185307179586);
-  const float x2 = x * x;
-  float s = x2 * (-1.0f / 39916800.0f) + (1.0f / 362880.0f);
-  s = s * x2 - (1.0f / 5040.0f);
-  s = s * x2 + (1.0f / 120.0f);
-  s = s * x2 - (1.0f / 6.0f);
-  s = (s * x2) * x + x;
-  float c = x2 * (-1.0f / 3628800.0f) + (1.0f / 40320.0f);
-  c = c * x2 - (1.0f / 720.0f);
-  c = c * x2 + (1.0f / 24.0f);
-  c = c * x2 - 0.5f;
-  c = c * x2 + 1.0f;
-  const int q = ((int)j) & 3;
-  float2 o;
-  o.x = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
-  o.y = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
-  return o;
-}
-
-__device__ __forceinline__ float2 half_phasor_a(double u) {
-  const double t = u * 0.07957747154594767;  // 1/(4 pi)
-  return phasor_rev_a(t - rint(t));
-}
-
-__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// Phi(a, b) = sum_n 2 i^n h_n(a, b) / (n+2)!  (FP64), returned as (re, im).
-__device__ __forceinline__ double2 phi_taylor(double a, double b) {
-  double an = 1.0, hn = 1.0, re = 1.0, im = 0.0;
-  an *= a; hn = fma(b, hn, an); im += hn * (1.0 / 3.0);
-  an *= a; hn = fma(b, hn, an); re -= hn * (1.0 / 12.0);
-  an *= a; hn = fma(b, hn, an); im -= hn * (1.0 / 60.0);
-  an *= a; hn = fma(b, hn, an); re += hn * (1.0 / 360.0);
-  an *= a; hn = fma(b, hn, an); im += hn * (1.0 / 2520.0);
-  an *= a; hn = fma(b, hn, an); re -= hn * (1.0 / 20160.0);
-  an *= a; hn = fma(b, hn, an); im -= hn * (1.0 / 181440.0);
-  an *= a; hn = fma(b, hn, an); re += hn * (1.0 / 1814400.0);
-  an *= a; hn = fma(b, hn, an); im += hn * (1.0 / 19958400.0);
-  an *= a; hn = fma(b, hn, an); re -= hn * (1.0 / 239500800.0);
-  an *= a; hn = fma(b, hn, an); im -= hn * (1.0 / 3113510400.0);
-  return make_double2(re, im);
-}
-
-// 2 g[0, a, b, d] by its Taylor series (relative to u0), for d in {0, a, b}:
-// returns the three values for d = 0, a, b.
-__device__ __forceinline__ void dd3_taylor(double a, double b, double2& T0, double2& T1,
-                                           double2& T2) {
-  // h2_n = h_n(a, b); h3_n(d) = d h3_{n-1}(d) + h2_n.  Coefficients
-  // c_n = 2 i^(n+1) / (n+3)!  : i/3, -1/12, -i/60, 1/360, ...
-  double an = 1.0, h2 = 1.0, hA = 1.0, hB = 1.0;
-  double r0 = 0.0, i0 = 1.0 / 3.0, r1 = 0.0, i1 = 1.0 / 3.0, r2 = 0.0, i2 = 1.0 / 3.0;
-  const double c[11] = {-1.0 / 12.0, -1.0 / 60.0, 1.0 / 360.0, 1.0 / 2520.0, -1.0 / 20160.0,
-                        -1.0 / 181440.0, 1.0 / 1814400.0, 1.0 / 19958400.0,
-                        -1.0 / 239500800.0, -1.0 / 3113510400.0, 1.0 / 43589145600.0};
-#pragma unroll
-  for (int n = 1; n <= 11; ++n) {
-    an *= a;
-    h2 = fma(b, h2, an);
-    hA = fma(a, hA, h2);
-    hB = fma(b, hB, h2);
-    const double cf = c[n - 1];
-    if (n & 1) {  // i^(n+1) real
-      r0 += cf * h2; r1 += cf * hA; r2 += cf * hB;
-    } else {
-      i0 += cf * h2; i1 += cf * hA; i2 += cf * hB;
-    }
-  }
-  T0 = make_double2(r0, i0);
-  T1 = make_double2(r1, i1);
-  T2 = make_double2(r2, i2);
-}
-
-// Sum over the 64 lanes of a wave with DPP moves (VALU only, no LDS):
-// quad_perm [1,0,3,2] and [2,3,0,1] give quad sums, row_half_mirror and
-// row_mirror row sums, row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3)
-// carry them into lane 63, which holds the total.  Lanes a move does not
-// write receive 0.  Every lane of the wave must be active.
-template <int kCtrl, int kRowMask>
-__device__ __forceinline__ float dpp_move(float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask, 0xf, false));
-}
-__device__ __forceinline__ float wave_total(float v) {
-  v += dpp_move<0xb1, 0xf>(v);
-  v += dpp_move<0x4e, 0xf>(v);
-  v += dpp_move<0x141, 0xf>(v);
-  v += dpp_move<0x140, 0xf>(v);
-  v += dpp_move<0x142, 0xa>(v);
-  v += dpp_move<0x143, 0xc>(v);
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
-
-// sinc'(h) = (cos h - sinc h) / h from cos h and sinc h; odd polynomial below
-// |h| = kSincPoly (truncation < 1e-9 there).  Both evaluated, then selected.
-__device__ __forceinline__ float dsinc_from(float h, float cos_h, float sinc_h) {
-  const float h2 = h * h;
-  float p = h2 * (1.0f / 45360.0f) - (1.0f / 840.0f);
-  p = p * h2 + (1.0f / 30.0f);
-  p = p * h2 - (1.0f / 3.0f);
-  p = p * h;
-  const float r = (cos_h - sinc_h) * __builtin_amdgcn_rcpf(h);
-  return fabsf(h) < kSincPoly ? p : r;
-}
-
-__device__ __forceinline__ float sinc_sel(float h, float sin_h) {
-  const float h2 = h * h;
-  float p = h2 * (1.0f / 362880.0f) - (1.0f / 5040.0f);
-  p = p * h2 + (1.0f / 120.0f);
-  p = p * h2 - (1.0f / 6.0f);
-  p = p * h2 + 1.0f;
-  const float r = sin_h * __builtin_amdgcn_rcpf(h);
-  return fabsf(h) < kSincPoly ? p : r;
-}
-
-// F = 2 g[u0, u1, u2] and G_j = dF/du_j for relative phases (0, a, b) and
-// half phasors Q0..Q2 of the absolute phases, span >= kSpanTaylor.  With the
-// middle node m and the outer p, q:  F = -2i T / D,  T = Q_m N,
-// N = Q_q S(h_q) - Q_p S(h_p),  D = u_q - u_p,  h the half gaps, S = sinc,
-// dQ/du = (i/2) Q, and
-//   G_q = -2i (Q_m dN/du_q / D - T / D^2),  dN/du_q =  Q_q (S'_q + i S_q) / 2,
-//   G_p = -2i (Q_m dN/du_p / D + T / D^2),  dN/du_p =  Q_p (S'_p - i S_p) / 2,
-//   G_m = i F - G_p - G_q   (translation: sum_j dF/du_j = i F).
-__device__ __forceinline__ void tri_main_adj(double al, double be, float2 Q0, float2 Q1,
-                                             float2 Q2, float2& F, float2& G0, float2& G1,
-                                             float2& G2) {
-  const float a = (float)al, b = (float)be;
-  const bool m0 = (al * be <= 0.0);
-  const bool m1 = (!m0) & (fabsf(a) < fabsf(b));
-  const bool m01 = m0 | m1;
-  const float um = m0 ? 0.0f : (m1 ? a : b);
-  const float up = m0 ? a : 0.0f;
-  const float uq = m01 ? b : a;
-  const float2 Qm = make_float2(m0 ? Q0.x : (m1 ? Q1.x : Q2.x), m0 ? Q0.y : (m1 ? Q1.y : Q2.y));
-  const float2 Qp = make_float2(m0 ? Q1.x : Q0.x, m0 ? Q1.y : Q0.y);
-  const float2 Qq = make_float2(m01 ? Q2.x : Q1.x, m01 ? Q2.y : Q1.y);
-  const float hp = 0.5f * (um - up), hq = 0.5f * (uq - um);
-  const float iD = __builtin_amdgcn_rcpf(uq - up);
-  // e^{i h_p} = Q_m conj Q_p, e^{i h_q} = Q_q conj Q_m
-  const float sin_p = Qm.y * Qp.x - Qm.x * Qp.y, cos_p = Qm.x * Qp.x + Qm.y * Qp.y;
-  const float sin_q = Qq.y * Qm.x - Qq.x * Qm.y, cos_q = Qq.x * Qm.x + Qq.y * Qm.y;
-  const float Sp = sinc_sel(hp, sin_p), Sq = sinc_sel(hq, sin_q);
-  const float dSp = dsinc_from(hp, cos_p, Sp), dSq = dsinc_from(hq, cos_q, Sq);
-  const float2 N = make_float2(Qq.x * Sq - Qp.x * Sp, Qq.y * Sq - Qp.y * Sp);
-  const float2 T = cmulf(Qm, N);
-  F = make_float2(2.0f * T.y * iD, -2.0f * T.x * iD);
-  const float2 dNq = cmulf(Qq, make_float2(0.5f * dSq, 0.5f * Sq));
-  const float2 dNp = cmulf(Qp, make_float2(0.5f * dSp, -0.5f * Sp));
-  const float2 Mq = cmulf(Qm, dNq), Mp = cmulf(Qm, dNp);
-  const float iD2 = iD * iD;
-  // -2i (x + i y) = 2y - 2i x
-  const float2 cq = make_float2(Mq.x * iD - T.x * iD2, Mq.y * iD - T.y * iD2);
-  const float2 cp = make_float2(Mp.x * iD + T.x * iD2, Mp.y * iD + T.y * iD2);
-  const float2 Gq = make_float2(2.0f * cq.y, -2.0f * cq.x);
-  const float2 Gp = make_float2(2.0f * cp.y, -2.0f * cp.x);
-  const float2 Gm = make_float2(-F.y - Gp.x - Gq.x, F.x - Gp.y - Gq.y);
-  G0 = make_float2(m0 ? Gm.x : Gp.x, m0 ? Gm.y : Gp.y);
-  G1 = make_float2(m0 ? Gp.x : (m1 ? Gm.x : Gq.x), m0 ? Gp.y : (m1 ? Gm.y : Gq.y));
-  G2 = make_float2(m01 ? Gq.x : Gm.x, m01 ? Gq.y : Gm.y);
-}
-
 // One workgroup per (patch, node range): one lane per node in blocks of 64
 // nodes; per block the waves fill an LDS table of the patch vertices' half
 // phasors, then stream the patch's faces (a quarter each, wave-uniform, scalar

@@ -1,0 +1,381 @@
+// Affine-DoF derivatives of the form factor at every representative node,
+// for the Gauss-Newton Hessian blocks of the volume matrix
+// (cpp/src/compute_intersection_volume_multi_object.cpp:236-292):
+//
+//   H_ij = (1/8 pi^3) sum_q w_q Re( dA_i/dtheta_i (dA_j/dtheta_j)^H ),
+//
+// a weighted Gram over derivative columns.  Object i owns columns 12 i + t,
+// t = [t_x, t_y, t_z, A_00, A_01, ..., A_22] (the AffineDoF layout), holding
+// X_t(q) = sqrt(w_q) dA/dtheta_t(k_q) in FP32 and the column max |X|.  On a
+// centrally symmetric grid dA(-k)/dtheta = -conj(dA(k)/dtheta), so the
+// representative nodes with summed weights give the full sum, as for V.
+//
+// Triangles, world vertices v_j = A vref_j + t (the derivative of
+// cpp/src/dof/affine_dof.cpp:171-232, taken through the vertices):
+//   dA_face/dv_j = kappa G_j k + F khat x m_j,
+//   m_0 = (e2 - e1)/2, m_1 = -e2/2, m_2 = e1/2,  F = 2 g[u], G_j = dF/du_j,
+//   dA/dt = i k A,
+//   dA/dA_rc = sum_faces [ kappa k_r sum_j G_j vref_j,c + F ([khat]_x N)_rc ],
+//   N_bc = sum_j m_j,b vref_j,c  (per face, from dtheta records).
+// Disks / Gaussians (affine_dof.cpp:233-316): with k' = A^T k, c' = A c + t,
+// reference centre c and normal n,
+//   dA/dt = i k A_e,
+//   dA/dA_rc = e^{i k.c'} [ i A_par k_r c_c + S khat_r n_c
+//                           + (k'.n / |k|) dS k_r (k'_c - (k'.n) n_c) ],
+//   disk: S = 2 area J1(x)/x (area for x < 1e-10), x = rho r,
+//         dS = 2 rho area (J1(x)/x)'/r (-rho^2 area / 4 for x < 1e-10);
+//   Gaussian: S = w exp(-sigma^2 r^2 / 2), dS = -sigma^2 S.
+#include "device_common.hpp"
+#include "pii_internal.hpp"
+#include "tri_derivs.hpp"
+
+namespace pii {
+
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kWaves = 4;
+constexpr int kRecDtheta = 18;  // floats per face: vref_0..2 (9), N (9, row b, column c)
+
+// Per face (patch order): reference corners and N_bc = sum_j m_j,b vref_j,c.
+__global__ void __launch_bounds__(256) k_dtheta_records(const ObjDesc* __restrict__ objs,
+                                                        const int* __restrict__ eprefix, int N,
+                                                        int total, float* __restrict__ out,
+                                                        const long long* __restrict__ ooff) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int o = find_owner(eprefix, N, i);
+  const ObjDesc& d = objs[o];
+  if (d.type != PII_GEOM_TRIANGLE) return;
+  const int e = i - eprefix[o];
+  const int f = (int)d.patch_faces[e];
+  const int i0 = d.faces[3 * f], i1 = d.faces[3 * f + 1], i2 = d.faces[3 * f + 2];
+  const double* W = d.world;
+  double e1[3], e2[3], m[3][3], vr[3][3];
+  for (int c = 0; c < 3; ++c) {
+    e1[c] = W[3 * i1 + c] - W[3 * i0 + c];
+    e2[c] = W[3 * i2 + c] - W[3 * i0 + c];
+    vr[0][c] = d.verts[3 * i0 + c];
+    vr[1][c] = d.verts[3 * i1 + c];
+    vr[2][c] = d.verts[3 * i2 + c];
+  }
+  for (int c = 0; c < 3; ++c) {
+    m[0][c] = 0.5 * (e2[c] - e1[c]);
+    m[1][c] = -0.5 * e2[c];
+    m[2][c] = 0.5 * e1[c];
+  }
+  float* r = out + ooff[o] + (size_t)kRecDtheta * e;
+  for (int j = 0; j < 3; ++j)
+    for (int c = 0; c < 3; ++c) r[3 * j + c] = (float)vr[j][c];
+  for (int b = 0; b < 3; ++b)
+    for (int c = 0; c < 3; ++c)
+      r[9 + 3 * b + c] = (float)(m[0][b] * vr[0][c] + m[1][b] * vr[1][c] + m[2][b] * vr[2][c]);
+}
+
+// J1(x)/x and its derivative -J2(x)/x for x >= 0 in FP64: the power series
+// in y = x^2/4 below 12 (25 terms), Hankel asymptotics of J0 and J1 above
+// (6 terms each of P and Q, coefficients from a_k(nu) = prod_{l<=k} (4 nu^2 -
+// (2l-1)^2) / (k! 8^k)).
+struct HankelPQ {
+  double p0[6], q0[6], p1[6], q1[6];
+  constexpr HankelPQ() : p0(), q0(), p1(), q1() {
+    for (int nu = 0; nu <= 1; ++nu) {
+      double a[13] = {};
+      a[0] = 1.0;
+      for (int k = 1; k <= 12; ++k)
+        a[k] = a[k - 1] * (4.0 * nu * nu - (2.0 * k - 1) * (2.0 * k - 1)) / (8.0 * k);
+      for (int k = 0; k < 6; ++k) {
+        const double sp = (k % 2 == 0) ? 1.0 : -1.0;
+        (nu == 0 ? p0 : p1)[k] = sp * a[2 * k];
+        (nu == 0 ? q0 : q1)[k] = sp * a[2 * k + 1];
+      }
+    }
+  }
+};
+__device__ constexpr HankelPQ kHankel{};
+
+struct J1OxSeries {
+  double c[25];
+  constexpr J1OxSeries() : c() {
+    double t = 0.5;
+    for (int m = 0; m < 25; ++m) {
+      c[m] = t;
+      t = -t / ((double)(m + 1) * (double)(m + 2));
+    }
+  }
+};
+__device__ constexpr J1OxSeries kJ1Ox{};
+
+__device__ __forceinline__ void j1ox_and_prime(double x, double& f, double& fp) {
+  if (x < 12.0) {
+    const double y = 0.25 * x * x;
+    double s = kJ1Ox.c[24], sp = 24.0 * kJ1Ox.c[24];
+#pragma unroll
+    for (int m = 23; m >= 0; --m) {
+      s = fma(s, y, kJ1Ox.c[m]);
+      if (m >= 1) sp = fma(sp, y, (double)m * kJ1Ox.c[m]);
+    }
+    f = s;
+    fp = sp * 0.5 * x;  // d/dx sum c_m y^m = sum m c_m y^(m-1) x/2
+    return;
+  }
+  const double inv = 1.0 / x, i2 = inv * inv;
+  double P0 = 0.0, Q0 = 0.0, P1 = 0.0, Q1 = 0.0;
+#pragma unroll
+  for (int k = 5; k >= 0; --k) {
+    P0 = fma(P0, i2, kHankel.p0[k]);
+    Q0 = fma(Q0, i2, kHankel.q0[k]);
+    P1 = fma(P1, i2, kHankel.p1[k]);
+    Q1 = fma(Q1, i2, kHankel.q1[k]);
+  }
+  Q0 *= inv;
+  Q1 *= inv;
+  double s0, c0, s1, c1;
+  sincos(x - 0.7853981633974483, &s0, &c0);   // x - pi/4
+  sincos(x - 2.356194490192345, &s1, &c1);    // x - 3 pi/4
+  const double amp = sqrt(0.6366197723675814 * inv);
+  const double J0 = amp * (P0 * c0 - Q0 * s0);
+  const double J1 = amp * (P1 * c1 - Q1 * s1);
+  f = J1 * inv;
+  fp = (J0 - 2.0 * J1 * inv) * inv;
+}
+
+__global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
+    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh,
+    const float* __restrict__ drec, const long long* __restrict__ doff, float2* __restrict__ Xt,
+    int ldX, unsigned* __restrict__ colmax) {
+  __shared__ float2 tab[kPatchVerts][kLanes];
+  const BlockObj bo = xcd_block_map();
+  const ObjDesc& d = objs[bo.obj];
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int q = bo.blk * kLanes + lane;
+  const int qc = q < Qh ? q : Qh - 1;
+  const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
+  const float hx = (float)krep[3 * (size_t)Qh + qc], hy = (float)krep[4 * (size_t)Qh + qc],
+              hz = (float)krep[5 * (size_t)Qh + qc];
+  const double km = krep[6 * (size_t)Qh + qc];
+  const float fk[3] = {(float)kx, (float)ky, (float)kz};
+  const float fh[3] = {hx, hy, hz};
+  // accumulators: A (for dA/dt = i k A) and dA/dA_rc, FP32 per patch / element
+  // block, FP64 across them
+  double sA[2] = {0.0, 0.0}, gA[9][2] = {};
+
+  if (d.type == PII_GEOM_TRIANGLE) {
+    const auto* hdr = as_const(d.patch_hdr);
+    const auto* pverts = as_const(d.patch_verts);
+    const auto* world = as_const(d.world);
+    const auto* rec = as_const(d.rec);
+    const auto* dr = as_const(drec) + doff[bo.obj];
+    for (int p = 0; p < d.npatch; ++p) {
+      const int vbeg = hdr[4 * p], vcnt = hdr[4 * p + 1];
+      const int fbeg = hdr[4 * p + 2], fcnt = hdr[4 * p + 3];
+      for (int s = wave; s < vcnt; s += kWaves) {
+        const int gv = pverts[vbeg + s];
+        const double u = kx * world[3 * gv] + ky * world[3 * gv + 1] + kz * world[3 * gv + 2];
+        tab[s][lane] = half_phasor_a(u);
+      }
+      __syncthreads();
+      float pA[2] = {0.f, 0.f}, pg[9][2] = {};
+      for (int f = wave; f < fcnt; f += kWaves) {
+        const auto* r = rec + (size_t)kRecFastTri * (fbeg + f);
+        const double al = kx * r[0] + ky * r[1] + kz * r[2];
+        const double be = kx * r[3] + ky * r[4] + kz * r[5];
+        const auto* rf = reinterpret_cast<const PII_CONST_AS float*>(r + 6);
+        const float kap = 0.5f * (hx * rf[0] + hy * rf[1] + hz * rf[2]);
+        const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
+        const float2 Q0 = tab[loc & 0xff][lane], Q1 = tab[(loc >> 8) & 0xff][lane],
+                     Q2 = tab[(loc >> 16) & 0xff][lane];
+        const double H = fmax(0.0, fmax(al, be)) - fmin(0.0, fmin(al, be));
+        float2 F, G0, G1, G2;
+        if (H < kSpanTaylor) {
+          double2 T0, T1, T2;
+          dd3_taylor(al, be, T0, T1, T2);
+          const double2 tf = phi_taylor(al, be);
+          const float2 P0 = cmulf(Q0, Q0);
+          F = cmulf(P0, make_float2((float)tf.x, (float)tf.y));
+          G0 = cmulf(P0, make_float2((float)T0.x, (float)T0.y));
+          G1 = cmulf(P0, make_float2((float)T1.x, (float)T1.y));
+          G2 = cmulf(P0, make_float2((float)T2.x, (float)T2.y));
+        } else {
+          tri_main_adj(al, be, Q0, Q1, Q2, F, G0, G1, G2);
+        }
+        const auto* t = dr + (size_t)kRecDtheta * (fbeg + f);  // vref (9), N (9)
+        pA[0] += kap * F.x;
+        pA[1] += kap * F.y;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          // gv_c = sum_j G_j vref_j,c
+          const float gx = G0.x * t[c] + G1.x * t[3 + c] + G2.x * t[6 + c];
+          const float gy = G0.y * t[c] + G1.y * t[3 + c] + G2.y * t[6 + c];
+          // ([khat]_x N)_rc
+          const float D0 = fh[1] * t[9 + 6 + c] - fh[2] * t[9 + 3 + c];
+          const float D1 = fh[2] * t[9 + c] - fh[0] * t[9 + 6 + c];
+          const float D2 = fh[0] * t[9 + 3 + c] - fh[1] * t[9 + c];
+          const float Dr[3] = {D0, D1, D2};
+#pragma unroll
+          for (int rr = 0; rr < 3; ++rr) {
+            const float kk = kap * fk[rr];
+            pg[3 * rr + c][0] += kk * gx + F.x * Dr[rr];
+            pg[3 * rr + c][1] += kk * gy + F.y * Dr[rr];
+          }
+        }
+      }
+      sA[0] += (double)pA[0];
+      sA[1] += (double)pA[1];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        gA[i][0] += (double)pg[i][0];
+        gA[i][1] += (double)pg[i][1];
+      }
+      __syncthreads();
+    }
+  } else {
+    const auto* p = as_const(d.dofs);
+    const auto* rec = as_const(d.rec);
+    const auto* el = as_const(d.el);
+    const bool disk = d.type == PII_GEOM_DISK;
+    // k' = A^T k
+    const double kp[3] = {p[3] * kx + p[6] * ky + p[9] * kz, p[4] * kx + p[7] * ky + p[10] * kz,
+                          p[5] * kx + p[8] * ky + p[11] * kz};
+    const double kp2 = kp[0] * kp[0] + kp[1] * kp[1] + kp[2] * kp[2];
+    const double inv_km = km > 0.0 ? 1.0 / km : 0.0;
+    float pA[2] = {0.f, 0.f}, pg[9][2] = {};
+    int cnt = 0;
+    for (int e = wave; e < d.n; e += kWaves) {
+      const auto* r = rec + (size_t)kRecSurfel * e;   // c' (world), A n, rho|sigma, area|w
+      const auto* s = el + (size_t)kRecSurfel * e;    // c, n (reference), rho|sigma, area|w
+      const double kdn = kp[0] * s[3] + kp[1] * s[4] + kp[2] * s[5];
+      double r2 = kp2 - kdn * kdn;
+      double S, dS;
+      if (disk) {
+        r2 = fmax(0.0, r2);
+        const double rr = sqrt(r2);
+        const double x = s[6] * rr;
+        if (x < 1e-10) {
+          S = s[7];
+          dS = -0.25 * s[6] * s[6] * s[7];
+        } else {
+          double f, fp;
+          j1ox_and_prime(x, f, fp);
+          S = 2.0 * s[7] * f;
+          dS = 2.0 * fp / rr * s[6] * s[7];
+        }
+      } else {
+        S = s[7] * exp(-0.5 * s[6] * s[6] * r2);
+        dS = -s[6] * s[6] * S;
+      }
+      const double Apar = S * kdn * inv_km;
+      const double tau = kdn * inv_km * dS;
+      const float2 ph = half_phasor_a(2.0 * (kx * r[0] + ky * r[1] + kz * r[2]));
+      // e^{i u} from the half-phasor routine on 2u
+      const float fA = (float)Apar;
+      pA[0] += fA * ph.x;
+      pA[1] += fA * ph.y;
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float im = (float)(Apar * (rr == 0 ? kx : rr == 1 ? ky : kz) * s[c]);
+          const float re = (float)(S * (rr == 0 ? hx : rr == 1 ? hy : hz) * s[3 + c] +
+                                   tau * (rr == 0 ? kx : rr == 1 ? ky : kz) *
+                                       (kp[c] - kdn * s[3 + c]));
+          // (re + i im) e^{i u}
+          pg[3 * rr + c][0] += re * ph.x - im * ph.y;
+          pg[3 * rr + c][1] += re * ph.y + im * ph.x;
+        }
+      }
+      if (++cnt == 64) {
+        sA[0] += (double)pA[0]; sA[1] += (double)pA[1];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+          gA[i][0] += (double)pg[i][0]; gA[i][1] += (double)pg[i][1];
+          pg[i][0] = pg[i][1] = 0.f;
+        }
+        pA[0] = pA[1] = 0.f;
+        cnt = 0;
+      }
+    }
+    sA[0] += (double)pA[0]; sA[1] += (double)pA[1];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      gA[i][0] += (double)pg[i][0]; gA[i][1] += (double)pg[i][1];
+    }
+  }
+  if (km < 1e-10) {  // dA/dtheta = 0 at k = 0 (affine_dof.cpp:181)
+    sA[0] = sA[1] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gA[i][0] = gA[i][1] = 0.0;
+  }
+
+  // Sum the four waves' partial sums of the same 64 nodes (through LDS: the
+  // phasor table is free after the last patch's barrier).
+  double* red = reinterpret_cast<double*>(&tab[0][0]);  // 20 x 64 doubles per wave slot
+  static_assert(sizeof(double) * 20 * kLanes * 2 <= sizeof(float2) * kPatchVerts * kLanes,
+                "reduction buffer must fit in the phasor table");
+  double v[20];
+  v[0] = sA[0];
+  v[1] = sA[1];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    v[2 + 2 * i] = gA[i][0];
+    v[3 + 2 * i] = gA[i][1];
+  }
+  __syncthreads();
+  // (w0 + w2) + (w1 + w3) in a fixed order, two slots
+  if (wave >= 2) {
+#pragma unroll
+    for (int i = 0; i < 20; ++i) red[((wave - 2) * 20 + i) * kLanes + lane] = v[i];
+  }
+  __syncthreads();
+  if (wave < 2) {
+#pragma unroll
+    for (int i = 0; i < 20; ++i) v[i] += red[(wave * 20 + i) * kLanes + lane];
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int i = 0; i < 20; ++i) red[i * kLanes + lane] = v[i];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int i = 0; i < 20; ++i) v[i] += red[i * kLanes + lane];
+  const double sw = (q < Qh) ? sqrt(krep[7 * (size_t)Qh + qc]) : 0.0;
+  const double kv[3] = {kx, ky, kz};
+  float2* X = Xt + (size_t)(12 * d.col) * ldX;
+  for (int t = 0; t < 12; ++t) {
+    double re, im;
+    if (t < 3) {  // i k_t A
+      re = -kv[t] * v[1];
+      im = kv[t] * v[0];
+    } else {
+      re = v[2 + 2 * (t - 3)];
+      im = v[3 + 2 * (t - 3)];
+    }
+    const float2 x = make_float2((float)(re * sw), (float)(im * sw));
+    if (q < ldX) X[(size_t)t * ldX + q] = (q < Qh) ? x : make_float2(0.f, 0.f);
+    float mval = (q < Qh) ? fmaxf(fabsf(x.x), fabsf(x.y)) : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) mval = fmaxf(mval, __shfl_xor(mval, off));
+    if (lane == 0) atomicMax(colmax + 12 * d.col + t, __float_as_uint(mval));
+  }
+}
+
+}  // namespace
+
+void launch_dtheta_records(hipStream_t s, const ObjDesc* d_objs, const int* d_eprefix, int N,
+                           int total_e, float* d_out, const long long* d_off) {
+  if (total_e <= 0) return;
+  hipLaunchKernelGGL(k_dtheta_records, dim3((total_e + 255) / 256), dim3(256), 0, s, d_objs,
+                     d_eprefix, N, total_e, d_out, d_off);
+}
+
+void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
+                          int Qh, const float* d_drec, const long long* d_doff, float2* d_Xt,
+                          int ldX, unsigned* d_colmax) {
+  if (N <= 0 || Qh <= 0) return;
+  hipLaunchKernelGGL(k_dtheta_affine, dim3((ldX + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves),
+                     0, s, d_objs, d_krep, Qh, d_drec, d_doff, d_Xt, ldX, d_colmax);
+}
+
+}  // namespace pii

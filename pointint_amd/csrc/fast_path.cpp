@@ -93,6 +93,61 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
   PII_HIP(hipStreamSynchronize(s));
 }
 
+void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                           double* H_host, double* V_host) {
+  const int N = b.N, Qh = kg.Qh, D = 12 * N;
+  for (const ObjDesc& d : b.h_objs)
+    if (d.dof_kind != PII_DOF_AFFINE)
+      throw Error(PII_ERR_UNSUPPORTED, "Gauss-Newton Hessian supports AffineDoF objects only");
+  if (V_host) volume_matrix_fast(c, b, kg, gram_mode, V_host);
+  hipStream_t s = c.stream;
+  const int ldX = (int)align_up((size_t)Qh, 64);
+  // per-face records of triangle objects
+  std::vector<long long> off(N, 0);
+  size_t tot = 0;
+  for (int slot = 0; slot < N; ++slot) {
+    off[slot] = (long long)tot;
+    if (b.h_objs[slot].type == PII_GEOM_TRIANGLE) tot += align_up((size_t)b.h_objs[slot].n * 18, 64);
+  }
+  const size_t rec_bytes = align_up(sizeof(float) * std::max<size_t>(tot, 1), 256);
+  c.w.dth.ensure(rec_bytes + sizeof(long long) * N);
+  float* d_drec = c.w.dth.as<float>();
+  long long* d_off = reinterpret_cast<long long*>(static_cast<char*>(c.w.dth.p) + rec_bytes);
+  PII_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(long long) * N, hipMemcpyHostToDevice, s));
+  launch_dtheta_records(s, b.d_objs, b.d_eprefix, N, b.total_e, d_drec, d_off);
+  PII_HIP(hipGetLastError());
+  c.w.Xt.ensure(sizeof(float2) * (size_t)ldX * D);
+  c.w.cmax2.ensure(sizeof(unsigned) * (size_t)D);
+  unsigned* colmax = c.w.cmax2.as<unsigned>();
+  PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * D, s));
+  c.phase_begin(1);
+  launch_dtheta_affine(s, b.d_objs, N, kg.rep.as<double>(), Qh, d_drec, d_off,
+                       c.w.Xt.as<float2>(), ldX, colmax);
+  c.phase_end(1);
+  PII_HIP(hipGetLastError());
+  c.w.H.ensure(sizeof(double) * (size_t)D * D);
+  c.phase_begin(2);
+  if (gram_mode == PII_GRAM_FP64) {
+    const int nsplit = gram_fp64_splits(Qh, D);
+    c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
+    launch_gram_fp64_x(s, c.w.Xt.as<float2>(), Qh, D, ldX, c.w.part.as<double>(), nsplit,
+                       c.w.H.as<double>());
+  } else if (gram_mode == PII_GRAM_BF16X3 || gram_mode == PII_GRAM_FP16X3) {
+    const int K = 2 * ldX;
+    const int nsplit = gram_split_splits(K, D);
+    c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
+    launch_gram_split(s, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.Xt.as<float>(), D, K, colmax,
+                      c.w.part.as<double>(), nsplit, c.w.H.as<double>());
+  } else {
+    throw Error(PII_ERR_INVALID, "unknown Gram mode");
+  }
+  c.phase_end(2);
+  PII_HIP(hipGetLastError());
+  PII_HIP(hipMemcpyAsync(H_host, c.w.H.p, sizeof(double) * (size_t)D * D, hipMemcpyDeviceToHost,
+                         s));
+  PII_HIP(hipStreamSynchronize(s));
+}
+
 void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geoms,
                        const KGridDev& kg, const double* G, double* const* grads,
                        double* V_host) {

@@ -87,6 +87,12 @@ void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, doubl
 void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
                         double* V_host);
 
+// Gauss-Newton Hessian blocks H (12N x 12N, row-major; block (i, j) rows
+// 12 i.., columns 12 j..) of affine-DoF objects; V (may be null) of the same
+// call.
+void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                           double* H_host, double* V_host);
+
 // Vertex-DoF / affine-DoF gradients of L = sum_ij G_ij V_ij (triangles).
 void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geoms,
                        const KGridDev& kg, const double* G, double* const* grads,

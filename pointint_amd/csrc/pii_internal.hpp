@@ -133,6 +133,7 @@ struct GeometryDev {
 // Per-call device workspaces (grow-only, reused across calls on one stream).
 struct Workspace {
   DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw, mrec, gam;
+  DevBuf dth, Xt, cmax2, H;  // Gauss-Newton Hessian: records, dA/dtheta columns, maxima, H
 };
 
 struct Context {
@@ -209,6 +210,14 @@ void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double
 // The same for disk and Gaussian objects.
 void launch_series_surfel(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
                           int Qh, int Mh, int out_mode, double2* d_A, int ldA, float2* d_X,
+                          int ldX, unsigned* d_colmax);
+// Affine-DoF derivatives for the Gauss-Newton Hessian: per-face records
+// (reference corners + N, 18 floats, patch order, at d_off[slot]) and the
+// columns X_t = sqrt(w) dA/dtheta_t (12 per object, column 12 col + t).
+void launch_dtheta_records(hipStream_t s, const ObjDesc* d_objs, const int* d_eprefix, int N,
+                           int total_e, float* d_out, const long long* d_off);
+void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
+                          int Qh, const float* d_drec, const long long* d_doff, float2* d_Xt,
                           int ldX, unsigned* d_colmax);
 // Full-grid values from representative values: A(-k) = -conj(A(k)).
 void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,

@@ -1,7 +1,8 @@
 // Example C++ caller: the two-cube configuration through the reference-named
 // C++ API (Lebedev 302 x 32 k-grid, unit cube and a copy shifted by
-// (0.3, 0.2, 0.1)).  Prints the 2 x 2 volume matrix, then the pairwise volume
-// and its gradient w.r.t. the second cube's affine DoFs.
+// (0.3, 0.2, 0.1)).  Prints the 2 x 2 volume matrix, the pairwise volume and
+// its gradient w.r.t. the second cube's affine DoFs, and the translation
+// corner of the pair's Gauss-Newton cross Hessian.
 //
 //   usage: example_cube_pair <lebedev_029.txt> [fp64|fast]
 #include <cstdio>
@@ -39,6 +40,16 @@ int main(int argc, char** argv) {
                                                  ComputationFlags::GRADIENT);
   std::printf("pair %.15f grad_t2 %.15f %.15f %.15f\n", pr.volume, pr.grad_geom2[0],
               pr.grad_geom2[1], pr.grad_geom2[2]);
+  // Gauss-Newton Hessian of the pair: the translation-translation corner of
+  // the cross block
+  auto ph = compute_intersection_volume_cuda(geoms[0], geoms[1], aff, aff, id, id, kg,
+                                             ComputationFlags::VOLUME_ONLY |
+                                                 ComputationFlags::HESSIAN);
+  if (ph.hessian_cross.rows() != 12) return 1;
+  std::printf("hcross_tt");
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) std::printf(" %.15e", ph.hessian_cross(a, b));
+  std::printf("\n");
   std::printf("self %.15f\n", compute_volume_cpu(geoms[0]));
   return 0;
 }

@@ -350,8 +350,25 @@ IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
       }
     }
   }
-  if (needs_hessian(flags))
-    std::cerr << "PoIntInt: Gauss-Newton Hessians are not implemented in this build" << std::endl;
+  if (needs_hessian(flags)) {
+    // Gauss-Newton blocks (compute_intersection_volume_multi_object.cpp:236-292):
+    // hessian_ij[i][j] from the device Gram of dA/dtheta columns; hessian_ii /
+    // hessian_jj are zero in this form.  AffineDoF objects only.
+    const int D = 12 * n;
+    std::vector<double> H((size_t)D * D);
+    if (!report(pii_volume_matrix_hessian(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(),
+                                          co.kg, g_gram, H.data(), nullptr),
+                "hessian"))
+      return res;
+    res.hessian_ii.assign(n, std::vector<MatrixXd>(n, MatrixXd::Zero(12, 12)));
+    res.hessian_jj.assign(n, std::vector<MatrixXd>(n, MatrixXd::Zero(12, 12)));
+    res.hessian_ij.assign(n, std::vector<MatrixXd>(n, MatrixXd(12, 12)));
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        for (int a = 0; a < 12; ++a)
+          for (int c = 0; c < 12; ++c)
+            res.hessian_ij[i][j](a, c) = H[(size_t)(12 * i + a) * D + 12 * j + c];
+  }
   return res;
 }
 
@@ -379,6 +396,12 @@ IntersectionVolumeResult compute_intersection_volume_cuda(
   if (needs_gradient(flags) && m.grad_matrix.size() == 2) {
     r.grad_geom1 = m.grad_matrix[0][1];
     r.grad_geom2 = m.grad_matrix[1][0];
+  }
+  // compute_intersection_volume_cpu.cpp:116-149: geom1 / geom2 / cross blocks
+  if (needs_hessian(flags) && m.hessian_ij.size() == 2) {
+    r.hessian_geom1 = m.hessian_ij[0][0];
+    r.hessian_geom2 = m.hessian_ij[1][1];
+    r.hessian_cross = m.hessian_ij[0][1];
   }
   return r;
 }

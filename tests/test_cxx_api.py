@@ -37,6 +37,8 @@ def _run(mode):
             vals["grad_t2"] = np.array([float(x) for x in parts[3:6]])
         elif parts[0] == "self":
             vals["self"] = float(parts[1])
+        elif parts[0] == "hcross_tt":
+            vals["hcross_tt"] = np.array([float(x) for x in parts[1:10]]).reshape(3, 3)
     return vals
 
 
@@ -55,3 +57,9 @@ def test_cxx_example_cube_pair(oracle, mode, tol):
     B = z["J"][:, 0]
     ref = oracle.vjp(g1, oracle.IDENTITY_DOFS, oracle.DOF_AFFINE, dirs, kmag, wq, B)[:3]
     assert np.allclose(v["grad_t2"], ref, rtol=1e-4, atol=1e-9)
+    # Gauss-Newton cross block, translations: dA/dt = i k A, so
+    # H_tt = (1/8 pi^3) sum_q w_q k_a k_b Re(A_0 conj A_1)
+    k = dirs * kmag[:, None]
+    c = wq * (z["J"][:, 0] * np.conj(z["J"][:, 1])).real / (8 * np.pi ** 3)
+    H_tt = (k * c[:, None]).T @ k
+    assert np.allclose(v["hcross_tt"], H_tt, rtol=1e-4, atol=1e-9 * np.abs(H_tt).max())

@@ -1,0 +1,82 @@
+"""GPU Gauss-Newton Hessian blocks (AffineDoF) against the FP64 oracle.
+
+Reference: the HESSIAN branch of the reference's matrix driver
+(cpp/src/compute_intersection_volume_multi_object.cpp:236-292),
+H_ij = (1/8 pi^3) sum_q w_q Re(dA_i/dtheta_i (dA_j/dtheta_j)^H), assembled here
+from the oracle's per-node affine gradients (`oracle.compute_A_gradient`,
+itself pinned against central differences in test_oracle_pins.py).
+Gate: ||H - H64||_F / ||H64||_F < 1e-5, and every diagonal block to 1e-4 of
+its own norm.
+"""
+import numpy as np
+import pytest
+
+import pointint_amd as P
+from pointint_amd import _native as N
+from pointint_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(oracle, rng):
+    V1, F1 = W.icosphere(1)
+    Pc, Nn, _ = W.sphere_pointcloud(120)
+    out = []
+    # triangle mesh, disks, Gaussians, each under a rotation + scale + shift
+    out.append((P.make_triangle_mesh(V1 * 0.45, F1), oracle.geom_mesh(V1 * 0.45, F1)))
+    rho = rng.uniform(0.02, 0.05, size=120)
+    out.append((P.make_point_cloud(Pc * 0.4, Nn, rho), oracle.geom_disks(Pc * 0.4, Nn, rho)))
+    sig = rng.uniform(0.02, 0.05, size=120)
+    wt = rng.uniform(0.5, 1.5, size=120) * 0.01
+    out.append((P.make_gaussian_splat(Pc * 0.35, Nn, sig, wt),
+                oracle.geom_gaussians(Pc * 0.35, Nn, sig, wt)))
+    dofs = []
+    for _ in out:
+        R = W.random_rotation(rng)
+        A = R @ np.diag(rng.uniform(0.8, 1.2, 3))
+        t = rng.uniform(-0.2, 0.2, size=3)
+        dofs.append(np.concatenate([t, A.ravel()]))
+    return out, dofs
+
+
+@pytest.mark.parametrize("gram", ["fp64", "fp16x3"])
+def test_hessian_matches_oracle(oracle, gram):
+    rng = np.random.default_rng(51)
+    objs, dofs = _scene(oracle, rng)
+    d, w = oracle.load_lebedev(P.lebedev_path(9))
+    dirs, kmag, wq = oracle.build_kgrid(d, w, 8)
+    kg = P.KGrid(dirs, kmag, wq)
+    n = len(objs)
+    G = np.zeros((len(kmag), 12 * n), dtype=np.complex128)
+    for i, ((_, og), x) in enumerate(zip(objs, dofs)):
+        for q in range(len(kmag)):
+            G[q, 12 * i:12 * i + 12] = oracle.compute_A_gradient(og, dirs[q] * kmag[q], x,
+                                                                 oracle.DOF_AFFINE)
+    H64 = (G.T * wq) @ np.conj(G)
+    H64 = H64.real / (8 * np.pi ** 3)
+    H, V = P.volume_matrix_hessian([g for g, _ in objs], kg, dofs, gram=gram, return_volume=True)
+    assert H.shape == (12 * n, 12 * n)
+    assert np.allclose(H, H.T, rtol=0, atol=1e-12 * np.abs(H).max())
+    e = np.linalg.norm(H - H64) / np.linalg.norm(H64)
+    print(f"{gram}: ||H - H64|| / ||H64|| = {e:.2e}")
+    assert e < 1e-5, e
+    for i in range(n):
+        b, b64 = H[12 * i:12 * i + 12, 12 * i:12 * i + 12], H64[12 * i:12 * i + 12, 12 * i:12 * i + 12]
+        assert np.linalg.norm(b - b64) / np.linalg.norm(b64) < 1e-4, i
+    Vo, _ = oracle.volume_matrix([og for _, og in objs], dirs, kmag, wq, dofs,
+                                 [oracle.DOF_AFFINE] * n)
+    dd = np.sqrt(np.outer(np.diag(Vo), np.diag(Vo)))
+    assert np.max(np.abs(V - Vo) / dd) < 1e-5
+
+
+def test_hessian_rejects_vertex_dofs():
+    V1, F1 = W.icosphere(1)
+    kg = P.kgrid_from_rule(9, 4)
+    g = P.make_triangle_mesh(V1, F1)
+    with pytest.raises(N.PiiError):
+        N.check(N.load().pii_volume_matrix_hessian(
+            g.ctx.handle, 1, np.array([g.handle.value], dtype=np.uintp).ctypes.data,
+            np.array([1], dtype=np.int32).ctypes.data,
+            np.array([V1.ravel().ctypes.data], dtype=np.uintp).ctypes.data,
+            np.array([V1.size], dtype=np.int32).ctypes.data, kg.handle, 0,
+            np.zeros((V1.size, V1.size)).ctypes.data, None))
