@@ -202,6 +202,105 @@ def run_diff(args):
     print(json.dumps(out))
 
 
+def align_pair(level=5, seed=8):
+    """Two star-shaped anisotropic icospheres (level 5: 20,480 triangles), the
+    second rotated by 20 degrees, scaled by 0.95 and shifted."""
+    rng = np.random.default_rng(seed)
+    V, F = W.icosphere(level)
+    V = W.star_perturb(V, rng, 0.1) * np.array([0.5, 0.35, 0.22])
+    ax = np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8])
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    th = np.deg2rad(20.0)
+    R = np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
+    return (V, F), (0.95 * V @ R.T + np.array([0.05, -0.03, 0.04]), F)
+
+
+def run_align(args):
+    """Newton iteration of the alignment consumer on the device: the SIM_ALL
+    evaluation (V, per-pair gradients and the Gauss-Newton block of the pair,
+    one pass) plus one function-only line-search evaluation, for two
+    20,480-triangle meshes on the application's default grid (Lebedev 302 x
+    L=32, Q = 9,664).  Also runs the C++ optimizer (pointint_align) end to
+    end on the same meshes."""
+    import subprocess
+    import tempfile
+    (V1, F1), (V2, F2) = align_pair()
+    kg = P.kgrid_from_rule(29, 32)
+    g1, g2 = P.make_triangle_mesh(V1, F1), P.make_triangle_mesh(V2, F2)
+    ctx = P.Context.default()
+    ident = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], dtype=np.float64)
+    x = np.concatenate([[0.01, -0.02, 0.01], np.eye(3).ravel() * 1.02])
+
+    def iteration():
+        V, G, H = P.volume_matrix_derivatives([g1, g2], kg, [ident, x], gram="fp16x3")
+        Vf = P.compute_volume_matrix([g1, g2], kg, [ident, x], ["affine"] * 2, gram="fp16x3")
+        return V, G, H, Vf
+
+    for _ in range(args.warmup):
+        iteration()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        iteration()
+    t1 = time.perf_counter()
+    tm = ctx.timings()
+    ctx.set_profiling(False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    # end to end: warm start + Newton in C++ (one process, geometry resident)
+    e2e = None
+    exe = os.path.join(ROOT, "pointint_amd", "pointint_align")
+    with tempfile.TemporaryDirectory() as td:
+        paths = []
+        for nm, (V, F) in (("m1", (V1, F1)), ("m2", (V2, F2))):
+            pth = os.path.join(td, nm + ".off")
+            with open(pth, "w") as f:
+                f.write(f"OFF\n{len(V)} {len(F)} 0\n")
+                np.savetxt(f, V, fmt="%.17g")
+                np.savetxt(f, np.c_[np.full(len(F), 3), F], fmt="%d")
+            paths.append(pth)
+        r = subprocess.run([exe, *paths, "29", "32", "--measure", "dice", "--max-iter", "30"],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode == 0:
+            res = {ln[1:].split()[0]: ln[1:].split()[1:] for ln in r.stdout.splitlines()
+                   if ln.startswith("@")}
+            n_eval = r.stdout.count("Trying alpha")
+            e2e = {"converged": bool(int(res["final"][0])),
+                   "iterations": int(res["final"][1]) + 1, "dice": float(res["final"][2]),
+                   "total_ms": float(res["final"][4]), "line_search_evals": n_eval,
+                   "ms_per_iteration": float(res["final"][4]) / (int(res["final"][1]) + 1),
+                   "status": " ".join(res["status"])}
+        else:
+            print(f"[bench] pointint_align failed: {r.stderr[-500:]}", file=sys.stderr)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        import align_oracle as AO
+        og1, og2 = O.geom_mesh(V1, F1), O.geom_mesh(V2, F2)
+        stride = max(1, kg.Q // 48)
+        idx = np.arange(0, kg.Q, stride)
+        t0 = time.perf_counter()
+        AO.pair_terms(O, og1, og2, x, kg.dirs[idx], kg.kmag[idx], kg.w[idx], AO.ALL)
+        AO.pair_terms(O, og1, og2, x, kg.dirs[idx], kg.kmag[idx], kg.w[idx], AO.FUNC)
+        t_s = time.perf_counter() - t0
+        cpu = {"value": 1.0 / (t_s * kg.Q / len(idx)), "unit": "Newton iterations/s",
+               "cores": O.threads(), "kind": "port",
+               "sample": f"{len(idx)} of {kg.Q} k-nodes (one SIM_ALL + one function "
+                         f"evaluation), extrapolated linearly in Q; {t_s:.1f} s on "
+                         f"{cpu_model()}"}
+    out = {
+        "metric": "alignment: Newton iterations/s (SIM_ALL evaluation + one line-search "
+                  "evaluation, N=2)",
+        "value": 1e3 / ms, "unit": "Newton iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded star-shaped meshes)",
+        "config": {"workload": "2 meshes x 20,480 triangles, AffineDoF, Lebedev 302 x L=32 "
+                               "(Q=9,664), Dice", "Q": kg.Q},
+        "phase_ms": tm, "end_to_end": e2e, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def run_serving(args):
     """Poisson query stream against a resident pool of star-shaped meshes."""
     from pointint_amd import serving as S
@@ -252,7 +351,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--n-obj", type=int, default=256)
     ap.add_argument("--workload", default="icospheres",
-                    choices=["icospheres", "mixed", "diff", "serving"])
+                    choices=["icospheres", "mixed", "diff", "serving", "align"])
     ap.add_argument("--fd-vertices", type=int, default=200)
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--duration", type=float, default=10.0)
@@ -275,6 +374,8 @@ def main():
         return run_diff(args)
     if args.workload == "serving":
         return run_serving(args)
+    if args.workload == "align":
+        return run_align(args)
     if args.workload == "mixed":
         wl = W.config_mixed()
         if args.gram == "bf16x3" and "--gram" not in sys.argv:

@@ -139,6 +139,9 @@ struct DoFParameterization {
 struct AffineDoF : DoFParameterization {
   int num_dofs() const override { return 12; }
   int dof_kind() const override { return 0; }
+  // |det A| x the divergence-theorem volume of the reference geometry
+  // (cpp/src/dof/affine_dof.cpp:324-384), on the host.
+  double compute_volume(const Geometry& geom, const VectorXd& dofs) const;
 };
 // [v0x, v0y, v0z, v1x, ...]
 struct TriangleMeshDoF : DoFParameterization {

@@ -95,7 +95,8 @@ int pii_context_synchronize(pii_context* ctx);
 /* Phase timing with HIP events on the context's stream (the reference's
  * enable_profiling flag, e.g.
  * cpp/src/compute_intersection_volume_multi_object.cu:626-695, which used host
- * clocks).  Phases: 0 pre-pass, 1 form factor, 2 Gram, 3 adjoint, and within
+ * clocks).  Phases: 0 pre-pass, 1 form factor, 2 Gram, 3 derivatives (the
+ * adjoint, or the forward dA/dtheta columns of the affine entries), and within
  * phase 1 on the fast path: 4 moment-series kernel, 5 per-face kernel; ms[p]
  * holds the last call's duration of phase p (0 when it did not run). */
 int pii_context_set_profiling(pii_context* ctx, int enable);
@@ -178,6 +179,25 @@ int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
 int pii_volume_matrix_hessian(pii_context* ctx, int n, const pii_geometry* const* geoms,
                               const int* dof_kinds, const double* const* dofs, const int* ndofs,
                               const pii_kgrid* kg, int gram_mode, double* H, double* V);
+
+/* Volume matrix, per-pair gradients and Gauss-Newton Hessian blocks of
+ * AffineDoF objects (triangles, disks, Gaussians) from one pass: the fast form
+ * factors A_i, the forward-mode columns dA_i/dtheta_i and one Gram over all
+ * 13n columns.  Outputs (each may be NULL, not all):
+ *   V[n*n]          volume matrix (row-major);
+ *   grad[n*n*12]    grad[(i*n + j)*12 + t] = dV_ij/dtheta_i,t
+ *                   = (1/8 pi^3) sum_q w_q Re(dA_i/dtheta_t conj A_j), which
+ *                   for i == j is half the full derivative of V_ii;
+ *   H[12n*12n]      as pii_volume_matrix_hessian.
+ * Replaces compute_intersection_volume_matrix_cuda with GRADIENT and/or
+ * HESSIAN for AffineDoF objects: its grad_matrix[i][j] is grad block (i, j)
+ * (cpp/src/compute_intersection_volume_multi_object.cu:437-520, from dA/dtheta
+ * as here; cpp/include/compute_intersection_volume_multi_object.hpp:22-26 for
+ * the i == j convention), its hessian_ij[i][j] block (i, j) of H
+ * (compute_intersection_volume_multi_object.cu:186-227, 523-600). */
+int pii_volume_matrix_affine_derivatives(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                                         const double* const* dofs, const pii_kgrid* kg,
+                                         int gram_mode, double* V, double* grad, double* H);
 
 /* Adjoint: gradients of L = sum_ij dL_dV[i*n+j] V_ij with respect to every
  * object's DoFs, written to grads[i] (ndofs[i] doubles), without forming

@@ -25,4 +25,5 @@ from .core import (  # noqa: F401
     make_triangle_mesh,
     volume_matrix_vjp,
     volume_matrix_hessian,
+    volume_matrix_derivatives,
 )

@@ -19,6 +19,7 @@ OUT = os.path.join(HERE, "libpointint_hip.so")
 CXX_DIR = os.path.join(HERE, "cxx")
 OUT_CXX = os.path.join(HERE, "libpointint_cxx.so")
 OUT_EXAMPLE = os.path.join(HERE, "example_cube_pair")
+OUT_ALIGN = os.path.join(HERE, "pointint_align")
 ROCM_INCLUDE = "/opt/rocm/include"
 BUILD_DIR = os.path.join(HERE, "_build")
 
@@ -62,30 +63,31 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, OUT)
     build_cxx()
     if verbose:
-        print(f"built {OUT}, {OUT_CXX}, {OUT_EXAMPLE}")
+        print(f"built {OUT}, {OUT_CXX}, {OUT_EXAMPLE}, {OUT_ALIGN}")
     return OUT
 
 
 def build_cxx() -> None:
-    """The reference-named C++ API (host code on the C ABI) and its example,
-    compiled with g++ and linked against libpointint_hip.so."""
+    """The reference-named C++ API (host code on the C ABI: matrix driver and
+    alignment) and its two programs, compiled with g++ and linked against
+    libpointint_hip.so."""
     cxx = os.environ.get("CXX", "g++")
     common = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE,
               "-I", ROCM_INCLUDE, "-Wall"]
-    tmp = OUT_CXX + ".tmp"
-    cmd = [cxx, *common, "-shared", "-o", tmp, os.path.join(CXX_DIR, "pointint_cxx.cpp"),
-           "-L", HERE, "-lpointint_hip", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"C++ API build failed: {' '.join(cmd)}\n{r.stderr}")
-    os.replace(tmp, OUT_CXX)
-    tmp = OUT_EXAMPLE + ".tmp"
-    cmd = [cxx, *common, "-o", tmp, os.path.join(CXX_DIR, "example_cube_pair.cpp"),
-           "-L", HERE, "-lpointint_cxx", "-lpointint_hip", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"C++ example build failed: {' '.join(cmd)}\n{r.stderr}")
-    os.replace(tmp, OUT_EXAMPLE)
+
+    def run(out, cmd, what):
+        tmp = out + ".tmp"
+        r = subprocess.run([*cmd, "-o", tmp], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{what} build failed: {' '.join(cmd)}\n{r.stderr}")
+        os.replace(tmp, out)
+
+    run(OUT_CXX, [cxx, *common, "-shared", os.path.join(CXX_DIR, "pointint_cxx.cpp"),
+                  os.path.join(CXX_DIR, "align.cpp"), "-L", HERE, "-lpointint_hip",
+                  "-Wl,-rpath,$ORIGIN"], "C++ API")
+    for out, src in ((OUT_EXAMPLE, "example_cube_pair.cpp"), (OUT_ALIGN, "align_main.cpp")):
+        run(out, [cxx, *common, os.path.join(CXX_DIR, src), "-L", HERE, "-lpointint_cxx",
+                  "-lpointint_hip", "-Wl,-rpath,$ORIGIN"], src)
 
 
 if __name__ == "__main__":

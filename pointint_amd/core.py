@@ -68,7 +68,7 @@ class Context:
         """Last call's phase durations (ms) from HIP events on the context stream."""
         ms = np.zeros(6)
         N.check(N.load().pii_context_timings(self._h, N.ptr(ms), 6))
-        return {"prepass": ms[0], "form_factor": ms[1], "gram": ms[2], "adjoint": ms[3],
+        return {"prepass": ms[0], "form_factor": ms[1], "gram": ms[2], "derivatives": ms[3],
                 "ff_series": ms[4], "ff_per_face": ms[5]}
 
     def work(self) -> dict:
@@ -362,6 +362,24 @@ def volume_matrix_hessian(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None,
                                                _GRAM[gram], H.ctypes.data,
                                                V.ctypes.data if V is not None else None))
     return (H, V) if return_volume else H
+
+
+def volume_matrix_derivatives(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None,
+                              gram: str = "fp16x3", hessian: bool = True):
+    """V (n x n), grad (n, n, 12) and, with hessian, the Gauss-Newton H
+    (12n x 12n) of AffineDoF objects from one pass.  grad[i, j] =
+    dV_ij/dtheta_i = (1/8 pi^3) sum_q w_q Re(dA_i/dtheta conj A_j), half the
+    full derivative for i == j (the matrix driver's grad_matrix convention)."""
+    n = len(geoms)
+    ctx = geoms[0].ctx
+    gh, _kh, dh, _nh, _keep = _batch_args(geoms, dofs, None if dofs is None else ["affine"] * n)
+    V = np.empty((n, n))
+    G = np.empty((n, n, 12))
+    H = np.empty((12 * n, 12 * n)) if hessian else None
+    N.check(N.load().pii_volume_matrix_affine_derivatives(
+        ctx.handle, n, gh.ctypes.data, dh.ctypes.data, kgrid.handle, _GRAM[gram], V.ctypes.data,
+        G.ctypes.data, H.ctypes.data if H is not None else None))
+    return (V, G, H) if hessian else (V, G)
 
 
 def gram_from_J(J: np.ndarray, kgrid: KGrid, ctx: Optional[Context] = None) -> np.ndarray:

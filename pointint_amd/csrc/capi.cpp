@@ -10,6 +10,7 @@
 #include <new>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "device_common.hpp"
@@ -607,29 +608,58 @@ int pii_geometry_create_tris(pii_context* ctx, const double* tri, const int32_t*
     GeometryDev& d = g->g;
     d.type = PII_GEOM_TRIANGLE;
     d.n = nt;
-    d.nv = 3 * nt;
-    d.h_verts.resize(9 * (size_t)nt);
     d.h_faces.resize(3 * (size_t)nt);
-    d.h_dofvid.resize(3 * (size_t)nt);
+    d.h_verts.clear();
+    d.h_dofvid.clear();
+    d.h_verts.reserve(9 * (size_t)nt);
+    d.h_dofvid.reserve(3 * (size_t)nt);
+    // Corners with the same vertex id become one vertex when their positions
+    // agree to rounding (a + e1 and a + e2 reproduce the mesh vertex to about
+    // one ulp): shared vertices let the form-factor kernels reuse each vertex
+    // phasor across its faces.  Corners without an id, or whose position
+    // disagrees, keep a vertex of their own.
+    std::unordered_map<int, int> first;  // vertex id -> vertex index
+    first.reserve(vid ? 2 * (size_t)nt : 0);
     int maxv = -1;
+    if (vid)
+      for (size_t i = 0; i < 3 * (size_t)nt; ++i) maxv = std::max(maxv, vid[i]);
+    auto add_vertex = [&](const double* p, int dv) {
+      d.h_verts.insert(d.h_verts.end(), p, p + 3);
+      d.h_dofvid.push_back(dv);
+      return (int)d.h_dofvid.size() - 1;
+    };
     for (int i = 0; i < nt; ++i) {
       const double* r = tri + 12 * (size_t)i;
-      for (int c = 0; c < 3; ++c) {
-        const double off0 = (c == 1) ? r[3] : (c == 2) ? r[6] : 0.0;
-        const double off1 = (c == 1) ? r[4] : (c == 2) ? r[7] : 0.0;
-        const double off2 = (c == 1) ? r[5] : (c == 2) ? r[8] : 0.0;
-        d.h_verts[9 * (size_t)i + 3 * c + 0] = r[0] + off0;
-        d.h_verts[9 * (size_t)i + 3 * c + 1] = r[1] + off1;
-        d.h_verts[9 * (size_t)i + 3 * c + 2] = r[2] + off2;
-        d.h_faces[3 * (size_t)i + c] = 3 * i + c;
-        const int v = vid ? vid[3 * (size_t)i + c] : -1;
-        d.h_dofvid[3 * (size_t)i + c] = v;
-        maxv = std::max(maxv, v);
-      }
       // A face with any negative vid contributes nothing under vertex DoFs.
-      if (vid && (vid[3 * (size_t)i] < 0 || vid[3 * (size_t)i + 1] < 0 || vid[3 * (size_t)i + 2] < 0))
-        for (int c = 0; c < 3; ++c) d.h_dofvid[3 * (size_t)i + c] = -1;
+      const bool has_ids = vid && vid[3 * (size_t)i] >= 0 && vid[3 * (size_t)i + 1] >= 0 &&
+                           vid[3 * (size_t)i + 2] >= 0;
+      for (int c = 0; c < 3; ++c) {
+        double p[3];
+        for (int k = 0; k < 3; ++k) p[k] = r[k] + (c == 0 ? 0.0 : r[3 * c + k]);
+        const int dv = has_ids ? vid[3 * (size_t)i + c] : -1;
+        int v = -1;
+        if (dv >= 0) {
+          const auto it = first.find(dv);
+          const int f = it == first.end() ? -1 : it->second;
+          if (f >= 0) {
+            const double* q = &d.h_verts[3 * (size_t)f];
+            const double tol = 1e-14 * (std::fabs(q[0]) + std::fabs(q[1]) + std::fabs(q[2]) +
+                                        std::fabs(r[0]) + std::fabs(r[1]) + std::fabs(r[2]));
+            if (std::fabs(q[0] - p[0]) <= tol && std::fabs(q[1] - p[1]) <= tol &&
+                std::fabs(q[2] - p[2]) <= tol)
+              v = f;
+          }
+          if (v < 0) {
+            v = add_vertex(p, dv);
+            if (f < 0) first[dv] = v;
+          }
+        } else {
+          v = add_vertex(p, -1);
+        }
+        d.h_faces[3 * (size_t)i + c] = v;
+      }
     }
+    d.nv = (int)d.h_dofvid.size();
     d.ndof_verts = maxv + 1;
     finish_triangles(ctx, d);
     *out = g.release();
@@ -791,6 +821,26 @@ int pii_volume_matrix_hessian(pii_context* ctx, int n, const pii_geometry* const
     Batch b;
     b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
     volume_matrix_hessian(ctx->c, b, kg->k, gram_mode, H, V);
+  });
+}
+
+int pii_volume_matrix_affine_derivatives(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                                         const double* const* dofs, const pii_kgrid* kg,
+                                         int gram_mode, double* V, double* grad, double* H) {
+  return guarded([&] {
+    require(ctx && kg, "pii_volume_matrix_affine_derivatives: NULL argument");
+    require(V || grad || H, "pii_volume_matrix_affine_derivatives: no output requested");
+    require(n > 0 && geoms && dofs, "pii_volume_matrix_affine_derivatives: bad arrays");
+    for (int i = 0; i < n; ++i)
+      require(geoms[i] && dofs[i], "pii_volume_matrix_affine_derivatives: NULL object");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    std::vector<const GeometryDev*> gs(n);
+    for (int i = 0; i < n; ++i) gs[i] = &geoms[i]->g;
+    const std::vector<int> kinds(n, PII_DOF_AFFINE), nd(n, 12);
+    Batch b;
+    b.build(ctx->c, n, gs.data(), kinds.data(), dofs, nd.data(), true, ctx->c.series_grid(kg->k));
+    volume_matrix_affine_all(ctx->c, b, kg->k, gram_mode, V, grad, H);
   });
 }
 

@@ -93,15 +93,35 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
   PII_HIP(hipStreamSynchronize(s));
 }
 
-void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
-                           double* H_host, double* V_host) {
-  const int N = b.N, Qh = kg.Qh, D = 12 * N;
+// Gram of the columns [sqrt(w) A_0..A_{N-1} | sqrt(w) dA_i/dtheta_t] of
+// AffineDoF objects: the first N columns (when with_A) come from the fast
+// form-factor path, the 12 N derivative columns (12 col + t) from the
+// forward-mode kernel.  Leaves the (D x D) result, D = (with_A ? 13 : 12) N,
+// in c.w.H.  Entry (a, b) is (1/8 pi^3) sum_q w_q Re(x_a conj x_b) over the
+// full grid, so the A-A block is V, the dA-A block holds dV_ij/dtheta_i
+// (i != j; half of it for i == j) and the dA-dA block the Gauss-Newton H.
+static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                       bool with_A) {
+  const int N = b.N, Qh = kg.Qh;
+  const int na = with_A ? N : 0, D = na + 12 * N;
   for (const ObjDesc& d : b.h_objs)
     if (d.dof_kind != PII_DOF_AFFINE)
-      throw Error(PII_ERR_UNSUPPORTED, "Gauss-Newton Hessian supports AffineDoF objects only");
-  if (V_host) volume_matrix_fast(c, b, kg, gram_mode, V_host);
+      throw Error(PII_ERR_UNSUPPORTED, "affine derivatives support AffineDoF objects only");
+  if (gram_mode != PII_GRAM_FP64 && gram_mode != PII_GRAM_BF16X3 && gram_mode != PII_GRAM_FP16X3)
+    throw Error(PII_ERR_INVALID, "unknown Gram mode");
   hipStream_t s = c.stream;
   const int ldX = (int)align_up((size_t)Qh, 64);
+  c.w.Xt.ensure(sizeof(float2) * (size_t)ldX * D);
+  c.w.cmax2.ensure(sizeof(unsigned) * (size_t)D);
+  float2* X = c.w.Xt.as<float2>();
+  unsigned* colmax = c.w.cmax2.as<unsigned>();
+  PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * D, s));
+  if (with_A) {
+    c.phase_begin(1);
+    launch_form_factors_fast(s, b, kg, 1, nullptr, 0, X, ldX, colmax, &c);
+    c.phase_end(1);
+    PII_HIP(hipGetLastError());
+  }
   // per-face records of triangle objects
   std::vector<long long> off(N, 0);
   size_t tot = 0;
@@ -116,36 +136,58 @@ void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int g
   PII_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(long long) * N, hipMemcpyHostToDevice, s));
   launch_dtheta_records(s, b.d_objs, b.d_eprefix, N, b.total_e, d_drec, d_off);
   PII_HIP(hipGetLastError());
-  c.w.Xt.ensure(sizeof(float2) * (size_t)ldX * D);
-  c.w.cmax2.ensure(sizeof(unsigned) * (size_t)D);
-  unsigned* colmax = c.w.cmax2.as<unsigned>();
-  PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * D, s));
-  c.phase_begin(1);
+  c.phase_begin(3);
   launch_dtheta_affine(s, b.d_objs, N, kg.rep.as<double>(), Qh, d_drec, d_off,
-                       c.w.Xt.as<float2>(), ldX, colmax);
-  c.phase_end(1);
+                       X + (size_t)na * ldX, ldX, colmax + na);
+  c.phase_end(3);
   PII_HIP(hipGetLastError());
   c.w.H.ensure(sizeof(double) * (size_t)D * D);
   c.phase_begin(2);
   if (gram_mode == PII_GRAM_FP64) {
     const int nsplit = gram_fp64_splits(Qh, D);
     c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
-    launch_gram_fp64_x(s, c.w.Xt.as<float2>(), Qh, D, ldX, c.w.part.as<double>(), nsplit,
-                       c.w.H.as<double>());
-  } else if (gram_mode == PII_GRAM_BF16X3 || gram_mode == PII_GRAM_FP16X3) {
+    launch_gram_fp64_x(s, X, Qh, D, ldX, c.w.part.as<double>(), nsplit, c.w.H.as<double>());
+  } else {
     const int K = 2 * ldX;
     const int nsplit = gram_split_splits(K, D);
     c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
     launch_gram_split(s, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.Xt.as<float>(), D, K, colmax,
                       c.w.part.as<double>(), nsplit, c.w.H.as<double>());
-  } else {
-    throw Error(PII_ERR_INVALID, "unknown Gram mode");
   }
   c.phase_end(2);
   PII_HIP(hipGetLastError());
-  PII_HIP(hipMemcpyAsync(H_host, c.w.H.p, sizeof(double) * (size_t)D * D, hipMemcpyDeviceToHost,
-                         s));
-  PII_HIP(hipStreamSynchronize(s));
+  return D;
+}
+
+void volume_matrix_affine_all(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                              double* V_host, double* grad_host, double* H_host) {
+  const int N = b.N;
+  const bool with_A = V_host || grad_host;
+  const int na = with_A ? N : 0;
+  const int D = affine_gram(c, b, kg, gram_mode, with_A);
+  std::vector<double> G((size_t)D * D);
+  PII_HIP(hipMemcpyAsync(G.data(), c.w.H.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost,
+                         c.stream));
+  PII_HIP(hipStreamSynchronize(c.stream));
+  auto at = [&](int r, int col) { return G[(size_t)r * D + col]; };
+  if (V_host)
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) V_host[(size_t)i * N + j] = at(i, j);
+  if (grad_host)
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j)
+        for (int t = 0; t < 12; ++t)
+          grad_host[((size_t)i * N + j) * 12 + t] = at(na + 12 * i + t, j);
+  if (H_host) {
+    const int DH = 12 * N;
+    for (int r = 0; r < DH; ++r)
+      for (int col = 0; col < DH; ++col) H_host[(size_t)r * DH + col] = at(na + r, na + col);
+  }
+}
+
+void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                           double* H_host, double* V_host) {
+  volume_matrix_affine_all(c, b, kg, gram_mode, V_host, nullptr, H_host);
 }
 
 void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geoms,

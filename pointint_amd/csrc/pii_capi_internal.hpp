@@ -93,6 +93,12 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
 void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
                            double* H_host, double* V_host);
 
+// V (N x N), grad[(i N + j) 12 + t] = dV_ij/dtheta_i,t (half for i == j) and
+// the Gauss-Newton H of affine-DoF objects from one Gram over the A and
+// dA/dtheta columns; every output may be null.
+void volume_matrix_affine_all(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                              double* V_host, double* grad_host, double* H_host);
+
 // Vertex-DoF / affine-DoF gradients of L = sum_ij G_ij V_ij (triangles).
 void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geoms,
                        const KGridDev& kg, const double* G, double* const* grads,

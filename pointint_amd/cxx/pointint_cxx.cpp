@@ -13,6 +13,7 @@
 #include <mutex>
 #include <stdexcept>
 
+#include "cxx_internal.hpp"
 #include "pointint_hip.h"
 
 namespace PoIntInt {
@@ -23,6 +24,13 @@ constexpr double kPi = 3.14159265358979323846;
 
 int g_ff = PII_FF_FAST;
 int g_gram = PII_GRAM_FP16X3;
+
+}  // namespace
+
+namespace detail {
+
+int ff_precision() { return g_ff; }
+int gram_mode() { return g_gram; }
 
 // One context per process on device 0, created on first use.
 pii_context* context() {
@@ -102,6 +110,15 @@ pii_kgrid* upload(pii_context* ctx, const KGrid& kg) {
     return nullptr;
   return out;
 }
+
+}  // namespace detail
+
+namespace {
+
+using detail::context;
+using detail::report;
+using detail::upload;
+using detail::CallObjects;
 
 double3 d3(double x, double y, double z) {
   double3 v;
@@ -274,6 +291,22 @@ KGrid build_kgrid(const std::vector<std::array<double, 3>>& leb_dirs,
 }
 
 // ---- drivers ----
+// hessian_ij[i][j] = block (i, j) of the (12n x 12n) Gauss-Newton matrix;
+// hessian_ii / hessian_jj are zero in that form
+// (compute_intersection_volume_multi_object.cu:186-227).
+static void fill_hessian_blocks(IntersectionVolumeMatrixResult& res, int n,
+                                const std::vector<double>& H) {
+  const int D = 12 * n;
+  res.hessian_ii.assign(n, std::vector<MatrixXd>(n, MatrixXd::Zero(12, 12)));
+  res.hessian_jj.assign(n, std::vector<MatrixXd>(n, MatrixXd::Zero(12, 12)));
+  res.hessian_ij.assign(n, std::vector<MatrixXd>(n, MatrixXd(12, 12)));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int a = 0; a < 12; ++a)
+        for (int c = 0; c < 12; ++c)
+          res.hessian_ij[i][j](a, c) = H[(size_t)(12 * i + a) * D + 12 * j + c];
+}
+
 IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
     const std::vector<Geometry>& geoms, const std::vector<std::shared_ptr<DoFParameterization>>& dofs,
     const std::vector<VectorXd>& dof_vectors, const KGrid& kgrid, ComputationFlags flags,
@@ -293,33 +326,93 @@ IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
   CallObjects co;
   co.kg = upload(ctx, kgrid);
   if (!co.kg) return res;
-  std::vector<int> kinds(n), nd(n);
-  std::vector<const double*> dp(n);
+  std::vector<int> kinds(n);
   for (int i = 0; i < n; ++i) {
     pii_geometry* g = upload(ctx, geoms[i]);
     if (!g) return res;
     co.geoms.push_back(g);
     kinds[i] = dofs[i] ? dofs[i]->dof_kind() : PII_DOF_AFFINE;
+  }
+  std::vector<const pii_geometry*> gc(co.geoms.begin(), co.geoms.end());
+  return detail::volume_matrix_resident(ctx, gc, kinds, dof_vectors, co.kg,
+                                        (int)kgrid.kmag.size(), flags, enable_profiling);
+}
+
+namespace detail {
+
+IntersectionVolumeMatrixResult volume_matrix_resident(
+    pii_context* ctx, const std::vector<const pii_geometry*>& gc, const std::vector<int>& kinds,
+    const std::vector<VectorXd>& dof_vectors, const pii_kgrid* kg, int Q, ComputationFlags flags,
+    bool enable_profiling) {
+  IntersectionVolumeMatrixResult res;
+  const int n = (int)gc.size();
+  res.volume_matrix = MatrixXd::Zero(n, n);
+  if (n == 0) return res;
+  std::vector<int> nd(n);
+  std::vector<const double*> dp(n);
+  for (int i = 0; i < n; ++i) {
     nd[i] = (int)dof_vectors[i].size();
     dp[i] = dof_vectors[i].data();
   }
+  struct Keep {  // kg / co.geoms of the driver body below
+    const pii_kgrid* kg;
+  } co{kg};
   if (enable_profiling) pii_context_set_profiling(ctx, 1);
-  std::vector<const pii_geometry*> gc(co.geoms.begin(), co.geoms.end());
-  if (!report(pii_volume_matrix(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(), co.kg,
+  bool all_affine = true;
+  for (int i = 0; i < n; ++i) all_affine = all_affine && kinds[i] == PII_DOF_AFFINE;
+  const bool one_pass = all_affine && (needs_gradient(flags) || needs_hessian(flags));
+  auto print_profile = [&] {
+    if (!enable_profiling) return;
+    double ms[4];
+    pii_context_timings(ctx, ms, 4);
+    std::cout << "\n=== MI355X Multi-Object Volume Computation Profiler ===\n"
+              << "Number of objects: " << n << "\nK-grid nodes: " << Q
+              << "\n  Pre-pass:    " << ms[0] << " ms\n  Form factor: " << ms[1]
+              << " ms\n  Gram:        " << ms[2] << " ms";
+    if (one_pass) std::cout << "\n  dA/dtheta:   " << ms[3] << " ms";
+    std::cout << std::endl;
+    pii_context_set_profiling(ctx, 0);
+  };
+  if (one_pass) {
+    // AffineDoF objects: as the reference's own GRADIENT / HESSIAN branch,
+    // dA/dtheta columns next to A, and V, grad_matrix and the Gauss-Newton
+    // blocks from one Gram (compute_intersection_volume_multi_object.cu:437-600).
+    const int D = 12 * n;
+    std::vector<double> G((size_t)n * n * 12), H;
+    if (needs_hessian(flags)) H.resize((size_t)D * D);
+    if (!report(pii_volume_matrix_affine_derivatives(ctx, n, gc.data(), dp.data(), kg, g_gram,
+                                                     res.volume_matrix.data.data(), G.data(),
+                                                     H.empty() ? nullptr : H.data()),
+                "compute_intersection_volume_matrix_cuda")) {
+      res.volume_matrix = MatrixXd::Zero(n, n);
+      return res;
+    }
+    print_profile();
+    // the FP64 form-factor setting keeps V on the FP64 path
+    if (g_ff != PII_FF_FAST &&
+        !report(pii_volume_matrix(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(), kg,
+                                  g_ff, g_gram, res.volume_matrix.data.data()),
+                "compute_intersection_volume_matrix_cuda")) {
+      res.volume_matrix = MatrixXd::Zero(n, n);
+      return res;
+    }
+    if (needs_gradient(flags)) {
+      res.grad_matrix.assign(n, std::vector<VectorXd>(n));
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          res.grad_matrix[i][j].assign(G.begin() + ((size_t)i * n + j) * 12,
+                                       G.begin() + ((size_t)i * n + j + 1) * 12);
+    }
+    if (needs_hessian(flags)) fill_hessian_blocks(res, n, H);
+    return res;
+  }
+  if (!report(pii_volume_matrix(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(), kg,
                                 g_ff, g_gram, res.volume_matrix.data.data()),
               "compute_intersection_volume_matrix_cuda")) {
     res.volume_matrix = MatrixXd::Zero(n, n);
     return res;
   }
-  if (enable_profiling) {
-    double ms[4];
-    pii_context_timings(ctx, ms, 4);
-    std::cout << "\n=== MI355X Multi-Object Volume Computation Profiler ===\n"
-              << "Number of objects: " << n << "\nK-grid nodes: " << kgrid.kmag.size()
-              << "\n  Pre-pass:    " << ms[0] << " ms\n  Form factor: " << ms[1]
-              << " ms\n  Gram:        " << ms[2] << " ms" << std::endl;
-    pii_context_set_profiling(ctx, 0);
-  }
+  print_profile();
   if (needs_gradient(flags)) {
     // grad_matrix[i][j] = dV_ij/dtheta_i from one VJP per unordered pair.
     res.grad_matrix.assign(n, std::vector<VectorXd>(n));
@@ -335,7 +428,7 @@ IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
         std::fill(G.begin(), G.end(), 0.0);
         G[(size_t)i * n + j] = 1.0;
         if (!report(pii_volume_matrix_vjp(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(),
-                                          co.kg, G.data(), gp.data(), nullptr),
+                                          kg, G.data(), gp.data(), nullptr),
                     "gradient")) {
           res.grad_matrix.clear();
           return res;
@@ -357,20 +450,15 @@ IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
     const int D = 12 * n;
     std::vector<double> H((size_t)D * D);
     if (!report(pii_volume_matrix_hessian(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(),
-                                          co.kg, g_gram, H.data(), nullptr),
+                                          kg, g_gram, H.data(), nullptr),
                 "hessian"))
       return res;
-    res.hessian_ii.assign(n, std::vector<MatrixXd>(n, MatrixXd::Zero(12, 12)));
-    res.hessian_jj.assign(n, std::vector<MatrixXd>(n, MatrixXd::Zero(12, 12)));
-    res.hessian_ij.assign(n, std::vector<MatrixXd>(n, MatrixXd(12, 12)));
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j)
-        for (int a = 0; a < 12; ++a)
-          for (int c = 0; c < 12; ++c)
-            res.hessian_ij[i][j](a, c) = H[(size_t)(12 * i + a) * D + 12 * j + c];
+    fill_hessian_blocks(res, n, H);
   }
   return res;
 }
+
+}  // namespace detail
 
 IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
     const std::vector<Geometry>& geometries, const KGrid& kgrid, int blockSize,
@@ -448,6 +536,18 @@ double compute_volume_cpu(const Geometry& g, bool) {
       v += (d.c.x * d.n.x + d.c.y * d.n.y + d.c.z * d.n.z) * d.w;
   }
   return v / 3.0;
+}
+
+// cpp/src/dof/affine_dof.cpp:324-384
+double AffineDoF::compute_volume(const Geometry& g, const VectorXd& dofs) const {
+  if (dofs.size() != 12) {
+    std::cerr << "AffineDoF::compute_volume: expected 12 DoFs, got " << dofs.size() << std::endl;
+    return 0.0;
+  }
+  const double* A = dofs.data() + 3;
+  const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                     A[2] * (A[3] * A[7] - A[4] * A[6]);
+  return compute_volume_cpu(g) * std::fabs(det);
 }
 
 // cpp/src/analytical_intersection.cpp:15-60
