@@ -140,10 +140,41 @@ __device__ __forceinline__ void j1ox_and_prime(double x, double& f, double& fp) 
   fp = (J0 - 2.0 * J1 * inv) * inv;
 }
 
+// The 12 columns X_t = sqrt(w) dA/dtheta_t of node q from the sums v = {A,
+// dA/dA_rc (row-major)} (re, im interleaved): dA/dt_t = i k_t A.  Called by
+// whole waves (the column maxima are wave reductions).
+__device__ __forceinline__ void write_columns(const double (&v)[20], int q, int Qh, double kx,
+                                              double ky, double kz, const double* krep, int col,
+                                              float2* __restrict__ Xt, int ldX,
+                                              unsigned* __restrict__ colmax, int lane) {
+  const int qc = q < Qh ? q : Qh - 1;
+  const double sw = (q < Qh) ? sqrt(krep[7 * (size_t)Qh + qc]) : 0.0;
+  const double kv[3] = {kx, ky, kz};
+  float2* X = Xt + (size_t)(12 * col) * ldX;
+  for (int t = 0; t < 12; ++t) {
+    double re, im;
+    if (t < 3) {  // i k_t A
+      re = -kv[t] * v[1];
+      im = kv[t] * v[0];
+    } else {
+      re = v[2 + 2 * (t - 3)];
+      im = v[3 + 2 * (t - 3)];
+    }
+    const float2 x = make_float2((float)(re * sw), (float)(im * sw));
+    if (q < ldX) X[(size_t)t * ldX + q] = (q < Qh) ? x : make_float2(0.f, 0.f);
+    float mval = (q < Qh) ? fmaxf(fabsf(x.x), fabsf(x.y)) : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) mval = fmaxf(mval, __shfl_xor(mval, off));
+    if (lane == 0) atomicMax(colmax + 12 * col + t, __float_as_uint(mval));
+  }
+}
+
+// One workgroup per (object, 64 nodes[, element split]).  Split launches
+// store the 20 sums per node to part[((z * nobj + obj) * nblk * 64 + node) *
+// 20 + i] for k_dtheta_finish.
 __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh,
     const float* __restrict__ drec, const long long* __restrict__ doff, float2* __restrict__ Xt,
-    int ldX, unsigned* __restrict__ colmax) {
+    int ldX, unsigned* __restrict__ colmax, double* __restrict__ part) {
   __shared__ float2 tab[kPatchVerts][kLanes];
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
@@ -167,7 +198,9 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
     const auto* world = as_const(d.world);
     const auto* rec = as_const(d.rec);
     const auto* dr = as_const(drec) + doff[bo.obj];
-    for (int p = 0; p < d.npatch; ++p) {
+    const int pchunk = (d.npatch + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int pb = (int)blockIdx.z * pchunk, pe = min(d.npatch, pb + pchunk);
+    for (int p = pb; p < pe; ++p) {
       const int vbeg = hdr[4 * p], vcnt = hdr[4 * p + 1];
       const int fbeg = hdr[4 * p + 2], fcnt = hdr[4 * p + 3];
       for (int s = wave; s < vcnt; s += kWaves) {
@@ -242,7 +275,9 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
     const double inv_km = km > 0.0 ? 1.0 / km : 0.0;
     float pA[2] = {0.f, 0.f}, pg[9][2] = {};
     int cnt = 0;
-    for (int e = wave; e < d.n; e += kWaves) {
+    const int echunk = (d.n + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int eb = (int)blockIdx.z * echunk, ee = min(d.n, eb + echunk);
+    for (int e = eb + wave; e < ee; e += kWaves) {
       const auto* r = rec + (size_t)kRecSurfel * e;   // c' (world), A n, rho|sigma, area|w
       const auto* s = el + (size_t)kRecSurfel * e;    // c, n (reference), rho|sigma, area|w
       const double kdn = kp[0] * s[3] + kp[1] * s[4] + kp[2] * s[5];
@@ -341,24 +376,37 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
   if (wave != 0) return;
 #pragma unroll
   for (int i = 0; i < 20; ++i) v[i] += red[i * kLanes + lane];
-  const double sw = (q < Qh) ? sqrt(krep[7 * (size_t)Qh + qc]) : 0.0;
-  const double kv[3] = {kx, ky, kz};
-  float2* X = Xt + (size_t)(12 * d.col) * ldX;
-  for (int t = 0; t < 12; ++t) {
-    double re, im;
-    if (t < 3) {  // i k_t A
-      re = -kv[t] * v[1];
-      im = kv[t] * v[0];
-    } else {
-      re = v[2 + 2 * (t - 3)];
-      im = v[3 + 2 * (t - 3)];
-    }
-    const float2 x = make_float2((float)(re * sw), (float)(im * sw));
-    if (q < ldX) X[(size_t)t * ldX + q] = (q < Qh) ? x : make_float2(0.f, 0.f);
-    float mval = (q < Qh) ? fmaxf(fabsf(x.x), fabsf(x.y)) : 0.0f;
-    for (int off = 32; off > 0; off >>= 1) mval = fmaxf(mval, __shfl_xor(mval, off));
-    if (lane == 0) atomicMax(colmax + 12 * d.col + t, __float_as_uint(mval));
+  if (part) {
+    double* o = part + (((size_t)blockIdx.z * gridDim.y + bo.obj) * gridDim.x * kLanes +
+                        (size_t)bo.blk * kLanes + lane) * 20;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) o[i] = v[i];
+    return;
   }
+  write_columns(v, q, Qh, kx, ky, kz, krep, d.col, Xt, ldX, colmax, lane);
+}
+
+// Finishes a split launch: sums the nsplit partials of each node in split
+// order and writes its columns.  One wave per (object, 64 nodes).
+__global__ void __launch_bounds__(kLanes) k_dtheta_finish(
+    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh,
+    float2* __restrict__ Xt, int ldX, unsigned* __restrict__ colmax,
+    const double* __restrict__ part, int nsplit) {
+  const int obj = blockIdx.y, blk = blockIdx.x, lane = threadIdx.x;
+  const ObjDesc& d = objs[obj];
+  const int q = blk * kLanes + lane;
+  const int qc = q < Qh ? q : Qh - 1;
+  const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
+  double v[20];
+#pragma unroll
+  for (int i = 0; i < 20; ++i) v[i] = 0.0;
+  const size_t stride = (size_t)gridDim.y * gridDim.x * kLanes * 20;
+  const double* src = part + ((size_t)obj * gridDim.x * kLanes + (size_t)blk * kLanes + lane) * 20;
+  for (int z = 0; z < nsplit; ++z) {
+#pragma unroll
+    for (int i = 0; i < 20; ++i) v[i] += src[(size_t)z * stride + i];
+  }
+  write_columns(v, q, Qh, kx, ky, kz, krep, d.col, Xt, ldX, colmax, lane);
 }
 
 }  // namespace
@@ -370,12 +418,22 @@ void launch_dtheta_records(hipStream_t s, const ObjDesc* d_objs, const int* d_ep
                      d_eprefix, N, total_e, d_out, d_off);
 }
 
+size_t dtheta_split_doubles(int N, int ldX, int nsplit) {
+  return nsplit > 1 ? (size_t)nsplit * N * ((ldX + kLanes - 1) / kLanes) * kLanes * 20 : 0;
+}
+
 void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
                           int Qh, const float* d_drec, const long long* d_doff, float2* d_Xt,
-                          int ldX, unsigned* d_colmax) {
+                          int ldX, unsigned* d_colmax, int nsplit, double* d_part) {
   if (N <= 0 || Qh <= 0) return;
-  hipLaunchKernelGGL(k_dtheta_affine, dim3((ldX + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves),
-                     0, s, d_objs, d_krep, Qh, d_drec, d_doff, d_Xt, ldX, d_colmax);
+  const int nblk = (ldX + kLanes - 1) / kLanes;
+  const bool split = nsplit > 1 && d_part;
+  double* part = split ? d_part : nullptr;
+  hipLaunchKernelGGL(k_dtheta_affine, dim3(nblk, N, split ? nsplit : 1), dim3(kLanes * kWaves), 0,
+                     s, d_objs, d_krep, Qh, d_drec, d_doff, d_Xt, ldX, d_colmax, part);
+  if (split)
+    hipLaunchKernelGGL(k_dtheta_finish, dim3(nblk, N), dim3(kLanes), 0, s, d_objs, d_krep, Qh,
+                       d_Xt, ldX, d_colmax, part, nsplit);
 }
 
 }  // namespace pii

@@ -6,14 +6,30 @@
 
 namespace pii {
 
+// Element split for launches with few workgroups: enough splits to reach
+// about `target` workgroups, each split keeping at least min_per elements.
+static int split_count(int base_wgs, int max_elems, int min_per, int target) {
+  constexpr int kFewWGs = 256;
+  if (base_wgs <= 0 || base_wgs >= kFewWGs || max_elems <= 0) return 1;
+  const int want = (target + base_wgs - 1) / base_wgs;
+  return std::max(1, std::min(want, max_elems / min_per));
+}
+
 void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
                               int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
-                              unsigned* d_colmax, Context* prof) {
+                              unsigned* d_colmax, Workspace& w, Context* prof) {
   const double* krep = kg.rep.as<double>();
+  const int N = b.N;
   bool tri_series = false, surfel_series = false;
+  int ser_tri_faces = 0, ser_surf_elems = 0, max_patches = 0, max_surf = 0;
   for (const ObjDesc& d : b.h_objs) {
     const bool tri = d.type == PII_GEOM_TRIANGLE;
-    if (d.nser > 0) (tri ? tri_series : surfel_series) = true;
+    if (d.nser > 0) {
+      (tri ? tri_series : surfel_series) = true;
+      (tri ? ser_tri_faces : ser_surf_elems) = std::max(tri ? ser_tri_faces : ser_surf_elems, d.n);
+    }
+    if (tri) max_patches = std::max(max_patches, d.npatch);
+    else max_surf = std::max(max_surf, d.n);
     if (!prof) continue;
     prof->work[tri ? 0 : 3] += (double)d.n * (kg.Qh - d.qbeg);
     if (d.nser > 0) {
@@ -21,20 +37,32 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
       prof->work[2] += (double)d.qbeg;
     }
   }
+  // splits and their partial-sum buffer (one buffer, used in launch order)
+  const int dirblk = (b.Mh + 63) / 64;
+  const int s_tri = tri_series ? split_count(dirblk * N, ser_tri_faces, 256, 512) : 1;
+  const int s_surf = surfel_series ? split_count(dirblk * N, ser_surf_elems, 256, 512) : 1;
+  const int span = (out_mode == 0 ? kg.Qh : ldX) - b.qbeg_min;
+  const int s_ff = std::max(split_count(((span + 63) / 64) * N, max_patches, 2, 512),
+                            split_count(((span + 63) / 64) * N, max_surf, 256, 512));
+  const size_t need = std::max({series_split_doubles(N, b.Mh, s_tri),
+                                series_split_doubles(N, b.Mh, s_surf),
+                                ff_split_doubles(N, std::max(span, 0), s_ff)});
+  if (need) w.split.ensure(sizeof(double) * need);
+  double* part = need ? w.split.as<double>() : nullptr;
   if (tri_series || surfel_series) {
     if (prof) prof->phase_begin(4);
     if (tri_series)
-      launch_series_tri(s, b.d_objs, b.N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
-                        d_colmax);
+      launch_series_tri(s, b.d_objs, N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
+                        d_colmax, s_tri, part);
     if (surfel_series)
-      launch_series_surfel(s, b.d_objs, b.N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
-                           d_colmax);
+      launch_series_surfel(s, b.d_objs, N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
+                           d_colmax, s_surf, part);
     if (prof) prof->phase_end(4);
     PII_HIP(hipGetLastError());
   }
   if (prof) prof->phase_begin(5);
-  launch_ff_fast(s, b.d_objs, b.N, krep, kg.Qh, b.qbeg_min, out_mode, d_A, ldA, d_X, ldX,
-                 d_colmax);
+  launch_ff_fast(s, b.d_objs, N, krep, kg.Qh, b.qbeg_min, out_mode, d_A, ldA, d_X, ldX,
+                 d_colmax, s_ff, part);
   if (prof) prof->phase_end(5);
   PII_HIP(hipGetLastError());
 }
@@ -46,7 +74,7 @@ void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, doubl
   c.w.X.ensure(sizeof(double2) * (size_t)ldA * N);
   c.w.J.ensure(sizeof(double2) * (size_t)Q * N);
   c.phase_begin(1);
-  launch_form_factors_fast(s, b, kg, 0, c.w.X.as<double2>(), ldA, nullptr, 0, nullptr, &c);
+  launch_form_factors_fast(s, b, kg, 0, c.w.X.as<double2>(), ldA, nullptr, 0, nullptr, c.w, &c);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   launch_expand(s, c.w.X.as<double2>(), ldA, kg.rep_index.as<int>(), kg.anti_index.as<int>(), Qh,
@@ -67,7 +95,7 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
   unsigned* colmax = c.w.colmax.as<unsigned>();
   PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s));
   c.phase_begin(1);
-  launch_form_factors_fast(s, b, kg, 1, nullptr, 0, c.w.X.as<float2>(), ldX, colmax, &c);
+  launch_form_factors_fast(s, b, kg, 1, nullptr, 0, c.w.X.as<float2>(), ldX, colmax, c.w, &c);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   c.w.V.ensure(sizeof(double) * (size_t)N * N);
@@ -118,7 +146,7 @@ static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_
   PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * D, s));
   if (with_A) {
     c.phase_begin(1);
-    launch_form_factors_fast(s, b, kg, 1, nullptr, 0, X, ldX, colmax, &c);
+    launch_form_factors_fast(s, b, kg, 1, nullptr, 0, X, ldX, colmax, c.w, &c);
     c.phase_end(1);
     PII_HIP(hipGetLastError());
   }
@@ -136,9 +164,22 @@ static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_
   PII_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(long long) * N, hipMemcpyHostToDevice, s));
   launch_dtheta_records(s, b.d_objs, b.d_eprefix, N, b.total_e, d_drec, d_off);
   PII_HIP(hipGetLastError());
+  // one workgroup per CU at this kernel's register use: split few-object
+  // launches to about three rounds of workgroups
+  int max_patches = 0, max_surf = 0;
+  for (const ObjDesc& d : b.h_objs) {
+    if (d.type == PII_GEOM_TRIANGLE) max_patches = std::max(max_patches, d.npatch);
+    else max_surf = std::max(max_surf, d.n);
+  }
+  const int base = ((ldX + 63) / 64) * N;
+  const int nsplit = std::max(split_count(base, max_patches, 2, 768),
+                              split_count(base, max_surf, 256, 768));
+  const size_t need = dtheta_split_doubles(N, ldX, nsplit);
+  if (need) c.w.split.ensure(sizeof(double) * need);
   c.phase_begin(3);
   launch_dtheta_affine(s, b.d_objs, N, kg.rep.as<double>(), Qh, d_drec, d_off,
-                       X + (size_t)na * ldX, ldX, colmax + na);
+                       X + (size_t)na * ldX, ldX, colmax + na, nsplit,
+                       need ? c.w.split.as<double>() : nullptr);
   c.phase_end(3);
   PII_HIP(hipGetLastError());
   c.w.H.ensure(sizeof(double) * (size_t)D * D);
@@ -205,7 +246,7 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
   unsigned* colmax = c.w.colmax.as<unsigned>();
   PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s));
   c.phase_begin(1);
-  launch_form_factors_fast(s, b, kg, 1, nullptr, 0, c.w.X.as<float2>(), ldX, colmax, &c);
+  launch_form_factors_fast(s, b, kg, 1, nullptr, 0, c.w.X.as<float2>(), ldX, colmax, c.w, &c);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   if (V_host) {

@@ -225,7 +225,8 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
                                                  const double* __restrict__ krep, int Qh,
                                                  int out_mode, double2* __restrict__ Aout,
                                                  int ldA, float2* __restrict__ Xout, int ldX,
-                                                 unsigned* __restrict__ colmax) {
+                                                 unsigned* __restrict__ colmax,
+                                                 double2* __restrict__ part, int ldP) {
   // Vertex phasor table; after the last patch its first 4 KB hold the
   // cross-wave reduction (every wave has passed the patch loop's final barrier).
   __shared__ float2 tab[kPatchVerts][kLanes];
@@ -243,8 +244,8 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
   const int q0 = d.qbeg + bo.blk * kLanes;
   if (q0 >= (out_mode == 0 ? Qh : ldX)) return;
   const int q = q0 + lane;
-  if (q0 >= Qh) {  // padding columns of X only
-    if (wave == 0 && q < ldX) Xout[(size_t)d.col * ldX + q] = make_float2(0.f, 0.f);
+  if (q0 >= Qh) {  // padding columns of X only (the finishing kernel's, when split)
+    if (!part && wave == 0 && q < ldX) Xout[(size_t)d.col * ldX + q] = make_float2(0.f, 0.f);
     return;
   }
   const int qc = q < Qh ? q : Qh - 1;
@@ -275,7 +276,10 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
     const auto* pverts = as_const(d.patch_verts);
     const auto* world = as_const(d.world);
     const auto* rec = as_const(d.rec);
-    for (int p = 0; p < d.npatch; ++p) {
+    // patches [pb, pe) of this workgroup's split (all when not split)
+    const int pchunk = (d.npatch + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int pb = (int)blockIdx.z * pchunk, pe = min(d.npatch, pb + pchunk);
+    for (int p = pb; p < pe; ++p) {
       const int vbeg = hdr[4 * p], vcnt = hdr[4 * p + 1];
       const int fbeg = hdr[4 * p + 2], fcnt = hdr[4 * p + 3];
       for (int s = wave; s < vcnt; s += kWaves) {
@@ -338,7 +342,9 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
     float px = 0.f, py = 0.f;
     int cnt = 0;
     const auto* rec = as_const(d.rec);
-    for (int e = wave; e < d.n; e += kWaves) {
+    const int echunk = (d.n + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int eb = (int)blockIdx.z * echunk, ee = min(d.n, eb + echunk);
+    for (int e = eb + wave; e < ee; e += kWaves) {
       const auto* r = rec + (size_t)kRecSurfel * e;
       const double kdn = kx * r[3] + ky * r[4] + kz * r[5];
       const double r2 = kp2 - kdn * kdn;
@@ -379,6 +385,11 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
   if (wave == 0) {
     double sx = 0.0, sy = 0.0;
     for (int w = 0; w < kWaves; ++w) { sx += red[w][lane].x; sy += red[w][lane].y; }
+    if (part) {  // split: partial sums, finished by k_ff_finish
+      part[((size_t)blockIdx.z * gridDim.y + bo.obj) * ldP + (size_t)bo.blk * kLanes + lane] =
+          make_double2(sx, sy);
+      return;
+    }
     if (out_mode == 0) {
       if (q < Qh) Aout[(size_t)d.col * ldA + q] = make_double2(sx, sy);
     } else {
@@ -391,6 +402,51 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
       if (lane == 0) atomicMax(colmax + d.col, __float_as_uint(m));
     }
   }
+}
+
+// Finishes a split launch: node i of object obj (q = qbeg + i) sums the
+// nsplit partials in split order, then writes A or X and the column maximum
+// as the unsplit epilogue of k_ff_fast.
+__global__ void __launch_bounds__(256) k_ff_finish(const ObjDesc* __restrict__ objs,
+                                                   const double* __restrict__ krep, int Qh,
+                                                   int out_mode, double2* __restrict__ Aout,
+                                                   int ldA, float2* __restrict__ Xout, int ldX,
+                                                   unsigned* __restrict__ colmax,
+                                                   const double2* __restrict__ part, int ldP,
+                                                   int nsplit) {
+  const int obj = blockIdx.y;
+  const ObjDesc& d = objs[obj];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = d.qbeg + i;
+  const int limit = out_mode == 0 ? Qh : ldX;
+  const bool valid = i < ldP && q < limit;
+  double sx = 0.0, sy = 0.0;
+  if (valid && q < Qh) {
+    const double2* p = part + (size_t)obj * ldP + i;
+    const size_t stride = (size_t)gridDim.y * ldP;
+    for (int z = 0; z < nsplit; ++z) {
+      const double2 v = p[(size_t)z * stride];
+      sx += v.x;
+      sy += v.y;
+    }
+  }
+  if (out_mode == 0) {
+    if (valid && q < Qh) Aout[(size_t)d.col * ldA + q] = make_double2(sx, sy);
+    return;
+  }
+  float m = 0.0f;
+  if (valid) {
+    float2 x = make_float2(0.f, 0.f);
+    if (q < Qh) {
+      const double sw = sqrt(krep[7 * (size_t)Qh + q]);
+      x = make_float2((float)(sx * sw), (float)(sy * sw));
+      m = fmaxf(fabsf(x.x), fabsf(x.y));
+    }
+    Xout[(size_t)d.col * ldX + q] = x;
+  }
+  // column maximum over the wave (every lane takes part in the shuffles)
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(colmax + d.col, __float_as_uint(m));
 }
 
 // Full-grid expansion of representative-node values: A(-k) = -conj(A(k)).
@@ -410,13 +466,25 @@ __global__ void __launch_bounds__(256) k_expand(const double2* __restrict__ Arep
 
 }  // namespace
 
+size_t ff_split_doubles(int N, int span, int nsplit) {
+  const size_t ldP = (size_t)(span + kLanes - 1) / kLanes * kLanes;
+  return nsplit > 1 ? 2 * (size_t)nsplit * N * ldP : 0;
+}
+
 void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                     int qbeg_min, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
-                    unsigned* d_colmax) {
+                    unsigned* d_colmax, int nsplit, double* d_part) {
   const int span = (out_mode == 0 ? Qh : ldX) - qbeg_min;
   if (N <= 0 || Qh <= 0 || span <= 0) return;
-  hipLaunchKernelGGL(k_ff_fast, dim3((span + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves), 0,
-                     s, d_objs, d_krep, Qh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+  const int nblk = (span + kLanes - 1) / kLanes;
+  const int ldP = nblk * kLanes;
+  const bool split = nsplit > 1 && d_part;
+  double2* part = split ? reinterpret_cast<double2*>(d_part) : nullptr;
+  hipLaunchKernelGGL(k_ff_fast, dim3(nblk, N, split ? nsplit : 1), dim3(kLanes * kWaves), 0, s,
+                     d_objs, d_krep, Qh, out_mode, d_A, ldA, d_X, ldX, d_colmax, part, ldP);
+  if (split)
+    hipLaunchKernelGGL(k_ff_finish, dim3((ldP + 255) / 256, N), dim3(256), 0, s, d_objs, d_krep,
+                       Qh, out_mode, d_A, ldA, d_X, ldX, d_colmax, part, ldP, nsplit);
 }
 
 void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,

@@ -49,6 +49,39 @@ struct SeriesCoef {
 };
 __device__ constexpr SeriesCoef kSeries{};
 
+// Sums the four waves' H_n into wave 0's registers through LDS, in the fixed
+// order (H0 + H2) + (H1 + H3).
+__device__ __forceinline__ void waves_to_wave0(double (&H)[P + 1], double (&red)[2][P + 1][kLanes],
+                                               int wave, int lane) {
+  if (wave >= 2) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) red[wave - 2][n][lane] = H[n];
+  }
+  __syncthreads();
+  if (wave < 2) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) H[n] += red[wave][n][lane];
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) red[0][n][lane] = H[n];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int n = 0; n <= P; ++n) H[n] += red[0][n][lane];
+  }
+}
+
+// Face-split launches (few objects): each workgroup of split z writes its
+// raw H_n to part[((z * nobj + obj) * nblk + blk) * (P + 1) * 64 + n * 64 +
+// lane]; a finishing kernel sums the splits in order and evaluates.
+__device__ __forceinline__ double* split_slot(double* part, int obj, int blk) {
+  return part + (((size_t)blockIdx.z * gridDim.y + obj) * gridDim.x + blk) * (size_t)(P + 1) *
+                    kLanes;
+}
+
 // Sums the four waves' H_n through LDS (in a fixed order: (H0 + H2) + (H1 +
 // H3)), applies the real factor of i^n (and, for triangles, 2/(n+2)!), then
 // each wave evaluates the series at every fourth of the object's series shells
@@ -122,15 +155,15 @@ __device__ __forceinline__ void reduce_and_evaluate(
   }
 }
 
-// One workgroup per (object, 64 directions): the four waves take every fourth
-// face, their H_n are summed through LDS, then the waves evaluate the series
-// at every fourth shell.  out_mode 0: A (FP64) to Aout[col*ldA + q]; 1:
+// One workgroup per (object, 64 directions[, face split]): the four waves
+// take every fourth face, their H_n are summed through LDS, then the waves
+// evaluate the series at every fourth shell (or, split, store the sums).  out_mode 0: A (FP64) to Aout[col*ldA + q]; 1:
 // X = sqrt(w) A (FP32) to Xout[col*ldX + q] and the column max |X| to
 // colmax[col].
 __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
     int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
-    unsigned* __restrict__ colmax) {
+    unsigned* __restrict__ colmax, double* __restrict__ part) {
   __shared__ double red[2][P + 1][kLanes];
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
@@ -145,7 +178,10 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
 #pragma unroll
   for (int n = 0; n <= P; ++n) H[n] = 0.0;
   const auto* rec = as_const(d.mrec);
-  const int nf = d.n;
+  // faces [fb, fe) of this workgroup's split (all faces when not split)
+  const int chunk = (d.n + (int)gridDim.z - 1) / (int)gridDim.z;
+  const int fb = (int)blockIdx.z * chunk;
+  const int fe = min(d.n, fb + chunk);
   // Projections of a face record: corners relative to the centre and the
   // area vector onto this lane's direction.
   auto project = [&](const auto* r, double& w0, double& w1, double& w2, double& s) {
@@ -154,13 +190,13 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
     w2 = dx * r[6] + dy * r[7] + dz * r[8];
     s = dx * r[9] + dy * r[10] + dz * r[11];
   };
-  double w0, w1, w2, s;
-  project(rec + (size_t)kRecMoment * (wave < nf ? wave : nf - 1), w0, w1, w2, s);
-  for (int f = wave; f < nf; f += kWaves) {
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0, s = 0.0;
+  if (fb < fe) project(rec + (size_t)kRecMoment * (fb + wave < fe ? fb + wave : fe - 1), w0, w1, w2, s);
+  for (int f = fb + wave; f < fe; f += kWaves) {
     const double c0 = w0, c1 = w1, c2 = w2, cs = s;
     // the next face's record loads are issued before the sums of this face
-    // and consumed after them (index clamped to the object)
-    const int fn = f + kWaves < nf ? f + kWaves : nf - 1;
+    // and consumed after them (index clamped to the range)
+    const int fn = f + kWaves < fe ? f + kWaves : fe - 1;
     const auto* rn = rec + (size_t)kRecMoment * fn;
     double t[kRecMoment];
 #pragma unroll
@@ -178,6 +214,15 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
       H[n] = fma(cs, habc, H[n]);
     }
     project(t, w0, w1, w2, s);
+  }
+  if (part) {
+    waves_to_wave0(H, red, wave, lane);
+    if (wave == 0) {
+      double* o = split_slot(part, bo.obj, bo.blk);
+#pragma unroll
+      for (int n = 0; n <= P; ++n) o[n * kLanes + lane] = H[n];
+    }
+    return;
   }
   reduce_and_evaluate<true>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
                             Aout, ldA, Xout, ldX, colmax);
@@ -211,7 +256,7 @@ template <bool kDisk>
 __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
     int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
-    unsigned* __restrict__ colmax) {
+    unsigned* __restrict__ colmax, double* __restrict__ part) {
   __shared__ double red[2][P + 1][kLanes];
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
@@ -237,7 +282,10 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
 #pragma unroll
   for (int n = 0; n <= P; ++n) H[n] = 0.0;
   const auto* rec = as_const(d.rec);
-  for (int e = wave; e < d.n; e += kWaves) {
+  const int chunk = (d.n + (int)gridDim.z - 1) / (int)gridDim.z;
+  const int eb = (int)blockIdx.z * chunk;
+  const int ee = min(d.n, eb + chunk);
+  for (int e = eb + wave; e < ee; e += kWaves) {
     const auto* r = rec + (size_t)kRecSurfel * e;
     const double mu = dx * r[3] + dy * r[4] + dz * r[5];
     const double w = dx * r[0] + dy * r[1] + dz * r[2] - dc;
@@ -276,29 +324,92 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
       }
     }
   }
+  if (part) {
+    waves_to_wave0(H, red, wave, lane);
+    if (wave == 0) {
+      double* o = split_slot(part, bo.obj, bo.blk);
+#pragma unroll
+      for (int n = 0; n <= P; ++n) o[n * kLanes + lane] = H[n];
+    }
+    return;
+  }
   reduce_and_evaluate<false>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
                              Aout, ldA, Xout, ldX, colmax);
 }
 
+// Finishes a face-split series launch: the four waves each sum every fourth
+// split's H_n for their 64 directions (splits in increasing order), then the
+// usual reduction and evaluation.  type selects the objects of the launch.
+template <bool kTriCoef>
+__global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
+    const ObjDesc* __restrict__ objs, int type, const double* __restrict__ krep, int Qh, int Mh,
+    int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
+    unsigned* __restrict__ colmax, const double* __restrict__ part, int nsplit) {
+  __shared__ double red[2][P + 1][kLanes];
+  const BlockObj bo = xcd_block_map();
+  const ObjDesc& d = objs[bo.obj];
+  if (d.nser <= 0 || d.type != type) return;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j = bo.blk * kLanes + lane;
+  const int jc = j < Mh ? j : Mh - 1;
+  const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
+               dz = krep[5 * (size_t)Qh + jc];
+  double H[P + 1];
+#pragma unroll
+  for (int n = 0; n <= P; ++n) H[n] = 0.0;
+  const size_t stride = (size_t)gridDim.y * gridDim.x * (P + 1) * kLanes;
+  const double* src = part + ((size_t)bo.obj * gridDim.x + bo.blk) * (size_t)(P + 1) * kLanes;
+  for (int z = wave; z < nsplit; z += kWaves) {
+    const double* o = src + (size_t)z * stride;
+#pragma unroll
+    for (int n = 0; n <= P; ++n) H[n] += o[n * kLanes + lane];
+  }
+  reduce_and_evaluate<kTriCoef>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
+                                Aout, ldA, Xout, ldX, colmax);
+}
+
 }  // namespace
+
+size_t series_split_doubles(int N, int Mh, int nsplit) {
+  return nsplit > 1 ? (size_t)nsplit * N * ((Mh + kLanes - 1) / kLanes) * (P + 1) * kLanes : 0;
+}
 
 void launch_series_surfel(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
                           int Qh, int Mh, int out_mode, double2* d_A, int ldA, float2* d_X,
-                          int ldX, unsigned* d_colmax) {
+                          int ldX, unsigned* d_colmax, int nsplit, double* d_part) {
   if (N <= 0 || Mh <= 0) return;
-  const dim3 grid((Mh + kLanes - 1) / kLanes, N);
+  const bool split = nsplit > 1 && d_part;
+  const dim3 grid((Mh + kLanes - 1) / kLanes, N, split ? nsplit : 1);
+  const dim3 grid2((Mh + kLanes - 1) / kLanes, N);
+  double* part = split ? d_part : nullptr;
   hipLaunchKernelGGL(k_series_surfel<true>, grid, dim3(kLanes * kWaves), 0, s, d_objs, d_krep, Qh,
-                     Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+                     Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax, part);
+  if (split)
+    hipLaunchKernelGGL(k_series_finish<false>, grid2, dim3(kLanes * kWaves), 0, s, d_objs,
+                       (int)PII_GEOM_DISK, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax,
+                       part, nsplit);
   hipLaunchKernelGGL(k_series_surfel<false>, grid, dim3(kLanes * kWaves), 0, s, d_objs, d_krep,
-                     Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+                     Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax, part);
+  if (split)
+    hipLaunchKernelGGL(k_series_finish<false>, grid2, dim3(kLanes * kWaves), 0, s, d_objs,
+                       (int)PII_GEOM_GAUSSIAN, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX,
+                       d_colmax, part, nsplit);
 }
 
 void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                        int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
-                       unsigned* d_colmax) {
+                       unsigned* d_colmax, int nsplit, double* d_part) {
   if (N <= 0 || Mh <= 0) return;
-  hipLaunchKernelGGL(k_series_tri, dim3((Mh + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves), 0,
-                     s, d_objs, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+  const bool split = nsplit > 1 && d_part;
+  double* part = split ? d_part : nullptr;
+  hipLaunchKernelGGL(k_series_tri, dim3((Mh + kLanes - 1) / kLanes, N, split ? nsplit : 1),
+                     dim3(kLanes * kWaves), 0, s, d_objs, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X,
+                     ldX, d_colmax, part);
+  if (split)
+    hipLaunchKernelGGL(k_series_finish<true>, dim3((Mh + kLanes - 1) / kLanes, N),
+                       dim3(kLanes * kWaves), 0, s, d_objs, (int)PII_GEOM_TRIANGLE, d_krep, Qh,
+                       Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax, part, nsplit);
 }
 
 }  // namespace pii

@@ -80,7 +80,7 @@ struct Batch {
 // null) records the two kernels' phases and the work counters.
 void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
                               int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
-                              unsigned* d_colmax, Context* prof);
+                              unsigned* d_colmax, Workspace& w, Context* prof);
 
 // Fast path (FP32 transcendentals, FP64 phases and Taylor branch).
 void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out_host);

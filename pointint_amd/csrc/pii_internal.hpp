@@ -134,6 +134,7 @@ struct GeometryDev {
 struct Workspace {
   DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw, mrec, gam;
   DevBuf dth, Xt, cmax2, H;  // Gauss-Newton Hessian: records, dA/dtheta columns, maxima, H
+  DevBuf split;              // partial sums of element-split launches (few objects)
 };
 
 struct Context {
@@ -199,18 +200,25 @@ void launch_gram_fp64_x(hipStream_t s, const float2* d_X, int Qh, int N, int ldX
 // q < ldX (zeros beyond Qh) and the column max |X| to d_colmax[col].
 // Each object starts at its own representative node ObjDesc::qbeg (nodes
 // below come from the moment series); qbeg_min is the smallest over objects.
+// nsplit > 1 (with d_part of ff_split_doubles(...) doubles) splits every
+// object's elements over nsplit workgroups per node block and sums their
+// partial results in a finishing kernel: the form of a launch with few
+// objects, which would otherwise occupy a few CUs.
 void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                     int qbeg_min, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
-                    unsigned* d_colmax);
+                    unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
+size_t ff_split_doubles(int N, int span, int nsplit);
 // Moment series of triangle objects at their first nser shells (objects with
-// nser = 0 are skipped); outputs as launch_ff_fast.
+// nser = 0 are skipped); outputs as launch_ff_fast; nsplit as launch_ff_fast
+// with d_part of series_split_doubles(...) doubles.
 void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                        int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
-                       unsigned* d_colmax);
+                       unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
 // The same for disk and Gaussian objects.
 void launch_series_surfel(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
                           int Qh, int Mh, int out_mode, double2* d_A, int ldA, float2* d_X,
-                          int ldX, unsigned* d_colmax);
+                          int ldX, unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
+size_t series_split_doubles(int N, int Mh, int nsplit);
 // Affine-DoF derivatives for the Gauss-Newton Hessian: per-face records
 // (reference corners + N, 18 floats, patch order, at d_off[slot]) and the
 // columns X_t = sqrt(w) dA/dtheta_t (12 per object, column 12 col + t).
@@ -218,7 +226,8 @@ void launch_dtheta_records(hipStream_t s, const ObjDesc* d_objs, const int* d_ep
                            int total_e, float* d_out, const long long* d_off);
 void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
                           int Qh, const float* d_drec, const long long* d_doff, float2* d_Xt,
-                          int ldX, unsigned* d_colmax);
+                          int ldX, unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
+size_t dtheta_split_doubles(int N, int ldX, int nsplit);
 // Full-grid values from representative values: A(-k) = -conj(A(k)).
 void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,
                    const int* d_anti, int Qh, int Q, int N, double2* d_Afull);

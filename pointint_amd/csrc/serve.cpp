@@ -125,7 +125,7 @@ struct Server {
     unsigned* colmax = s.w.colmax.as<unsigned>();
     PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s.stream));
     launch_form_factors_fast(s.stream, s.batch, *kg, 1, nullptr, 0, s.w.X.as<float2>(), ldX,
-                             colmax, nullptr);
+                             colmax, s.w, nullptr);
     PII_HIP(hipGetLastError());
     // One Gram per query over its column range.
     size_t vtot = 0, maxpart = 0;
