@@ -53,7 +53,7 @@ PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 (= non-packed FP32) peak
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB device copy
 # (fetch x2.00, write x1.00); profiles/ff_fast_counters_icospheres256_L64x1202.md
 # holds the counts and commands.  Counters cannot be read inside this run.
-MEASURED_TRAFFIC = {("icospheres", "k_ff_fast (per-element)"): 61.93e6}
+MEASURED_TRAFFIC = {("icospheres", "k_ff_fast (per-element)"): 60.77e6}
 PEAK_MFMA_F16_TFLOPS = 2500.0
 
 
