@@ -277,16 +277,23 @@ def run_align(args):
         import oracle as O
         import align_oracle as AO
         og1, og2 = O.geom_mesh(V1, F1), O.geom_mesh(V2, F2)
-        stride = max(1, kg.Q // 48)
-        idx = np.arange(0, kg.Q, stride)
-        t0 = time.perf_counter()
-        AO.pair_terms(O, og1, og2, x, kg.dirs[idx], kg.kmag[idx], kg.w[idx], AO.ALL)
-        AO.pair_terms(O, og1, og2, x, kg.dirs[idx], kg.kmag[idx], kg.w[idx], AO.FUNC)
-        t_s = time.perf_counter() - t0
+
+        def sample(stride):
+            idx = np.arange(0, kg.Q, stride)
+            t0 = time.perf_counter()
+            AO.pair_terms(O, og1, og2, x, kg.dirs[idx], kg.kmag[idx], kg.w[idx], AO.ALL)
+            AO.pair_terms(O, og1, og2, x, kg.dirs[idx], kg.kmag[idx], kg.w[idx], AO.FUNC)
+            return idx, time.perf_counter() - t0
+
+        # calibrate on a few nodes, then size the sample to about --cpu-seconds / 3
+        idx, t_cal = sample(max(1, kg.Q // 24))
+        n_target = int(len(idx) * args.cpu_seconds / 3.0 / max(t_cal, 1e-3))
+        idx, t_s = sample(max(1, kg.Q // max(len(idx), min(kg.Q, n_target))))
         cpu = {"value": 1.0 / (t_s * kg.Q / len(idx)), "unit": "Newton iterations/s",
                "cores": O.threads(), "kind": "port",
                "sample": f"{len(idx)} of {kg.Q} k-nodes (one SIM_ALL + one function "
-                         f"evaluation), extrapolated linearly in Q; {t_s:.1f} s on "
+                         f"evaluation; A on the threaded oracle, dA/dtheta node by node on "
+                         f"one thread), extrapolated linearly in Q; {t_s:.1f} s on "
                          f"{cpu_model()}"}
     out = {
         "metric": "alignment: Newton iterations/s (SIM_ALL evaluation + one line-search "

@@ -191,16 +191,60 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
                   const GeometryDev* const* geoms, const int* kinds, const double* const* dofs,
                   const int* ndofs, bool fast, const KGridDev* series_kg) {
   N = n;
-  // Largest objects first: the dispatcher then starts the longest work early.
+  for (int i = 0; i < N; ++i) validate_object(*geoms[i], kinds[i], ndofs[i]);
+  // Moment series plan per object: leading shells with |k| R <= kSeriesReach,
+  // when at least kSeriesMinShells of them (below that the per-element
+  // kernels are cheaper than one pass over the elements per direction).
+  // Disks and Gaussians also keep |k|^2 |A|_F^2 rho^2 / 4 <= 3
+  // (sigma^2 / 2 <= 0.49), where ten terms of the J1 (exponential) factor
+  // reach 1e-11.
+  constexpr int kSeriesMinShells = 4, kSurfelMinShells = 8;
+  std::vector<int> nser_i(N, 0);
+  std::vector<double> kser_i(N, 0.0);
+  std::vector<std::array<double, 3>> ctr_i(N, {0.0, 0.0, 0.0});
+  Mh = 0;
+  if (fast && series_kg && series_kg->nshell > 0) {
+    Mh = series_kg->Mh;
+    for (int i = 0; i < N; ++i) {
+      const GeometryDev& g = *geoms[i];
+      if (g.n == 0) continue;
+      const bool tri = g.type == PII_GEOM_TRIANGLE;
+      double R;
+      world_bounds(g, kinds[i], dofs[i], ctr_i[i].data(), R);
+      double cap2 = HUGE_VAL;  // bound on |k|^2 from the surfel factor
+      if (!tri) {
+        double an2 = 0.0;
+        for (int t = 3; t < 12; ++t) an2 += dofs[i][t] * dofs[i][t];
+        const double f = (g.type == PII_GEOM_DISK ? 0.25 : 0.5) * g.ref_smax * g.ref_smax * an2;
+        if (f > 0.0) cap2 = (g.type == PII_GEOM_DISK ? 3.0 : 0.49) / f;
+      }
+      int ns = 0;
+      double km = 0.0;
+      while (ns < series_kg->nshell) {
+        const double k = series_kg->h_shell_k[ns];
+        if (!(k >= 0.0 && k * R <= kSeriesReach && k * k <= cap2)) break;
+        km = std::max(km, k);
+        ++ns;
+      }
+      if (ns < (tri ? kSeriesMinShells : kSurfelMinShells)) continue;
+      nser_i[i] = ns;
+      kser_i[i] = km;
+    }
+  }
+  // Most per-element work first (elements x nodes left to the per-element
+  // kernels): the dispatcher starts the longest work early, and the XCD
+  // mapping deals the objects to the XCDs in this order.
+  const int Qn = series_kg ? series_kg->Qh : 0;
+  std::vector<double> cost(N);
+  for (int i = 0; i < N; ++i)
+    cost[i] = (double)geoms[i]->n * (double)(fast && Mh > 0 ? Qn - nser_i[i] * Mh : 1);
   order.resize(N);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return geoms[a]->n > geoms[b]->n; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
   // DoF vectors, concatenated.
   std::vector<size_t> dof_off(N);
   size_t total_dofs = 0;
   for (int i = 0; i < N; ++i) {
-    validate_object(*geoms[i], kinds[i], ndofs[i]);
     dof_off[i] = total_dofs;
     total_dofs += align_up((size_t)ndofs[i], 32);
   }
@@ -227,49 +271,19 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   }
   w.world.ensure(std::max<size_t>(total_w, 1) * sizeof(double));
   w.faces.ensure(std::max<size_t>(total_rec, 1) * sizeof(double));
-  // Moment series plan: leading shells with |k| R <= kSeriesReach, when at
-  // least kSeriesMinShells of them (below that the per-element kernels are
-  // cheaper than one pass over the elements per direction).  Disks and
-  // Gaussians also keep |k|^2 |A|_F^2 rho^2 / 4 <= 3 (sigma^2 / 2 <= 0.49),
-  // where ten terms of the J1 (exponential) factor reach 1e-11.
-  constexpr int kSeriesMinShells = 4, kSurfelMinShells = 8;
   std::vector<int> nser(N, 0);
   std::vector<double> kser(N, 0.0);
   std::vector<std::array<double, 3>> ctr(N, {0.0, 0.0, 0.0});
   std::vector<size_t> moff(N, 0);
   size_t total_m = 0;
-  Mh = 0;
-  if (fast && series_kg && series_kg->nshell > 0) {
-    Mh = series_kg->Mh;
-    for (int slot = 0; slot < N; ++slot) {
-      const int i = order[slot];
-      const GeometryDev& g = *geoms[i];
-      if (g.n == 0) continue;
-      const bool tri = g.type == PII_GEOM_TRIANGLE;
-      double R;
-      world_bounds(g, kinds[i], dofs[i], ctr[slot].data(), R);
-      double cap2 = HUGE_VAL;  // bound on |k|^2 from the surfel factor
-      if (!tri) {
-        double an2 = 0.0;
-        for (int t = 3; t < 12; ++t) an2 += dofs[i][t] * dofs[i][t];
-        const double f = (g.type == PII_GEOM_DISK ? 0.25 : 0.5) * g.ref_smax * g.ref_smax * an2;
-        if (f > 0.0) cap2 = (g.type == PII_GEOM_DISK ? 3.0 : 0.49) / f;
-      }
-      int ns = 0;
-      double km = 0.0;
-      while (ns < series_kg->nshell) {
-        const double k = series_kg->h_shell_k[ns];
-        if (!(k >= 0.0 && k * R <= kSeriesReach && k * k <= cap2)) break;
-        km = std::max(km, k);
-        ++ns;
-      }
-      if (ns < (tri ? kSeriesMinShells : kSurfelMinShells)) continue;
-      nser[slot] = ns;
-      kser[slot] = km;
-      if (tri) {
-        moff[slot] = total_m;
-        total_m += align_up((size_t)g.n * kRecMoment, 32);
-      }
+  for (int slot = 0; slot < N; ++slot) {
+    const int i = order[slot];
+    nser[slot] = nser_i[i];
+    kser[slot] = kser_i[i];
+    ctr[slot] = ctr_i[i];
+    if (nser[slot] > 0 && geoms[i]->type == PII_GEOM_TRIANGLE) {
+      moff[slot] = total_m;
+      total_m += align_up((size_t)geoms[i]->n * kRecMoment, 32);
     }
   }
   if (total_m) w.mrec.ensure(total_m * sizeof(double));
