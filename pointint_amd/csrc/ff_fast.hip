@@ -112,9 +112,10 @@ __device__ __forceinline__ float j1ox_small(float x) {
 // sinc(h) from h and sin(h): polynomial below |h| = kSincPoly, sin(h)/h above
 // (both evaluated, then selected, so lanes do not diverge).
 __device__ __forceinline__ float sinc_from(float h, float sin_h) {
+  // 1 - h^2/6 + h^4/120 - h^6/5040: the first omitted term, h^8/9!, is below
+  // 1.1e-8 for |h| < 0.5
   const float h2 = h * h;
-  float p = h2 * (1.0f / 362880.0f) - (1.0f / 5040.0f);
-  p = p * h2 + (1.0f / 120.0f);
+  float p = h2 * (-1.0f / 5040.0f) + (1.0f / 120.0f);
   p = p * h2 - (1.0f / 6.0f);
   p = p * h2 + 1.0f;
   const float r = sin_h * __builtin_amdgcn_rcpf(h);
@@ -177,11 +178,14 @@ __device__ __forceinline__ float2 tri_main(double al, double be, float g2, float
   return make_float2(ty * f, -tx * f);
 }
 
-// gamma * e^{i u0} * Phi(alpha, beta) with a per-lane branch on the span.
+// gamma * e^{i u0} * Phi(alpha, beta) with a per-lane branch on the span
+// max(0, a, b) - min(0, a, b) = max(|a|, |b|, |a - b|), taken in FP32 (both
+// sides of the threshold are accurate near it).
 __device__ __forceinline__ float2 tri_term(double al, double be, float g2, float2 Q0, float2 Q1,
                                            float2 Q2) {
-  const double H = fmax(0.0, fmax(al, be)) - fmin(0.0, fmin(al, be));
-  if (H < kSpanTaylor) return tri_taylor<12>(al, be, g2, Q0);
+  const float a = (float)al, b = (float)be;
+  const float H = fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(a - b));
+  if (H < (float)kSpanTaylor) return tri_taylor<12>(al, be, g2, Q0);
   return tri_main(al, be, g2, Q0, Q1, Q2);
 }
 
