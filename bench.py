@@ -349,8 +349,8 @@ def run_serving(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ff", default="fast", choices=["fast", "fp64"])
     ap.add_argument("--gram", default="bf16x3", choices=["fp64", "bf16x3", "fp16x3"])
     ap.add_argument("--no-fp64-ref", action="store_true", help="skip the plain FP64 GPU run")

@@ -187,6 +187,19 @@ static void world_bounds(const GeometryDev& g, int kind, const double* dofs, dou
   R = std::sqrt(r2) * (1.0 + 1e-12);
 }
 
+// Smallest series order of kSeriesOrders whose tail bound at X = |k| R,
+// X^(P+1) / (P+1)! / (1 - X / (P+2)), is below 3.6e-10 (of the area).
+static int series_order(double X) {
+  for (int i = 0; i < kNumSeriesOrders; ++i) {
+    const int P = kSeriesOrders[i];
+    if (X >= P + 2) continue;
+    double t = 1.0;  // X^(P+1) / (P+1)!
+    for (int m = 1; m <= P + 1; ++m) t *= X / m;
+    if (t / (1.0 - X / (P + 2)) < 3.6e-10) return P;
+  }
+  return kSeriesOrder;
+}
+
 void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
                   const GeometryDev* const* geoms, const int* kinds, const double* const* dofs,
                   const int* ndofs, bool fast, const KGridDev* series_kg) {
@@ -199,7 +212,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   // (sigma^2 / 2 <= 0.49), where ten terms of the J1 (exponential) factor
   // reach 1e-11.
   constexpr int kSeriesMinShells = 4, kSurfelMinShells = 8;
-  std::vector<int> nser_i(N, 0);
+  std::vector<int> nser_i(N, 0), nord_i(N, 0);
   std::vector<double> kser_i(N, 0.0);
   std::vector<std::array<double, 3>> ctr_i(N, {0.0, 0.0, 0.0});
   Mh = 0;
@@ -229,6 +242,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
       if (ns < (tri ? kSeriesMinShells : kSurfelMinShells)) continue;
       nser_i[i] = ns;
       kser_i[i] = km;
+      nord_i[i] = tri ? series_order(km * R) : kSeriesOrder;
     }
   }
   // Most per-element work first (elements x nodes left to the per-element
@@ -271,7 +285,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   }
   w.world.ensure(std::max<size_t>(total_w, 1) * sizeof(double));
   w.faces.ensure(std::max<size_t>(total_rec, 1) * sizeof(double));
-  std::vector<int> nser(N, 0);
+  std::vector<int> nser(N, 0), nord(N, 0);
   std::vector<double> kser(N, 0.0);
   std::vector<std::array<double, 3>> ctr(N, {0.0, 0.0, 0.0});
   std::vector<size_t> moff(N, 0);
@@ -279,6 +293,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   for (int slot = 0; slot < N; ++slot) {
     const int i = order[slot];
     nser[slot] = nser_i[i];
+    nord[slot] = nord_i[i];
     kser[slot] = kser_i[i];
     ctr[slot] = ctr_i[i];
     if (nser[slot] > 0 && geoms[i]->type == PII_GEOM_TRIANGLE) {
@@ -313,6 +328,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     d.world = w.world.as<double>() + wv_off[slot];
     d.rec = w.faces.as<double>() + rec_off[slot];
     d.nser = nser[slot];
+    d.nord = nord[slot];
     d.qbeg = nser[slot] * Mh;
     d.kser = kser[slot];
     for (int k = 0; k < 3; ++k) d.ctr[k] = ctr[slot][k];

@@ -18,9 +18,9 @@ struct ObjDesc {
   int dof_kind;        // PII_DOF_*
   int col;             // output column
   // Low-|k| moment series (triangles on a shell-structured k-grid): the first
-  // nser shells come from the series, representative nodes q >= qbeg = nser*Mh
-  // from the per-face kernel.
-  int nser, qbeg, pad0, pad1;
+  // nser shells come from the series (of order nord for triangles),
+  // representative nodes q >= qbeg = nser*Mh from the per-face kernel.
+  int nser, qbeg, nord, pad1;
   double ctr[3];       // expansion centre of the series (world frame)
   double kser;         // largest |k| of the object's series shells
   const double* verts;        // nv*3 reference-frame vertices
@@ -52,6 +52,11 @@ constexpr int kRecMoment = 12;
 // P = 56 and |k| R <= 15 the tail is below 3.6e-10 of the total area.
 constexpr int kSeriesOrder = 56;
 constexpr double kSeriesReach = 15.0;
+// Triangle objects take the smallest of these orders whose tail bound at the
+// object's own largest |k| R is below 3.6e-10 of the area (kernels exist for
+// each).
+constexpr int kSeriesOrders[] = {32, 40, 44, 48, 52, 56};
+constexpr int kNumSeriesOrders = 6;
 // Per (object, direction) record of the series adjoint: Gamma_0..P, then
 // (m+3) Gamma_{m+1} for m < P; padded to a multiple of 8 doubles.
 constexpr int kGammaStride = (2 * kSeriesOrder + 1 + 7) / 8 * 8;

@@ -16,9 +16,11 @@
 //   h_p = (u_m - u_p)/2,  h_q = (u_q - u_m)/2,
 //
 // and sin(h) = Im(Q_b conj Q_a) comes from the phasors themselves.  When the
-// span u_q - u_p is below kSpanTaylor the divided difference is summed from
-// its Taylor series in FP64:  Phi(a, b) = sum_n 2 i^n h_n(a, b) / (n+2)!,
-// h_n the complete homogeneous symmetric polynomial.
+// span u_q - u_p is small the divided difference is summed from its Taylor
+// series in FP64:  Phi(a, b) = sum_n 2 i^n h_n(a, b) / (n+2)!, h_n the complete
+// homogeneous symmetric polynomial: for a whole face x wave when a per-face
+// span bound puts every lane below kSpanTaylor (order from the bound), else
+// per lane below kLaneTaylor.
 //
 // A workgroup (4 waves) owns 64 k-nodes (one per lane, same nodes in every
 // wave) of one object.  Faces are grouped in patches of <= 64 vertices; per
@@ -35,6 +37,11 @@ namespace {
 constexpr int kLanes = 64;
 constexpr int kWaves = 4;
 constexpr double kSpanTaylor = 1.0;   // Taylor branch below this span (radians)
+// Inside a face x wave whose span bound exceeds kSpanTaylor, single lanes with
+// a span below kLaneTaylor take the Taylor sum (7 terms suffice there); the
+// others, spans >= 0.3, take the main formula, whose FP32 phasor error over
+// the span stays below 1e-6 of the term.
+constexpr float kLaneTaylor = 0.3f;
 constexpr float kSincPoly = 0.5f;     // sinc(h) by polynomial below |h| = 0.5
 
 // e^{i 2 pi r} for r in revolutions (|r| <= 0.5), FP32 result, on the
@@ -180,12 +187,13 @@ __device__ __forceinline__ float2 tri_main(double al, double be, float g2, float
 
 // gamma * e^{i u0} * Phi(alpha, beta) with a per-lane branch on the span
 // max(0, a, b) - min(0, a, b) = max(|a|, |b|, |a - b|), taken in FP32 (both
-// sides of the threshold are accurate near it).
+// sides of the threshold are accurate near it).  Lanes below kLaneTaylor
+// take the 7-term Taylor sum (first omitted term < 1e-8 there).
 __device__ __forceinline__ float2 tri_term(double al, double be, float g2, float2 Q0, float2 Q1,
                                            float2 Q2) {
   const float a = (float)al, b = (float)be;
   const float H = fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(a - b));
-  if (H < (float)kSpanTaylor) return tri_taylor<12>(al, be, g2, Q0);
+  if (H < kLaneTaylor) return tri_taylor<7>(al, be, g2, Q0);
   return tri_main(al, be, g2, Q0, Q1, Q2);
 }
 

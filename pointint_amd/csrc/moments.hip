@@ -16,8 +16,10 @@
 // H_n depends on the direction only, so one pass over the faces per direction
 // serves every shell of the grid with kappa R <= kSeriesReach (R bounds
 // |v - c|).  Each term is bounded by Area (kappa R)^n / n!, so the tail after
-// P = kSeriesOrder terms is below Area X^(P+1)/(P+1)! / (1 - X/(P+2)) at
-// X = kappa R: 3.6e-10 Area for P = 56, X = 15.  The largest term, about
+// P terms is below Area X^(P+1)/(P+1)! / (1 - X/(P+2)) at X = kappa R:
+// 3.6e-10 Area for P = 56, X = 15.  Each triangle object takes the smallest
+// order of kSeriesOrders that meets that bound at its own largest series
+// kappa R (nord; a kernel per order).  The largest term, about
 // Area e^X / sqrt(2 pi X), keeps FP64 rounding below 4e-11 Area.
 //
 // h_n by the recurrences (all FP64, 4 operations per order and face)
@@ -49,76 +51,72 @@ struct SeriesCoef {
 };
 __device__ constexpr SeriesCoef kSeries{};
 
-// Sums the four waves' H_n into wave 0's registers through LDS, in the fixed
-// order (H0 + H2) + (H1 + H3).
-__device__ __forceinline__ void waves_to_wave0(double (&H)[P + 1], double (&red)[2][P + 1][kLanes],
-                                               int wave, int lane) {
-  if (wave >= 2) {
+// Sums the four waves' H_n into wave 0's registers through one LDS buffer,
+// in the fixed order ((H0 + H1) + H2) + H3.
+// dst(n) = f(n) for n = 0..PP in chunks of 8 orders: the scheduling barriers
+// keep the compiler from issuing a whole array of loads or stores at once
+// into fresh registers.
+template <int PP, class F>
+__device__ __forceinline__ void for_chunked(F f) {
 #pragma unroll
-    for (int n = 0; n <= P; ++n) red[wave - 2][n][lane] = H[n];
+  for (int n0 = 0; n0 <= PP; n0 += 8) {
+#pragma unroll
+    for (int n = n0; n < n0 + 8 && n <= PP; ++n) f(n);
+    __builtin_amdgcn_sched_barrier(0);
   }
+}
+
+// red = (H1 + H2) + H3 in LDS, in that order, the waves' H registers only
+// read (a conditional update of H would make the compiler keep two copies of
+// the array).  Wave 0 then folds in its own H.
+template <int PP>
+__device__ __forceinline__ void sum_waves_123(const double (&H)[PP + 1],
+                                              double (&red)[PP + 1][kLanes], int wave, int lane) {
+  if (wave == 1) for_chunked<PP>([&](int n) { red[n][lane] = H[n]; });
   __syncthreads();
-  if (wave < 2) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) H[n] += red[wave][n][lane];
-  }
+  if (wave == 2) for_chunked<PP>([&](int n) { red[n][lane] += H[n]; });
   __syncthreads();
-  if (wave == 1) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) red[0][n][lane] = H[n];
-  }
+  if (wave == 3) for_chunked<PP>([&](int n) { red[n][lane] += H[n]; });
   __syncthreads();
-  if (wave == 0) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) H[n] += red[0][n][lane];
-  }
+}
+
+// o[n * 64] = H[n] for n = 0..PP (the raw sums of a split workgroup).
+template <int PP>
+__device__ __forceinline__ void store_split(const double (&H)[PP + 1],
+                                            double (&red)[PP + 1][kLanes], int wave, int lane,
+                                            double* o) {
+  sum_waves_123<PP>(H, red, wave, lane);
+  if (wave == 0) for_chunked<PP>([&](int n) { o[n * kLanes] = red[n][lane] + H[n]; });
 }
 
 // Face-split launches (few objects): each workgroup of split z writes its
 // raw H_n to part[((z * nobj + obj) * nblk + blk) * (P + 1) * 64 + n * 64 +
-// lane]; a finishing kernel sums the splits in order and evaluates.
+// lane] (stride P + 1 whatever the order of the launch); a finishing kernel
+// sums the splits in order and evaluates.
 __device__ __forceinline__ double* split_slot(double* part, int obj, int blk) {
   return part + (((size_t)blockIdx.z * gridDim.y + obj) * gridDim.x + blk) * (size_t)(P + 1) *
                     kLanes;
 }
 
-// Sums the four waves' H_n through LDS (in a fixed order: (H0 + H2) + (H1 +
-// H3)), applies the real factor of i^n (and, for triangles, 2/(n+2)!), then
-// each wave evaluates the series at every fourth of the object's series shells
-// for its 64 directions.  out_mode 0: A (FP64) to Aout[col*ldA + q]; 1:
-// X = sqrt(w) A (FP32) to Xout[col*ldX + q] and the column max |X| to
-// colmax[col].
-template <bool kTriCoef>
+// Sums the four waves' H_n ((H1 + H2) + H3) + H0, applies the real factor of i^n
+// (and, for triangles, 2/(n+2)!) and leaves the coefficients in LDS; then each
+// wave evaluates the series at every fourth of the object's series shells for
+// its 64 directions, reading the coefficients from LDS.  out_mode 0: A (FP64)
+// to Aout[col*ldA + q]; 1: X = sqrt(w) A (FP32) to Xout[col*ldX + q] and the
+// column max |X| to colmax[col].
+template <int PP, bool kTriCoef>
 __device__ __forceinline__ void reduce_and_evaluate(
-    double (&H)[P + 1], double (&red)[2][P + 1][kLanes], int wave, int lane, int j, int jc,
+    double (&H)[PP + 1], double (&red)[PP + 1][kLanes], int wave, int lane, int j, int jc,
     double dx, double dy, double dz, const ObjDesc& d, const double* __restrict__ krep, int Qh,
     int Mh, int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout,
     int ldX, unsigned* __restrict__ colmax) {
-  // (H0 + H2) + (H1 + H3), in a fixed order
-  if (wave >= 2) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) red[wave - 2][n][lane] = H[n];
-  }
+  sum_waves_123<PP>(H, red, wave, lane);
+  if (wave == 0)
+    for_chunked<PP>([&](int n) {
+      red[n][lane] = (red[n][lane] + H[n]) *
+                     (kTriCoef ? kSeries.c[n] : (((n / 2) % 2 == 0) ? 1.0 : -1.0));
+    });
   __syncthreads();
-  if (wave < 2) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) H[n] += red[wave][n][lane];
-  }
-  __syncthreads();
-  if (wave == 1) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n) red[0][n][lane] = H[n];
-  }
-  __syncthreads();
-  if (wave == 0) {
-#pragma unroll
-    for (int n = 0; n <= P; ++n)
-      red[1][n][lane] = (H[n] + red[0][n][lane]) *
-                        (kTriCoef ? kSeries.c[n] : (((n / 2) % 2 == 0) ? 1.0 : -1.0));
-  }
-  __syncthreads();
-#pragma unroll
-  for (int n = 0; n <= P; ++n) H[n] = red[1][n][lane];
   const int nser = d.nser;
   const double dc = dx * d.ctr[0] + dy * d.ctr[1] + dz * d.ctr[2];
   float m = 0.0f;
@@ -127,12 +125,12 @@ __device__ __forceinline__ void reduce_and_evaluate(
     const double kap = krep[6 * (size_t)Qh + q];
     const double k2 = kap * kap;
     // even part E = sum a_{2t} kappa^{2t}, odd part O = sum a_{2t+1} kappa^{2t}
-    constexpr int ne = P / 2, no = (P - 1) / 2;
-    double E = H[2 * ne], O = H[2 * no + 1];
+    constexpr int ne = PP / 2, no = (PP - 1) / 2;
+    double E = red[2 * ne][lane], O = red[2 * no + 1][lane];
 #pragma unroll
-    for (int t = ne - 1; t >= 0; --t) E = fma(E, k2, H[2 * t]);
+    for (int t = ne - 1; t >= 0; --t) E = fma(E, k2, red[2 * t][lane]);
 #pragma unroll
-    for (int t = no - 1; t >= 0; --t) O = fma(O, k2, H[2 * t + 1]);
+    for (int t = no - 1; t >= 0; --t) O = fma(O, k2, red[2 * t + 1][lane]);
     O *= kap;
     double sn, cs;
     sincos(kap * dc, &sn, &cs);
@@ -155,28 +153,27 @@ __device__ __forceinline__ void reduce_and_evaluate(
   }
 }
 
-// One workgroup per (object, 64 directions[, face split]): the four waves
-// take every fourth face, their H_n are summed through LDS, then the waves
-// evaluate the series at every fourth shell (or, split, store the sums).  out_mode 0: A (FP64) to Aout[col*ldA + q]; 1:
-// X = sqrt(w) A (FP32) to Xout[col*ldX + q] and the column max |X| to
-// colmax[col].
-__global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
-    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
-    int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
-    unsigned* __restrict__ colmax, double* __restrict__ part) {
-  __shared__ double red[2][P + 1][kLanes];
-  const BlockObj bo = xcd_block_map();
-  const ObjDesc& d = objs[bo.obj];
-  if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE) return;  // uniform over the workgroup
+// One workgroup per (object, 64 directions[, face split]) for the objects of
+// series order PP (others exit): the four waves take every fourth face, their
+// H_n are summed through LDS, then the waves evaluate the series at every
+// fourth shell (or, split, store the sums).  out_mode 0: A (FP64) to
+// Aout[col*ldA + q]; 1: X = sqrt(w) A (FP32) to Xout[col*ldX + q] and the
+// column max |X| to colmax[col].
+template <int PP>
+__device__ __forceinline__ void series_tri_body(
+    const ObjDesc& d, BlockObj bo, double* red_lds, const double* __restrict__ krep, int Qh,
+    int Mh, int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout,
+    int ldX, unsigned* __restrict__ colmax, double* __restrict__ part) {
+  double (&red)[PP + 1][kLanes] = *reinterpret_cast<double (*)[PP + 1][kLanes]>(red_lds);
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int j = bo.blk * kLanes + lane;
   const int jc = j < Mh ? j : Mh - 1;
   const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
                dz = krep[5 * (size_t)Qh + jc];
-  double H[P + 1];
+  double H[PP + 1];
 #pragma unroll
-  for (int n = 0; n <= P; ++n) H[n] = 0.0;
+  for (int n = 0; n <= PP; ++n) H[n] = 0.0;
   const auto* rec = as_const(d.mrec);
   // faces [fb, fe) of this workgroup's split (all faces when not split)
   const int chunk = (d.n + (int)gridDim.z - 1) / (int)gridDim.z;
@@ -207,7 +204,7 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
     double b = 1.0, hab = 1.0, habc = 1.0;
     H[0] += cs;
 #pragma unroll
-    for (int n = 1; n <= P; ++n) {
+    for (int n = 1; n <= PP; ++n) {
       b *= c1;
       hab = fma(c0, hab, b);
       habc = fma(c2, habc, hab);
@@ -216,16 +213,41 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
     project(t, w0, w1, w2, s);
   }
   if (part) {
-    waves_to_wave0(H, red, wave, lane);
-    if (wave == 0) {
-      double* o = split_slot(part, bo.obj, bo.blk);
-#pragma unroll
-      for (int n = 0; n <= P; ++n) o[n * kLanes + lane] = H[n];
-    }
+    store_split<PP>(H, red, wave, lane, split_slot(part, bo.obj, bo.blk) + lane);
     return;
   }
-  reduce_and_evaluate<true>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
-                            Aout, ldA, Xout, ldX, colmax);
+  reduce_and_evaluate<PP, true>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
+                                Aout, ldA, Xout, ldX, colmax);
+}
+
+// One launch for all triangle objects: each workgroup runs the body of its
+// object's series order (a uniform branch).
+#define PII_SERIES_SWITCH(CALL)                          \
+  switch (d.nord) {                                      \
+    case 32: CALL(32); break;                            \
+    case 40: CALL(40); break;                            \
+    case 44: CALL(44); break;                            \
+    case 48: CALL(48); break;                            \
+    case 52: CALL(52); break;                            \
+    default: CALL(56); break;                            \
+  }
+static_assert(kNumSeriesOrders == 6 && kSeriesOrders[0] == 32 && kSeriesOrders[1] == 40 &&
+                  kSeriesOrders[2] == 44 && kSeriesOrders[3] == 48 && kSeriesOrders[4] == 52 &&
+                  kSeriesOrders[5] == 56,
+              "the order switch lists kSeriesOrders");
+
+__global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
+    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
+    int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
+    unsigned* __restrict__ colmax, double* __restrict__ part) {
+  __shared__ double red[P + 1][kLanes];
+  const BlockObj bo = xcd_block_map();
+  const ObjDesc& d = objs[bo.obj];
+  if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE) return;  // uniform over the workgroup
+#define PII_TRI_BODY(PP) \
+  series_tri_body<PP>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout, ldX, colmax, part)
+  PII_SERIES_SWITCH(PII_TRI_BODY)
+#undef PII_TRI_BODY
 }
 
 // Disks and Gaussians (affine DoFs; cpp/src/dof/affine_dof.cpp:103-167).  For
@@ -257,7 +279,7 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
     int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
     unsigned* __restrict__ colmax, double* __restrict__ part) {
-  __shared__ double red[2][P + 1][kLanes];
+  __shared__ double red[P + 1][kLanes];
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
   if (d.nser <= 0 || d.type != (kDisk ? PII_GEOM_DISK : PII_GEOM_GAUSSIAN)) return;
@@ -325,48 +347,60 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
     }
   }
   if (part) {
-    waves_to_wave0(H, red, wave, lane);
-    if (wave == 0) {
-      double* o = split_slot(part, bo.obj, bo.blk);
-#pragma unroll
-      for (int n = 0; n <= P; ++n) o[n * kLanes + lane] = H[n];
-    }
+    store_split<P>(H, red, wave, lane, split_slot(part, bo.obj, bo.blk) + lane);
     return;
   }
-  reduce_and_evaluate<false>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
-                             Aout, ldA, Xout, ldX, colmax);
+  reduce_and_evaluate<P, false>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
+                                Aout, ldA, Xout, ldX, colmax);
 }
 
 // Finishes a face-split series launch: the four waves each sum every fourth
 // split's H_n for their 64 directions (splits in increasing order), then the
-// usual reduction and evaluation.  type selects the objects of the launch.
-template <bool kTriCoef>
-__global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
-    const ObjDesc* __restrict__ objs, int type, const double* __restrict__ krep, int Qh, int Mh,
-    int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
-    unsigned* __restrict__ colmax, const double* __restrict__ part, int nsplit) {
-  __shared__ double red[2][P + 1][kLanes];
-  const BlockObj bo = xcd_block_map();
-  const ObjDesc& d = objs[bo.obj];
-  if (d.nser <= 0 || d.type != type) return;
+// usual reduction and evaluation.  type (and, for triangles, the order PP)
+// selects the objects of the launch.
+template <int PP, bool kTriCoef>
+__device__ __forceinline__ void series_finish_body(
+    const ObjDesc& d, BlockObj bo, double* red_lds, const double* __restrict__ krep, int Qh,
+    int Mh, int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout,
+    int ldX, unsigned* __restrict__ colmax, const double* __restrict__ part, int nsplit) {
+  double (&red)[PP + 1][kLanes] = *reinterpret_cast<double (*)[PP + 1][kLanes]>(red_lds);
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int j = bo.blk * kLanes + lane;
   const int jc = j < Mh ? j : Mh - 1;
   const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
                dz = krep[5 * (size_t)Qh + jc];
-  double H[P + 1];
+  double H[PP + 1];
 #pragma unroll
-  for (int n = 0; n <= P; ++n) H[n] = 0.0;
+  for (int n = 0; n <= PP; ++n) H[n] = 0.0;
   const size_t stride = (size_t)gridDim.y * gridDim.x * (P + 1) * kLanes;
   const double* src = part + ((size_t)bo.obj * gridDim.x + bo.blk) * (size_t)(P + 1) * kLanes;
   for (int z = wave; z < nsplit; z += kWaves) {
     const double* o = src + (size_t)z * stride;
-#pragma unroll
-    for (int n = 0; n <= P; ++n) H[n] += o[n * kLanes + lane];
+    for_chunked<PP>([&](int n) { H[n] += o[n * kLanes + lane]; });
   }
-  reduce_and_evaluate<kTriCoef>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
-                                Aout, ldA, Xout, ldX, colmax);
+  reduce_and_evaluate<PP, kTriCoef>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh,
+                                    out_mode, Aout, ldA, Xout, ldX, colmax);
+}
+
+__global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
+    const ObjDesc* __restrict__ objs, int type, const double* __restrict__ krep, int Qh, int Mh,
+    int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
+    unsigned* __restrict__ colmax, const double* __restrict__ part, int nsplit) {
+  __shared__ double red[P + 1][kLanes];
+  const BlockObj bo = xcd_block_map();
+  const ObjDesc& d = objs[bo.obj];
+  if (d.nser <= 0 || d.type != type) return;
+  if (type == PII_GEOM_TRIANGLE) {
+#define PII_FIN_BODY(PP)                                                                     \
+  series_finish_body<PP, true>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout, \
+                               ldX, colmax, part, nsplit)
+    PII_SERIES_SWITCH(PII_FIN_BODY)
+#undef PII_FIN_BODY
+  } else {
+    series_finish_body<P, false>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout, ldX,
+                                 colmax, part, nsplit);
+  }
 }
 
 }  // namespace
@@ -386,13 +420,13 @@ void launch_series_surfel(hipStream_t s, const ObjDesc* d_objs, int N, const dou
   hipLaunchKernelGGL(k_series_surfel<true>, grid, dim3(kLanes * kWaves), 0, s, d_objs, d_krep, Qh,
                      Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax, part);
   if (split)
-    hipLaunchKernelGGL(k_series_finish<false>, grid2, dim3(kLanes * kWaves), 0, s, d_objs,
+    hipLaunchKernelGGL(k_series_finish, grid2, dim3(kLanes * kWaves), 0, s, d_objs,
                        (int)PII_GEOM_DISK, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax,
                        part, nsplit);
   hipLaunchKernelGGL(k_series_surfel<false>, grid, dim3(kLanes * kWaves), 0, s, d_objs, d_krep,
                      Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax, part);
   if (split)
-    hipLaunchKernelGGL(k_series_finish<false>, grid2, dim3(kLanes * kWaves), 0, s, d_objs,
+    hipLaunchKernelGGL(k_series_finish, grid2, dim3(kLanes * kWaves), 0, s, d_objs,
                        (int)PII_GEOM_GAUSSIAN, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX,
                        d_colmax, part, nsplit);
 }
@@ -407,7 +441,7 @@ void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double
                      dim3(kLanes * kWaves), 0, s, d_objs, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X,
                      ldX, d_colmax, part);
   if (split)
-    hipLaunchKernelGGL(k_series_finish<true>, dim3((Mh + kLanes - 1) / kLanes, N),
+    hipLaunchKernelGGL(k_series_finish, dim3((Mh + kLanes - 1) / kLanes, N),
                        dim3(kLanes * kWaves), 0, s, d_objs, (int)PII_GEOM_TRIANGLE, d_krep, Qh,
                        Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax, part, nsplit);
 }

@@ -209,8 +209,9 @@ void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d
                     unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
 size_t ff_split_doubles(int N, int span, int nsplit);
 // Moment series of triangle objects at their first nser shells (objects with
-// nser = 0 are skipped); outputs as launch_ff_fast; nsplit as launch_ff_fast
-// with d_part of series_split_doubles(...) doubles.
+// nser = 0 are skipped), each at its own order nord; outputs as
+// launch_ff_fast; nsplit as launch_ff_fast with d_part of
+// series_split_doubles(...) doubles.
 void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                        int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                        unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
