@@ -189,8 +189,21 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
   const float fk[3] = {(float)kx, (float)ky, (float)kz};
   const float fh[3] = {hx, hy, hz};
   // accumulators: A (for dA/dt = i k A) and dA/dA_rc, FP32 per patch / element
-  // block, FP64 across them
-  double sA[2] = {0.0, 0.0}, gA[9][2] = {};
+  // block, FP64 across them in LDS (acc[wave][i][lane], i = 0..19: A re, im,
+  // then dA/dA_rc re, im): FP64 registers for them would push the kernel past
+  // 256 VGPRs, i.e. to one wave per SIMD
+  __shared__ double acc[kWaves][20][kLanes];
+#pragma unroll
+  for (int i = 0; i < 20; ++i) acc[wave][i][lane] = 0.0;
+  auto flush = [&](const float (&pA)[2], const float (&pg)[9][2]) {
+    acc[wave][0][lane] += (double)pA[0];
+    acc[wave][1][lane] += (double)pA[1];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      acc[wave][2 + 2 * i][lane] += (double)pg[i][0];
+      acc[wave][3 + 2 * i][lane] += (double)pg[i][1];
+    }
+  };
 
   if (d.type == PII_GEOM_TRIANGLE) {
     const auto* hdr = as_const(d.patch_hdr);
@@ -254,13 +267,7 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
           }
         }
       }
-      sA[0] += (double)pA[0];
-      sA[1] += (double)pA[1];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        gA[i][0] += (double)pg[i][0];
-        gA[i][1] += (double)pg[i][1];
-      }
+      flush(pA, pg);
       __syncthreads();
     }
   } else {
@@ -321,61 +328,29 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
         }
       }
       if (++cnt == 64) {
-        sA[0] += (double)pA[0]; sA[1] += (double)pA[1];
+        flush(pA, pg);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-          gA[i][0] += (double)pg[i][0]; gA[i][1] += (double)pg[i][1];
-          pg[i][0] = pg[i][1] = 0.f;
-        }
+        for (int i = 0; i < 9; ++i) pg[i][0] = pg[i][1] = 0.f;
         pA[0] = pA[1] = 0.f;
         cnt = 0;
       }
     }
-    sA[0] += (double)pA[0]; sA[1] += (double)pA[1];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-      gA[i][0] += (double)pg[i][0]; gA[i][1] += (double)pg[i][1];
-    }
-  }
-  if (km < 1e-10) {  // dA/dtheta = 0 at k = 0 (affine_dof.cpp:181)
-    sA[0] = sA[1] = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) gA[i][0] = gA[i][1] = 0.0;
+    flush(pA, pg);
   }
 
-  // Sum the four waves' partial sums of the same 64 nodes (through LDS: the
-  // phasor table is free after the last patch's barrier).
-  double* red = reinterpret_cast<double*>(&tab[0][0]);  // 20 x 64 doubles per wave slot
-  static_assert(sizeof(double) * 20 * kLanes * 2 <= sizeof(float2) * kPatchVerts * kLanes,
-                "reduction buffer must fit in the phasor table");
-  double v[20];
-  v[0] = sA[0];
-  v[1] = sA[1];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    v[2 + 2 * i] = gA[i][0];
-    v[3 + 2 * i] = gA[i][1];
-  }
-  __syncthreads();
-  // (w0 + w2) + (w1 + w3) in a fixed order, two slots
-  if (wave >= 2) {
-#pragma unroll
-    for (int i = 0; i < 20; ++i) red[((wave - 2) * 20 + i) * kLanes + lane] = v[i];
-  }
-  __syncthreads();
-  if (wave < 2) {
-#pragma unroll
-    for (int i = 0; i < 20; ++i) v[i] += red[(wave * 20 + i) * kLanes + lane];
-  }
-  __syncthreads();
-  if (wave == 1) {
-#pragma unroll
-    for (int i = 0; i < 20; ++i) red[i * kLanes + lane] = v[i];
-  }
+  // Sum the four waves' partial sums of the same 64 nodes in a fixed order,
+  // ((w0 + w1) + w2) + w3 (each wave reads only what it wrote itself before
+  // the barrier).
   __syncthreads();
   if (wave != 0) return;
+  double v[20];
 #pragma unroll
-  for (int i = 0; i < 20; ++i) v[i] += red[i * kLanes + lane];
+  for (int i = 0; i < 20; ++i)
+    v[i] = ((acc[0][i][lane] + acc[1][i][lane]) + acc[2][i][lane]) + acc[3][i][lane];
+  if (km < 1e-10) {  // dA/dtheta = 0 at k = 0 (affine_dof.cpp:181)
+#pragma unroll
+    for (int i = 0; i < 20; ++i) v[i] = 0.0;
+  }
   if (part) {
     double* o = part + (((size_t)blockIdx.z * gridDim.y + bo.obj) * gridDim.x * kLanes +
                         (size_t)bo.blk * kLanes + lane) * 20;

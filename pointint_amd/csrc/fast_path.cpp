@@ -6,12 +6,13 @@
 
 namespace pii {
 
-// Element split for launches with few workgroups: enough splits to reach
-// about `target` workgroups, each split keeping at least min_per elements.
+// Element split for launches with few workgroups: as many splits as keep the
+// launch within about `target` workgroups, each split keeping at least
+// min_per elements.
 static int split_count(int base_wgs, int max_elems, int min_per, int target) {
   constexpr int kFewWGs = 256;
   if (base_wgs <= 0 || base_wgs >= kFewWGs || max_elems <= 0) return 1;
-  const int want = (target + base_wgs - 1) / base_wgs;
+  const int want = std::max(1, target / base_wgs);
   return std::max(1, std::min(want, max_elems / min_per));
 }
 
@@ -164,16 +165,16 @@ static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_
   PII_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(long long) * N, hipMemcpyHostToDevice, s));
   launch_dtheta_records(s, b.d_objs, b.d_eprefix, N, b.total_e, d_drec, d_off);
   PII_HIP(hipGetLastError());
-  // one workgroup per CU at this kernel's register use: split few-object
-  // launches to about three rounds of workgroups
+  // two workgroups per CU at this kernel's register and LDS use: split
+  // few-object launches to about two rounds of workgroups
   int max_patches = 0, max_surf = 0;
   for (const ObjDesc& d : b.h_objs) {
     if (d.type == PII_GEOM_TRIANGLE) max_patches = std::max(max_patches, d.npatch);
     else max_surf = std::max(max_surf, d.n);
   }
   const int base = ((ldX + 63) / 64) * N;
-  const int nsplit = std::max(split_count(base, max_patches, 2, 768),
-                              split_count(base, max_surf, 256, 768));
+  const int nsplit = std::max(split_count(base, max_patches, 2, 1024),
+                              split_count(base, max_surf, 256, 1024));
   const size_t need = dtheta_split_doubles(N, ldX, nsplit);
   if (need) c.w.split.ensure(sizeof(double) * need);
   c.phase_begin(3);
