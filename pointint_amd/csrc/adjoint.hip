@@ -17,10 +17,10 @@
 //   dkappa/dv0 = khat x (e2 - e1)/2, dkappa/dv1 = -khat x e2 / 2,
 //   dkappa/dv2 = khat x e1 / 2.
 //
-// For spans of at least kSpanTaylor, G_j comes from differentiating the
+// For spans of at least kLaneTaylor, G_j comes from differentiating the
 // forward kernel's middle-node formula (tri_main_adj); below, from the Taylor
 // series 2 g[0, a, b, d] = sum_n 2 i^(n+1) h_n(a, b, d) / (n+3)! (times
-// e^{i u0}) in FP64.
+// e^{i u0}) in FP64, to order 7.
 //
 // This kernel covers the nodes from each object's qbeg (the high-|k| shells);
 // adjoint_series.hip covers the shells below.
@@ -95,17 +95,12 @@ __global__ void __launch_bounds__(256) k_adjoint_tri(const ObjDesc* __restrict__
       const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
       const float2 Q0 = tab[loc & 0xff][lane], Q1 = tab[(loc >> 8) & 0xff][lane],
                    Q2 = tab[(loc >> 16) & 0xff][lane];
-      const double H = fmax(0.0, fmax(al, be)) - fmin(0.0, fmin(al, be));
+      // span max(|a|, |b|, |a - b|) in FP32
+      const float fa = (float)al, fb = (float)be;
+      const float H = fmaxf(fmaxf(fabsf(fa), fabsf(fb)), fabsf(fa - fb));
       float2 F, G0, G1, G2;
-      if (H < kSpanTaylor) {
-        double2 T0, T1, T2;
-        dd3_taylor(al, be, T0, T1, T2);
-        const double2 tf = phi_taylor(al, be);
-        const float2 P0 = cmulf(Q0, Q0);  // e^{i u0}
-        F = cmulf(P0, make_float2((float)tf.x, (float)tf.y));
-        G0 = cmulf(P0, make_float2((float)T0.x, (float)T0.y));
-        G1 = cmulf(P0, make_float2((float)T1.x, (float)T1.y));
-        G2 = cmulf(P0, make_float2((float)T2.x, (float)T2.y));
+      if (H < kLaneTaylor) {
+        tri_taylor_adj<7>(al, be, Q0, F, G0, G1, G2);
       } else {
         tri_main_adj(al, be, Q0, Q1, Q2, F, G0, G1, G2);
       }

@@ -232,17 +232,12 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
         const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
         const float2 Q0 = tab[loc & 0xff][lane], Q1 = tab[(loc >> 8) & 0xff][lane],
                      Q2 = tab[(loc >> 16) & 0xff][lane];
-        const double H = fmax(0.0, fmax(al, be)) - fmin(0.0, fmin(al, be));
+        // span max(|a|, |b|, |a - b|) in FP32
+        const float fa = (float)al, fb = (float)be;
+        const float H = fmaxf(fmaxf(fabsf(fa), fabsf(fb)), fabsf(fa - fb));
         float2 F, G0, G1, G2;
-        if (H < kSpanTaylor) {
-          double2 T0, T1, T2;
-          dd3_taylor(al, be, T0, T1, T2);
-          const double2 tf = phi_taylor(al, be);
-          const float2 P0 = cmulf(Q0, Q0);
-          F = cmulf(P0, make_float2((float)tf.x, (float)tf.y));
-          G0 = cmulf(P0, make_float2((float)T0.x, (float)T0.y));
-          G1 = cmulf(P0, make_float2((float)T1.x, (float)T1.y));
-          G2 = cmulf(P0, make_float2((float)T2.x, (float)T2.y));
+        if (H < kLaneTaylor) {
+          tri_taylor_adj<7>(al, be, Q0, F, G0, G1, G2);
         } else {
           tri_main_adj(al, be, Q0, Q1, Q2, F, G0, G1, G2);
         }

@@ -14,6 +14,9 @@ namespace pii {
 namespace {
 
 constexpr double kSpanTaylor = 1.0;
+// Lanes with a span below kLaneTaylor take the Taylor sums; the main formula
+// above it (its FP32 error over the span squared stays near 1e-6 of G there).
+constexpr float kLaneTaylor = 0.3f;
 constexpr float kSincPoly = 0.5f;
 
 __device__ __forceinline__ float2 phasor_rev_a(double r) {
@@ -46,27 +49,42 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Phi(a, b) = sum_n 2 i^n h_n(a, b) / (n+2)!  (FP64), returned as (re, im).
+// Phi(a, b) = sum_{n <= NO} 2 i^n h_n(a, b) / (n+2)!  (FP64), returned as
+// (re, im).  |h_n| <= (n+1) H^n for span H, so the first omitted term is
+// below 2 (NO+2) H^(NO+1) / (NO+3)!.
+struct PhiCoef {
+  double c[12];  // 2/(n+2)! with the sign of the real factor of i^n
+  constexpr PhiCoef() : c() {
+    double f = 1.0;
+    for (int n = 0; n < 12; ++n) {
+      c[n] = (((n / 2) % 2 == 0) ? 1.0 : -1.0) * f;
+      f /= (double)(n + 3);
+    }
+  }
+};
+__device__ constexpr PhiCoef kPhiCoef{};
+
+template <int NO>
 __device__ __forceinline__ double2 phi_taylor(double a, double b) {
+  static_assert(NO >= 1 && NO <= 11, "order range");
   double an = 1.0, hn = 1.0, re = 1.0, im = 0.0;
-  an *= a; hn = fma(b, hn, an); im += hn * (1.0 / 3.0);
-  an *= a; hn = fma(b, hn, an); re -= hn * (1.0 / 12.0);
-  an *= a; hn = fma(b, hn, an); im -= hn * (1.0 / 60.0);
-  an *= a; hn = fma(b, hn, an); re += hn * (1.0 / 360.0);
-  an *= a; hn = fma(b, hn, an); im += hn * (1.0 / 2520.0);
-  an *= a; hn = fma(b, hn, an); re -= hn * (1.0 / 20160.0);
-  an *= a; hn = fma(b, hn, an); im -= hn * (1.0 / 181440.0);
-  an *= a; hn = fma(b, hn, an); re += hn * (1.0 / 1814400.0);
-  an *= a; hn = fma(b, hn, an); im += hn * (1.0 / 19958400.0);
-  an *= a; hn = fma(b, hn, an); re -= hn * (1.0 / 239500800.0);
-  an *= a; hn = fma(b, hn, an); im -= hn * (1.0 / 3113510400.0);
+#pragma unroll
+  for (int n = 1; n <= NO; ++n) {
+    an *= a;
+    hn = fma(b, hn, an);
+    if (n & 1) im = fma(hn, kPhiCoef.c[n], im);
+    else re = fma(hn, kPhiCoef.c[n], re);
+  }
   return make_double2(re, im);
 }
 
-// 2 g[0, a, b, d] by its Taylor series (relative to u0), for d in {0, a, b}:
-// returns the three values for d = 0, a, b.
+// 2 g[0, a, b, d] by its Taylor series (relative to u0) to order NO, for d
+// in {0, a, b}: returns the three values for d = 0, a, b.  The order-n term
+// has |h3_n| <= (n+1)(n+2)/2 H^n and factor 2/(n+3)!.
+template <int NO>
 __device__ __forceinline__ void dd3_taylor(double a, double b, double2& T0, double2& T1,
                                            double2& T2) {
+  static_assert(NO >= 1 && NO <= 11, "order range");
   // h2_n = h_n(a, b); h3_n(d) = d h3_{n-1}(d) + h2_n.  Coefficients
   // c_n = 2 i^(n+1) / (n+3)!  : i/3, -1/12, -i/60, 1/360, ...
   double an = 1.0, h2 = 1.0, hA = 1.0, hB = 1.0;
@@ -75,7 +93,7 @@ __device__ __forceinline__ void dd3_taylor(double a, double b, double2& T0, doub
                         -1.0 / 181440.0, 1.0 / 1814400.0, 1.0 / 19958400.0,
                         -1.0 / 239500800.0, -1.0 / 3113510400.0, 1.0 / 43589145600.0};
 #pragma unroll
-  for (int n = 1; n <= 11; ++n) {
+  for (int n = 1; n <= NO; ++n) {
     an *= a;
     h2 = fma(b, h2, an);
     hA = fma(a, hA, h2);
@@ -90,6 +108,25 @@ __device__ __forceinline__ void dd3_taylor(double a, double b, double2& T0, doub
   T0 = make_double2(r0, i0);
   T1 = make_double2(r1, i1);
   T2 = make_double2(r2, i2);
+}
+
+// F, G_0..2 by the Taylor sums for one lane: NO = 7 keeps the first omitted
+// terms below 1e-8 for spans up to kLaneTaylor, NO = 11 up to kSpanTaylor.
+template <int NO>
+__device__ __forceinline__ void tri_taylor_adj(double al, double be, float2 Q0, float2& F,
+                                               float2& G0, float2& G1, float2& G2) {
+  double2 T0, T1, T2;
+  dd3_taylor<NO>(al, be, T0, T1, T2);
+  const double2 tf = phi_taylor<NO>(al, be);
+  const float2 P0 = make_float2(Q0.x * Q0.x - Q0.y * Q0.y, 2.0f * Q0.x * Q0.y);  // e^{i u0}
+  const float2 fF = make_float2((float)tf.x, (float)tf.y);
+  F = make_float2(P0.x * fF.x - P0.y * fF.y, P0.x * fF.y + P0.y * fF.x);
+  const float2 f0 = make_float2((float)T0.x, (float)T0.y);
+  const float2 f1 = make_float2((float)T1.x, (float)T1.y);
+  const float2 f2 = make_float2((float)T2.x, (float)T2.y);
+  G0 = make_float2(P0.x * f0.x - P0.y * f0.y, P0.x * f0.y + P0.y * f0.x);
+  G1 = make_float2(P0.x * f1.x - P0.y * f1.y, P0.x * f1.y + P0.y * f1.x);
+  G2 = make_float2(P0.x * f2.x - P0.y * f2.y, P0.x * f2.y + P0.y * f2.x);
 }
 
 // Sum over the 64 lanes of a wave with DPP moves (VALU only, no LDS):
