@@ -481,6 +481,22 @@ def main():
                                          + 24.0 * sum(o.V.shape[0] for o in wl.objects)
                                          + 64.0 * kg.Qh + 8.0 * n_direct)
 
+    # The Gram against both of its roofs: algorithmic flops 4 Q per pair over
+    # the N (N + 1) / 2 unique pairs (the split's x3 products not counted),
+    # bytes X read once (8 B per representative node and object) + V written.
+    gram_flop = 2.0 * kg.Q * N * (N + 1)
+    gram_bytes = 8.0 * kg.Qh * N + 8.0 * N * N
+    g_s = float(np.mean(gram_ms)) * 1e-3
+    gram_roof = {"mfma_tflops": gram_flop / g_s / 1e12,
+                 "mfma_frac": gram_flop / g_s / 1e12 / PEAK_MFMA_F16_TFLOPS,
+                 "hbm_gbps": gram_bytes / g_s / 1e9, "hbm_frac": gram_bytes / g_s / 1e9 / 8000.0,
+                 "algorithmic_flop": gram_flop, "algorithmic_bytes": gram_bytes}
+    # sustained issue rates of this device from probe kernels, beside the
+    # published peaks (the chip's clock under load sets the difference)
+    measured_peaks = {"fp64_fma_tflops": ctx.issue_rate("fp64_fma") / 1e12,
+                      "fp32_fma_tflops": ctx.issue_rate("fp32_fma") / 1e12,
+                      "fp32_sin_per_s": ctx.issue_rate("sin")}
+
     cpu = None
     if not args.no_cpu:
         t_full, nq, t_s, cores = cpu_baseline(wl, kg, args.cpu_seconds)
@@ -520,9 +536,15 @@ def main():
         "gram_ms": float(np.mean(gram_ms)),
         "prepass_ms": float(np.mean(pre_ms)),
         "roofline": roofline,
+        "gram_roofline": gram_roof,
+        "measured_peaks": measured_peaks,
         "cpu_baseline": cpu,
     }
     out.update(ref)
+    if ref:
+        out["phase_speedup_vs_fp64"] = {
+            "form_factor": ref["fp64_ff_ms"] / out["ff_ms"],
+            "gram": ref["fp64_gram_ms"] / out["gram_ms"]}
     print(json.dumps(out))
 
 

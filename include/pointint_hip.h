@@ -213,6 +213,12 @@ int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* ge
                           const pii_kgrid* kg, const double* dL_dV, double* const* grads,
                           double* V);
 
+/* Measurement helper, no counterpart in the reference: the device's
+ * sustained VALU issue rate for the roofline figures of the benchmark.  kind
+ * 0: FP64 FMA (flop/s), 1: FP32 FMA (flop/s, non-packed, as this library is
+ * compiled), 2: FP32 hardware sine (per second).  Runs about 10-30 ms. */
+int pii_measure_issue_rate(pii_context* ctx, int kind, double* per_second);
+
 /* Gram of caller-provided form factors: J[n*Q] complex (object-major,
  * interleaved re/im, host memory) over the full grid of kg. */
 int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int gram_mode,

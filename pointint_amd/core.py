@@ -78,6 +78,14 @@ class Context:
         return {"tri_node": u[0], "series_face_dir": u[1], "series_node": u[2],
                 "surfel_node": u[3], "series_surfel_dir": u[4]}
 
+    def issue_rate(self, kind: str) -> float:
+        """Sustained device rate from a probe kernel: "fp64_fma" / "fp32_fma"
+        (flop/s) or "sin" (FP32 hardware sines/s)."""
+        k = {"fp64_fma": 0, "fp32_fma": 1, "sin": 2}[kind]
+        out = C.c_double()
+        N.check(N.load().pii_measure_issue_rate(self._h, k, C.byref(out)))
+        return out.value
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             N.load().pii_context_destroy(self._h)

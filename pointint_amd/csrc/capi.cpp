@@ -874,6 +874,22 @@ int pii_volume_matrix_affine_derivatives(pii_context* ctx, int n, const pii_geom
   });
 }
 
+int pii_measure_issue_rate(pii_context* ctx, int kind, double* per_second) {
+  return guarded([&] {
+    require(ctx && per_second, "pii_measure_issue_rate: NULL argument");
+    require(kind >= 0 && kind <= 2, "pii_measure_issue_rate: kind must be 0, 1 or 2");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    int cus = 0;
+    PII_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->c.device));
+    const int blocks = std::max(1, cus) * 8;
+    DevBuf sink;
+    sink.ensure(sizeof(double) * (size_t)blocks);
+    *per_second = probe_issue_rate(ctx->c.stream, kind, blocks, kind == 2 ? 16384 : 65536,
+                                   sink.p);
+  });
+}
+
 int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int gram_mode,
              double* V) {
   return guarded([&] {

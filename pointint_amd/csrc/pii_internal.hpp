@@ -229,6 +229,10 @@ void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const dou
                           int Qh, const float* d_drec, const long long* d_doff, float2* d_Xt,
                           int ldX, unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
 size_t dtheta_split_doubles(int N, int ldX, int nsplit);
+// Sustained issue rate of one probe launch (blocks x 256 threads, iters
+// rounds): kind 0 FP64 FMA flop/s, 1 FP32 FMA flop/s, 2 FP32 hardware sines/s.
+// d_sink: blocks doubles (written only under a condition that never holds).
+double probe_issue_rate(hipStream_t s, int kind, int blocks, int iters, void* d_sink);
 // Full-grid values from representative values: A(-k) = -conj(A(k)).
 void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,
                    const int* d_anti, int Qh, int Q, int N, double2* d_Afull);

@@ -132,3 +132,18 @@ def test_full_size_grid_errors(oracle):
         V = P.compute_volume_matrix(geoms, kg, ff="fast", gram=gram)
         print(gram, "scaled V error", vscaled(V, Vo))
         assert vscaled(V, Vo) < 1e-5
+
+
+def test_issue_rate_probe():
+    """The probe kernels behind the benchmark's measured peaks: positive rates
+    below the published MI355X peaks (FP64 vector 78.6 TFLOP/s; non-packed
+    FP32 the same; sine at a quarter of the FMA issue rate)."""
+    ctx = P.Context.default()
+    f64 = ctx.issue_rate("fp64_fma")
+    f32 = ctx.issue_rate("fp32_fma")
+    sn = ctx.issue_rate("sin")
+    print(f"FP64 FMA {f64 / 1e12:.1f} TFLOP/s, FP32 FMA {f32 / 1e12:.1f} TFLOP/s, "
+          f"sin {sn / 1e12:.2f} T/s")
+    assert 5e12 < f64 < 1.05 * 78.6e12
+    assert 5e12 < f32 < 1.05 * 157.3e12
+    assert 1e11 < sn < 1.05 * 78.6e12 / 2
