@@ -45,19 +45,12 @@ constexpr float kLaneTaylor = 0.3f;
 constexpr float kSincPoly = 0.5f;     // sinc(h) by polynomial below |h| = 0.5
 
 // e^{i 2 pi r} for r in revolutions (|r| <= 0.5), FP32 result, on the
-// hardware sine/cosine (revolution input): quadrant reduction in FP64 to
-// |f| <= 1/8 revolution, V_SIN/V_COS_F32 on f, quadrant rotation by selects.
-// Measured on MI355X: max abs error 1.25e-7 over [-0.5, 0.5] revolutions.
+// hardware sine/cosine (revolution input).  Measured on MI355X over
+// [-0.5, 0.5] revolutions: max abs error 1.25e-7, the same with or without a
+// quadrant reduction to |f| <= 1/8 revolution first, so none is made.
 __device__ __forceinline__ float2 phasor_rev_hw(double r) {
-  const double j = rint(4.0 * r);
-  const float f = (float)(r - 0.25 * j);
-  const float s = __builtin_amdgcn_sinf(f);
-  const float c = __builtin_amdgcn_cosf(f);
-  const int q = ((int)j) & 3;
-  float2 o;
-  o.x = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
-  o.y = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
-  return o;
+  const float f = (float)r;
+  return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
 }
 
 // e^{i 2 pi r} for r in revolutions (|r| <= 0.5), FP32 result.  Quadrant

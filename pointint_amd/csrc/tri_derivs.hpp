@@ -19,25 +19,11 @@ constexpr double kSpanTaylor = 1.0;
 constexpr float kLaneTaylor = 0.3f;
 constexpr float kSincPoly = 0.5f;
 
+// e^{i 2 pi r} for r in revolutions (|r| <= 0.5) on the hardware
+// sine/cosine (revolution input): max abs error 1.25e-7 on MI355X.
 __device__ __forceinline__ float2 phasor_rev_a(double r) {
-  const double j = rint(4.0 * r);
-  const float x = (float)((r - 0.25 * j) * 6.283185307179586);
-  const float x2 = x * x;
-  float s = x2 * (-1.0f / 39916800.0f) + (1.0f / 362880.0f);
-  s = s * x2 - (1.0f / 5040.0f);
-  s = s * x2 + (1.0f / 120.0f);
-  s = s * x2 - (1.0f / 6.0f);
-  s = (s * x2) * x + x;
-  float c = x2 * (-1.0f / 3628800.0f) + (1.0f / 40320.0f);
-  c = c * x2 - (1.0f / 720.0f);
-  c = c * x2 + (1.0f / 24.0f);
-  c = c * x2 - 0.5f;
-  c = c * x2 + 1.0f;
-  const int q = ((int)j) & 3;
-  float2 o;
-  o.x = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
-  o.y = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
-  return o;
+  const float f = (float)r;
+  return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
 }
 
 __device__ __forceinline__ float2 half_phasor_a(double u) {
