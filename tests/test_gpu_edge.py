@@ -1,0 +1,109 @@
+"""Edge cases of the fast path against the FP64 oracle.
+
+* Few objects (N = 1, 2, 3): every form-factor kernel runs element-split
+  (partial sums in split order, finishing kernels); triangles, disks and
+  Gaussians under affine DoFs, series on.  Gates as everywhere for the fast
+  path: ||A - A64||_w / ||A64||_w < 1e-6, |V - V64| < 1e-5 sqrt(V_ii V_jj).
+* Empty objects inside a fast batch give zero rows and columns and leave the
+  others untouched; zero-area (repeated-vertex) faces contribute nothing;
+  an object far from the origin (|t| = 40) keeps its accuracy (phases in
+  FP64).
+"""
+import numpy as np
+import pytest
+
+import pointint_amd as P
+from pointint_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def wrel(a, b, w):
+    return np.sqrt((w * np.abs(a - b) ** 2).sum() / (w * np.abs(b) ** 2).sum())
+
+
+@pytest.fixture(scope="module")
+def grid(oracle):
+    d, w = oracle.load_lebedev(P.lebedev_path(29))
+    dirs, kmag, wq = oracle.build_kgrid(d, w, 24)
+    return dirs, kmag, wq, P.KGrid(dirs, kmag, wq)
+
+
+def _objects(oracle, rng):
+    V3, F3 = W.icosphere(3)
+    Vm = W.star_perturb(V3, rng, 0.1) * 0.4
+    Pc, Nn, _ = W.sphere_pointcloud(3000)
+    rho = rng.uniform(0.01, 0.02, size=3000)
+    sig = rng.uniform(0.01, 0.02, size=3000)
+    wt = rng.uniform(0.5, 1.5, size=3000) * 1e-3
+    objs = [(P.make_triangle_mesh(Vm, F3), oracle.geom_mesh(Vm, F3)),
+            (P.make_point_cloud(Pc * 0.35, Nn, rho), oracle.geom_disks(Pc * 0.35, Nn, rho)),
+            (P.make_gaussian_splat(Pc * 0.3, Nn, sig, wt),
+             oracle.geom_gaussians(Pc * 0.3, Nn, sig, wt))]
+    dofs = [np.concatenate([rng.uniform(-0.15, 0.15, 3),
+                            (W.random_rotation(rng) @ np.diag(rng.uniform(0.8, 1.2, 3))).ravel()])
+            for _ in objs]
+    return objs, dofs
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_few_objects_split_path(oracle, grid, n):
+    dirs, kmag, wq, kg = grid
+    rng = np.random.default_rng(91)
+    objs, dofs = _objects(oracle, rng)
+    objs, dofs = objs[:n], dofs[:n]
+    J64 = np.stack([oracle.compute_A_grid(og, dirs, kmag, x, oracle.DOF_AFFINE)
+                    for (_, og), x in zip(objs, dofs)], axis=1)
+    V64 = oracle.gram(J64, wq)
+    for i, ((g, _), x) in enumerate(zip(objs, dofs)):
+        A = P.compute_form_factor(g, kg, x, "affine", precision="fast")
+        e = wrel(A, J64[:, i], wq)
+        assert e < 1e-6, (n, i, e)
+    for gram in ("fp64", "fp16x3"):
+        V = P.compute_volume_matrix([g for g, _ in objs], kg, dofs, ["affine"] * n, ff="fast",
+                                    gram=gram)
+        d = np.sqrt(np.outer(np.diag(V64), np.diag(V64)))
+        e = np.max(np.abs(V - V64) / d)
+        print(f"N={n} {gram}: scaled V error {e:.2e}")
+        assert e < 1e-5, (n, gram, e)
+
+
+def test_empty_objects_in_fast_batch(oracle, grid):
+    dirs, kmag, wq, kg = grid
+    V2, F2 = W.icosphere(2)
+    meshes = [(V2 * 0.3 + c, F2) for c in ([0, 0, 0], [0.2, 0.1, 0.0], [0.5, -0.2, 0.1])]
+    empty = P.make_triangle_mesh(np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int32))
+    geoms = [P.make_triangle_mesh(v, f) for v, f in meshes]
+    Vo, _ = oracle.volume_matrix([oracle.geom_mesh(v, f) for v, f in meshes], dirs, kmag, wq)
+    V = P.compute_volume_matrix([geoms[0], empty, geoms[1], empty, geoms[2]], kg, ff="fast",
+                                gram="fp64")
+    keep = [0, 2, 4]
+    assert np.all(V[[1, 3], :] == 0) and np.all(V[:, [1, 3]] == 0)
+    d = np.sqrt(np.outer(np.diag(Vo), np.diag(Vo)))
+    assert np.max(np.abs(V[np.ix_(keep, keep)] - Vo) / d) < 1e-5
+
+
+def test_degenerate_faces_and_far_objects(oracle, grid):
+    dirs, kmag, wq, kg = grid
+    V3, F3 = W.icosphere(3)
+    Vm = V3 * 0.35
+    # append zero-area faces: repeated vertices and a collinear triple
+    nv = len(Vm)
+    extra_v = np.array([[0.0, 0.0, 0.5], [0.0, 0.0, 0.6], [0.0, 0.0, 0.7]])
+    Vd = np.vstack([Vm, extra_v])
+    Fd = np.vstack([F3, [[0, 0, 1], [5, 5, 5], [nv, nv + 1, nv + 2]]]).astype(np.int32)
+    gd = P.make_triangle_mesh(Vd, Fd)
+    A_deg = P.compute_form_factor(gd, kg, precision="fast")
+    A_ref = oracle.compute_A_grid(oracle.geom_mesh(Vm, F3), dirs, kmag)
+    assert wrel(A_deg, A_ref, wq) < 1e-6
+    # the same mesh 40 units from the origin: A picks up only the phase e^{i k.t}
+    t = np.array([40.0, -25.0, 17.0])
+    x = np.concatenate([t, np.eye(3).ravel()])
+    g = P.make_triangle_mesh(Vm, F3)
+    A_far = P.compute_form_factor(g, kg, x, "affine", precision="fast")
+    A64 = oracle.compute_A_grid(oracle.geom_mesh(Vm, F3), dirs, kmag, x, oracle.DOF_AFFINE)
+    e = wrel(A_far, A64, wq)
+    print(f"far object: weighted rel error {e:.2e}")
+    assert e < 1e-6
+    ph = np.exp(1j * (dirs * kmag[:, None]) @ t)
+    assert wrel(A_far, A_ref * ph, wq) < 1e-6
