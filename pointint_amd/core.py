@@ -180,6 +180,7 @@ class Geometry:
 
     def __init__(self, handle, ctx: Context, kind: int, host: dict):
         self._h = handle
+        self._hv = handle.value or 0  # plain int for the batched calls' pointer arrays
         self.ctx = ctx
         self.type = kind
         self.host = host
@@ -306,7 +307,7 @@ def _batch_args(geoms, dofs, dof_kinds):
     would cost more than the GPU step for a few hundred objects).  Returns the
     arrays and the DoF vectors they point into (kept alive by the caller)."""
     n = len(geoms)
-    gh = np.fromiter((g.handle.value for g in geoms), dtype=np.uintp, count=n)
+    gh = np.fromiter((g._hv for g in geoms), dtype=np.uintp, count=n)
     if dofs is None and dof_kinds is None:
         dl = [IDENTITY_AFFINE]
         kh = np.full(n, _DOF["affine"], dtype=np.int32)

@@ -102,14 +102,17 @@ void kgrid_shells(KGridDev& k) {
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 void validate_object(const GeometryDev& g, int dof_kind, int ndofs) {
+  // messages are built only on failure (this runs per object per call)
   if (dof_kind == PII_DOF_AFFINE) {
-    require(ndofs == 12, "AffineDoF requires 12 DoFs, got " + std::to_string(ndofs));
+    if (ndofs != 12)
+      throw Error(PII_ERR_INVALID, "AffineDoF requires 12 DoFs, got " + std::to_string(ndofs));
   } else if (dof_kind == PII_DOF_TRIMESH) {
     if (g.type != PII_GEOM_TRIANGLE)
       throw Error(PII_ERR_UNSUPPORTED, "TriangleMeshDoF supports triangle geometry only");
-    require(ndofs == 3 * g.ndof_verts,
-            "TriangleMeshDoF requires " + std::to_string(3 * g.ndof_verts) + " DoFs, got " +
-                std::to_string(ndofs));
+    if (ndofs != 3 * g.ndof_verts)
+      throw Error(PII_ERR_INVALID, "TriangleMeshDoF requires " +
+                                       std::to_string(3 * g.ndof_verts) + " DoFs, got " +
+                                       std::to_string(ndofs));
   } else {
     throw Error(PII_ERR_UNSUPPORTED, "unknown DoF kind " + std::to_string(dof_kind));
   }
@@ -209,8 +212,8 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   // when at least kSeriesMinShells of them (below that the per-element
   // kernels are cheaper than one pass over the elements per direction).
   // Disks and Gaussians also keep |k|^2 |A|_F^2 rho^2 / 4 <= 3
-  // (sigma^2 / 2 <= 0.49), where ten terms of the J1 (exponential) factor
-  // reach 1e-11.
+  // (sigma^2 / 2 <= 0.49), which bounds the growth the J1 (exponential)
+  // factor adds to the series terms (the series keeps order kSeriesOrder).
   constexpr int kSeriesMinShells = 4, kSurfelMinShells = 8;
   std::vector<int> nser_i(N, 0), nord_i(N, 0);
   std::vector<double> kser_i(N, 0.0);

@@ -1,4 +1,5 @@
-// Low-|k| triangle form factors from a moment series.
+// Low-|k| form factors from moment series (triangles; disks and Gaussians
+// further below).
 //
 // The reference sums, per face f with area vector S_f, corners v_f0..v_f2 and
 // k = kappa d (cpp/src/dof/triangle_mesh_dof.cpp:36-71):
@@ -19,8 +20,9 @@
 // P terms is below Area X^(P+1)/(P+1)! / (1 - X/(P+2)) at X = kappa R:
 // 3.6e-10 Area for P = 56, X = 15.  Each triangle object takes the smallest
 // order of kSeriesOrders that meets that bound at its own largest series
-// kappa R (nord; a kernel per order).  The largest term, about
-// Area e^X / sqrt(2 pi X), keeps FP64 rounding below 4e-11 Area.
+// kappa R (nord; one launch, the order a uniform switch per workgroup).
+// The largest term, about Area e^X / sqrt(2 pi X), keeps FP64 rounding below
+// 4e-11 Area.
 //
 // h_n by the recurrences (all FP64, 4 operations per order and face)
 //   b_n = w1^n,  h_n(w0, w1) = w0 h_{n-1}(w0, w1) + b_n,
@@ -50,6 +52,39 @@ struct SeriesCoef {
   }
 };
 __device__ constexpr SeriesCoef kSeries{};
+
+// Scale factors g_n and recurrence factors K_n of the disk and Gaussian
+// series (see k_series_surfel).
+struct SurfelRecurrence {
+  double gd[P + 1], gg[P + 1];  // g_n: disk, Gaussian
+  double kd[P], kg[P];          // K_n: disk, Gaussian
+  constexpr SurfelRecurrence() : gd(), gg(), kd(), kg() {
+    double dm1 = 1.0, d0 = 1.0, gm1 = 1.0, g0 = 1.0;  // g_{n-1}, g_n
+    for (int n = 0; n <= P; ++n) {
+      gd[n] = d0;
+      gg[n] = g0;
+      if (n == P) break;
+      kd[n] = (2.0 * n + 3.0) * d0 / dm1;
+      kg[n] = g0 / gm1;
+      const double d1 = dm1 / ((n + 1.0) * (n + 3.0)), g1 = gm1 / (n + 1.0);
+      dm1 = d0;
+      d0 = d1;
+      gm1 = g0;
+      g0 = g1;
+    }
+  }
+};
+__device__ constexpr SurfelRecurrence kSurfel{};
+
+// Coefficient of the series sums: triangles 2/(n+2)!, disks and Gaussians
+// their g_n; each times the real factor (-1)^floor(n/2) of i^n.
+enum SeriesKind : int { kKindTri = 0, kKindDisk = 1, kKindGauss = 2 };
+template <int kKind>
+__device__ __forceinline__ double series_coef(int n) {
+  if (kKind == kKindTri) return kSeries.c[n];
+  const double g = kKind == kKindDisk ? kSurfel.gd[n] : kSurfel.gg[n];
+  return ((n / 2) % 2 == 0) ? g : -g;
+}
 
 // Sums the four waves' H_n into wave 0's registers through one LDS buffer,
 // in the fixed order ((H0 + H1) + H2) + H3.
@@ -98,13 +133,13 @@ __device__ __forceinline__ double* split_slot(double* part, int obj, int blk) {
                     kLanes;
 }
 
-// Sums the four waves' H_n ((H1 + H2) + H3) + H0, applies the real factor of i^n
-// (and, for triangles, 2/(n+2)!) and leaves the coefficients in LDS; then each
+// Sums the four waves' H_n ((H1 + H2) + H3) + H0, applies series_coef<kKind>
+// and leaves the coefficients in LDS; then each
 // wave evaluates the series at every fourth of the object's series shells for
 // its 64 directions, reading the coefficients from LDS.  out_mode 0: A (FP64)
 // to Aout[col*ldA + q]; 1: X = sqrt(w) A (FP32) to Xout[col*ldX + q] and the
 // column max |X| to colmax[col].
-template <int PP, bool kTriCoef>
+template <int PP, int kKind>
 __device__ __forceinline__ void reduce_and_evaluate(
     double (&H)[PP + 1], double (&red)[PP + 1][kLanes], int wave, int lane, int j, int jc,
     double dx, double dy, double dz, const ObjDesc& d, const double* __restrict__ krep, int Qh,
@@ -113,8 +148,7 @@ __device__ __forceinline__ void reduce_and_evaluate(
   sum_waves_123<PP>(H, red, wave, lane);
   if (wave == 0)
     for_chunked<PP>([&](int n) {
-      red[n][lane] = (red[n][lane] + H[n]) *
-                     (kTriCoef ? kSeries.c[n] : (((n / 2) % 2 == 0) ? 1.0 : -1.0));
+      red[n][lane] = (red[n][lane] + H[n]) * series_coef<kKind>(n);
     });
   __syncthreads();
   const int nser = d.nser;
@@ -216,8 +250,8 @@ __device__ __forceinline__ void series_tri_body(
     store_split<PP>(H, red, wave, lane, split_slot(part, bo.obj, bo.blk) + lane);
     return;
   }
-  reduce_and_evaluate<PP, true>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
-                                Aout, ldA, Xout, ldX, colmax);
+  reduce_and_evaluate<PP, kKindTri>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh,
+                                    out_mode, Aout, ldA, Xout, ldX, colmax);
 }
 
 // One launch for all triangle objects: each workgroup runs the body of its
@@ -254,26 +288,26 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
 // k = kappa d, with mu = d.(A n), sg = |A^T d|^2 - mu^2 and w = d.(c' - c):
 //   disk:      A_e = area mu J1(x)/x * 2 e^{i kappa d.c'},  x = rho kappa sqrt(sg)
 //   Gaussian:  A_e = wt mu exp(-sigma^2 kappa^2 sg / 2) e^{i kappa d.c'}
-// and with J1(x)/x = sum_m (-1)^m (x/2)^(2m) / (2 m! (m+1)!), (i kappa)^(2m) =
-// (-1)^m kappa^(2m), the coefficient of (i kappa)^n is
-//   H_n = sum_e a_e sum_{2m+j=n} b_m(e) w^j / j!,
-//   disk: a_e = area mu, b_m = u^m / (m! (m+1)!), u = rho^2 max(sg, 0) / 4;
-//   Gaussian: a_e = wt mu, b_m = v^m / m!,  v = sigma^2 sg / 2 (no clamp, as
-//   the reference CPU path).
-// m runs to M_e <= kSurfelTerms, chosen per element (uniform over the wave)
-// from the bound sg <= |A|_F^2 at the object's largest series |k|, so that
-// the first omitted term is below 1e-12.
-constexpr int kSurfelTerms = 10;
-
-struct InvInt {
-  double c[P + 1];
-  constexpr InvInt() : c() {
-    c[0] = 1.0;
-    for (int j = 1; j <= P; ++j) c[j] = 1.0 / (double)j;
-  }
-};
-__device__ constexpr InvInt kInvInt{};
-
+// With J1(x)/x = sum_m (-1)^m (x/2)^(2m) / (2 m! (m+1)!) and (i kappa)^(2m) =
+// (-1)^m kappa^(2m), A_e = a_e e^{i kappa d.c} f_e(i kappa) with
+//   disk:      a_e = area mu, f(z) = e^{w z} sum_m u^m z^(2m) / (m! (m+1)!),
+//              u = rho^2 max(sg, 0) / 4,
+//   Gaussian:  a_e = wt mu,   f(z) = e^{w z + v z^2},  v = sigma^2 sg / 2 (no
+//              clamp, as the reference CPU path),
+// so H_n = sum_e a_e c_n(e) with c_n the Taylor coefficients of f.  They obey
+// three-term recurrences (from the differential equations of f:
+// z f'' + (3 - 2 w z) f' + ((w^2 - 4u) z - 3 w) f = 0 for disks,
+// f' = (w + 2 v z) f for Gaussians):
+//   disk:      (n+1)(n+3) c_{n+1} = (2n+3) w c_n - (w^2 - 4u) c_{n-1}
+//   Gaussian:  (n+1) c_{n+1}      = w c_n + 2 v c_{n-1}
+// With c_n = g_n e_n and g_{n+1} = g_{n-1} / ((n+1)(n+3)) (disk) or
+// g_{n-1} / (n+1) (Gaussian), g_{-1} = g_0 = 1, the coefficient of e_{n-1}
+// becomes the element constant -(w^2 - 4u) or 2v:
+//   e_{n+1} = K_n w e_n + G e_{n-1},  K_n = (2n+3) g_n / g_{n-1} or g_n / g_{n-1},
+// four FP64 operations per order and element (two products, one FMA, the sum
+// into H), and g_n is applied once to the sums.  Every term of c_n has the
+// sign of w^n (u >= 0; v >= 0 but for rounding), so the forward recurrence
+// has no cancellation.
 template <bool kDisk>
 __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
@@ -296,10 +330,6 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
   const double atz = p[5] * dx + p[8] * dy + p[11] * dz;
   const double al = atx * atx + aty * aty + atz * atz;
   const double dc = dx * d.ctr[0] + dy * d.ctr[1] + dz * d.ctr[2];
-  // uniform bound of |k|^2 sg over this object's series shells
-  double an2 = 0.0;
-  for (int i = 3; i < 12; ++i) an2 += p[i] * p[i];
-  const double kb2 = an2 * d.kser * d.kser;
   double H[P + 1];
 #pragma unroll
   for (int n = 0; n <= P; ++n) H[n] = 0.0;
@@ -307,58 +337,50 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_surfel(
   const int chunk = (d.n + (int)gridDim.z - 1) / (int)gridDim.z;
   const int eb = (int)blockIdx.z * chunk;
   const int ee = min(d.n, eb + chunk);
+  // element record: centre c' (0..2), A n (3..5), rho or sigma (6), area or
+  // weight (7); the next element's record is loaded before this one's sums
+  double t[kRecSurfel] = {};
+  if (eb < ee) {
+    const auto* r = rec + (size_t)kRecSurfel * min(eb + wave, ee - 1);
+#pragma unroll
+    for (int i = 0; i < kRecSurfel; ++i) t[i] = r[i];
+  }
   for (int e = eb + wave; e < ee; e += kWaves) {
-    const auto* r = rec + (size_t)kRecSurfel * e;
-    const double mu = dx * r[3] + dy * r[4] + dz * r[5];
-    const double w = dx * r[0] + dy * r[1] + dz * r[2] - dc;
+    const double mu = dx * t[3] + dy * t[4] + dz * t[5];
+    const double w = dx * t[0] + dy * t[1] + dz * t[2] - dc;
     const double sg = al - mu * mu;
-    const double s2 = r[6] * r[6];
-    const double a = r[7] * mu;
-    double u, ub;
-    if (kDisk) {
-      u = 0.25 * s2 * fmax(sg, 0.0);
-      ub = 0.25 * s2 * kb2;
-    } else {
-      u = 0.5 * s2 * sg;
-      ub = 0.5 * s2 * kb2;
-    }
-    // terms of the J1 / exponential factor for this element (uniform)
-    int M = 0;
-    {
-      double t = 1.0;
-      while (M < kSurfelTerms) {
-        t *= kDisk ? ub / ((double)(M + 1) * (double)(M + 2)) : ub / (double)(M + 1);
-        if (t < 1e-12) break;
-        ++M;
-      }
-    }
-    double E[P + 1];
-    E[0] = 1.0;
+    const double s2 = t[6] * t[6];
+    const double a = t[7] * mu;
+    // G = 4u - w^2 (disk), 2v (Gaussian)
+    const double G = kDisk ? fma(s2, fmax(sg, 0.0), -w * w) : s2 * sg;
+    const auto* rn = rec + (size_t)kRecSurfel * min(e + kWaves, ee - 1);
 #pragma unroll
-    for (int jj = 1; jj <= P; ++jj) E[jj] = E[jj - 1] * w * kInvInt.c[jj];
-    double b = a;
+    for (int i = 0; i < kRecSurfel; ++i) t[i] = rn[i];
+    __builtin_amdgcn_sched_barrier(0);
+    double em1 = 0.0, e0 = a;
+    H[0] += e0;
 #pragma unroll
-    for (int m = 0; m <= kSurfelTerms; ++m) {
-      if (m <= M) {
-#pragma unroll
-        for (int jj = 0; jj + 2 * m <= P; ++jj) H[2 * m + jj] = fma(b, E[jj], H[2 * m + jj]);
-        b *= kDisk ? u / ((double)(m + 1) * (double)(m + 2)) : u / (double)(m + 1);
-      }
+    for (int n = 0; n < P; ++n) {
+      const double e1 = fma(e0 * w, kDisk ? kSurfel.kd[n] : kSurfel.kg[n], G * em1);
+      H[n + 1] += e1;
+      em1 = e0;
+      e0 = e1;
     }
   }
   if (part) {
     store_split<P>(H, red, wave, lane, split_slot(part, bo.obj, bo.blk) + lane);
     return;
   }
-  reduce_and_evaluate<P, false>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh, out_mode,
-                                Aout, ldA, Xout, ldX, colmax);
+  reduce_and_evaluate<P, kDisk ? kKindDisk : kKindGauss>(H, red, wave, lane, j, jc, dx, dy, dz,
+                                                         d, krep, Qh, Mh, out_mode, Aout, ldA,
+                                                         Xout, ldX, colmax);
 }
 
 // Finishes a face-split series launch: the four waves each sum every fourth
 // split's H_n for their 64 directions (splits in increasing order), then the
 // usual reduction and evaluation.  type (and, for triangles, the order PP)
 // selects the objects of the launch.
-template <int PP, bool kTriCoef>
+template <int PP, int kKind>
 __device__ __forceinline__ void series_finish_body(
     const ObjDesc& d, BlockObj bo, double* red_lds, const double* __restrict__ krep, int Qh,
     int Mh, int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout,
@@ -379,8 +401,8 @@ __device__ __forceinline__ void series_finish_body(
     const double* o = src + (size_t)z * stride;
     for_chunked<PP>([&](int n) { H[n] += o[n * kLanes + lane]; });
   }
-  reduce_and_evaluate<PP, kTriCoef>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh,
-                                    out_mode, Aout, ldA, Xout, ldX, colmax);
+  reduce_and_evaluate<PP, kKind>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh,
+                                 out_mode, Aout, ldA, Xout, ldX, colmax);
 }
 
 __global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
@@ -393,13 +415,16 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
   if (d.nser <= 0 || d.type != type) return;
   if (type == PII_GEOM_TRIANGLE) {
 #define PII_FIN_BODY(PP)                                                                     \
-  series_finish_body<PP, true>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout, \
-                               ldX, colmax, part, nsplit)
+  series_finish_body<PP, kKindTri>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, \
+                                   Xout, ldX, colmax, part, nsplit)
     PII_SERIES_SWITCH(PII_FIN_BODY)
 #undef PII_FIN_BODY
+  } else if (type == PII_GEOM_DISK) {
+    series_finish_body<P, kKindDisk>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout,
+                                     ldX, colmax, part, nsplit);
   } else {
-    series_finish_body<P, false>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout, ldX,
-                                 colmax, part, nsplit);
+    series_finish_body<P, kKindGauss>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout,
+                                      ldX, colmax, part, nsplit);
   }
 }
 

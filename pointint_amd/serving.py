@@ -39,7 +39,7 @@ class Server:
         """Queue a scene; returns its id, or None when refused by the budget."""
         n = len(geoms)
         d = N.f64(dofs).reshape(n, 12)
-        gh = np.fromiter((g.handle.value for g in geoms), dtype=np.uintp, count=n)
+        gh = np.fromiter((g._hv for g in geoms), dtype=np.uintp, count=n)
         qid = C.c_int64()
         rc = N.load().pii_server_submit(self._h, n, gh.ctypes.data, N.ptr(d), C.byref(qid))
         if rc == N.PII_ERR_BUDGET:
