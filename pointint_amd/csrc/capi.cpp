@@ -192,13 +192,23 @@ static void world_bounds(const GeometryDev& g, int kind, const double* dofs, dou
 
 // Smallest series order of kSeriesOrders whose tail bound at X = |k| R,
 // X^(P+1) / (P+1)! / (1 - X / (P+2)), is below 3.6e-10 (of the area).
+// One pass over m = 1..P+1 serves every candidate order (this runs per object
+// per call).
+struct InverseIntegers {
+  double v[kSeriesOrder + 2];
+  constexpr InverseIntegers() : v() {
+    for (int m = 1; m <= kSeriesOrder + 1; ++m) v[m] = 1.0 / m;
+  }
+};
+static constexpr InverseIntegers kInverse{};
+
 static int series_order(double X) {
+  double t = 1.0;  // X^m / m!
+  int m = 0;
   for (int i = 0; i < kNumSeriesOrders; ++i) {
     const int P = kSeriesOrders[i];
-    if (X >= P + 2) continue;
-    double t = 1.0;  // X^(P+1) / (P+1)!
-    for (int m = 1; m <= P + 1; ++m) t *= X / m;
-    if (t / (1.0 - X / (P + 2)) < 3.6e-10) return P;
+    for (; m < P + 1; ++m) t *= X * kInverse.v[m + 1];
+    if (X < P + 2 && t / (1.0 - X / (P + 2)) < 3.6e-10) return P;
   }
   return kSeriesOrder;
 }

@@ -28,6 +28,7 @@
 // the 64 nodes, then each wave streams a quarter of the patch's face records
 // (wave-uniform, scalar loads) and reads its three phasors from LDS.
 #include "device_common.hpp"
+#include "j1_modphase.hpp"
 #include "pii_internal.hpp"
 
 namespace pii {
@@ -53,38 +54,10 @@ __device__ __forceinline__ float2 phasor_rev_hw(double r) {
   return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
 }
 
-// e^{i 2 pi r} for r in revolutions (|r| <= 0.5), FP32 result.  Quadrant
-// reduction in FP64, polynomials in FP32 on |x| <= pi/4 (Taylor to x^11 / x^10:
-// truncation < 1e-9, far below FP32 rounding).
-__device__ __forceinline__ float2 phasor_rev(double r) {
-  const double j = rint(4.0 * r);
-  const float x = (float)((r - 0.25 * j) * 6.283185307179586);
-  const float x2 = x * x;
-  float s = x2 * (-1.0f / 39916800.0f) + (1.0f / 362880.0f);
-  s = s * x2 - (1.0f / 5040.0f);
-  s = s * x2 + (1.0f / 120.0f);
-  s = s * x2 - (1.0f / 6.0f);
-  s = (s * x2) * x + x;
-  float c = x2 * (-1.0f / 3628800.0f) + (1.0f / 40320.0f);
-  c = c * x2 - (1.0f / 720.0f);
-  c = c * x2 + (1.0f / 24.0f);
-  c = c * x2 - 0.5f;
-  c = c * x2 + 1.0f;
-  const int q = ((int)j) & 3;
-  float2 o;
-  o.x = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
-  o.y = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
-  return o;
-}
-
 // e^{i u/2} (half phasor) and e^{i u} (full phasor) of an FP64 phase.
 __device__ __forceinline__ float2 half_phasor(double u) {
   const double t = u * 0.07957747154594767;  // 1/(4 pi)
   return phasor_rev_hw(t - rint(t));
-}
-__device__ __forceinline__ float2 full_phasor(double u) {
-  const double t = u * 0.15915494309189535;  // 1/(2 pi)
-  return phasor_rev(t - rint(t));
 }
 __device__ __forceinline__ float2 full_phasor_hw(double u) {
   const double t = u * 0.15915494309189535;  // 1/(2 pi)
@@ -190,40 +163,31 @@ __device__ __forceinline__ float2 tri_term(double al, double be, float g2, float
   return tri_main(al, be, g2, Q0, Q1, Q2);
 }
 
-// Coefficients (-1)^m / (2 m! (m+1)!) of J1(x)/x in powers of y = x^2/4.
-struct J1Series {
-  double c[25];
-  constexpr J1Series() : c() {
-    double t = 0.5;
-    for (int m = 0; m < 25; ++m) {
-      c[m] = t;
-      t = -t / ((double)(m + 1) * (double)(m + 2));
-    }
-  }
-};
-__device__ constexpr J1Series kJ1Series{};
-
-// J1(x)/x for x >= 0 in FP64: the power series (Horner, 25 terms, truncation
-// < 1e-13 for x < 12) below 12, Hankel asymptotics above (6 terms each of P
-// and Q: relative error < 1e-9 at x = 12).
-__device__ __forceinline__ double j1ox_fast(double x) {
-  if (x < 12.0) {
-    const double y = 0.25 * x * x;
-    double s = kJ1Series.c[24];
+// J1(x)/x for x >= 4 in FP32 from the modulus and phase of J1 + i Y1:
+// J1(x) = m(y) sqrt(y) cos(x - 3 pi/4 + 2 pi p(y)), y = 1/x, with m and p the
+// degree-7 fits of j1_modphase.hpp (m relative 2.3e-9, p 6e-11 revolutions).
+// The phase in revolutions, x/(2 pi) - 3/8 + p, takes x/(2 pi) as the FP32
+// product plus its rounding error (an FMA) plus x times the low part of
+// 1/(2 pi): after the reduction to [-1/2, 1/2] it carries only the rounding
+// of that last sum (6e-8 revolutions), whatever the size of x.  Against
+// scipy.special.j1 in an FP32 emulation: 3.6e-8 absolute on [4, 400].
+__device__ __forceinline__ float j1ox_large(float x) {
+  constexpr float kInv2PiHi = 0.15915493667125702f;  // fl32(1/(2 pi))
+  constexpr float kInv2PiLo = 6.420638327e-09f;      // 1/(2 pi) - kInv2PiHi
+  const float y = __builtin_amdgcn_rcpf(x);
+  const float sy = __builtin_amdgcn_rsqf(x);           // sqrt(y)
+  float m = kJ1Mod[kJ1ModPhaseDeg];
+  float p = kJ1Phase[kJ1ModPhaseDeg];
 #pragma unroll
-    for (int m = 23; m >= 0; --m) s = fma(s, y, kJ1Series.c[m]);
-    return s;
+  for (int i = kJ1ModPhaseDeg - 1; i >= 0; --i) {
+    m = fmaf(m, y, kJ1Mod[i]);
+    p = fmaf(p, y, kJ1Phase[i]);
   }
-  const double inv = 1.0 / x;
-  const double i2 = inv * inv;
-  // P = 1 - a2/x^2 + a4/x^4 - ..., Q = a1/x - a3/x^3 + ...  (nu = 1)
-  const double P = 1.0 + i2 * (0.1171875 + i2 * (-0.144195556640625 + i2 * (0.6765925884246826 +
-                   i2 * (-6.883914268109947 + i2 * (121.59789187653587 + i2 * -3302.2722944808525)))));
-  const double Qs = inv * (0.375 + i2 * (-0.1025390625 + i2 * (0.2775764465332031 +
-                    i2 * (-1.993531733751297 + i2 * (27.24882731126854 + i2 * -603.8440767050702)))));
-  const float2 ph = full_phasor(x - 2.356194490192345);  // chi = x - 3 pi/4
-  const double amp = sqrt(0.6366197723675814 * inv);
-  return amp * (P * (double)ph.x - Qs * (double)ph.y) * inv;
+  const float r1 = x * kInv2PiHi;
+  const float e1 = fmaf(x, kInv2PiHi, -r1);  // rounding error of r1
+  float f = (r1 - rintf(r1)) + (fmaf(x, kInv2PiLo, e1) + (p - 0.375f));
+  f -= rintf(f);
+  return m * (y * sy) * __builtin_amdgcn_cosf(f);
 }
 
 __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ objs,
@@ -363,7 +327,7 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
         } else if (x < 4.0f) {
           j = j1ox_small(x);
         } else {
-          j = (float)j1ox_fast((double)x);
+          j = j1ox_large(x);
         }
         S = 2.0f * (float)r[7] * j;
       } else {

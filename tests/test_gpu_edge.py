@@ -107,3 +107,36 @@ def test_degenerate_faces_and_far_objects(oracle, grid):
     assert e < 1e-6
     ph = np.exp(1j * (dirs * kmag[:, None]) @ t)
     assert wrel(A_far, A_ref * ph, wq) < 1e-6
+
+
+def test_disk_bessel_large_arguments(oracle):
+    """Disks with x = rho |k_perp| from 0 to about 170 over the grid (radii
+    up to 0.15, |k| up to 1,036): per node |A - A64| <= 5e-7 of the total
+    area (2 area |dJ1/x| per disk, the FP32 Bessel factor good to ~1e-7
+    absolute on both of its branches, x < 4 and x >= 4); with and without
+    the moment series."""
+    d, w = oracle.load_lebedev(P.lebedev_path(29))
+    dirs, kmag, wq = oracle.build_kgrid(d, w, 48)
+    kg = P.KGrid(dirs, kmag, wq)
+    rng = np.random.default_rng(17)
+    n = 400
+    Pc, Nn, _ = W.sphere_pointcloud(n)
+    Pc = Pc * 0.3
+    rho = rng.uniform(0.01, 0.15, size=n)
+    x = np.concatenate([[0.05, -0.1, 0.2], (W.random_rotation(rng) @ np.diag([1.1, 0.9, 1.0])).ravel()])
+    g = P.make_point_cloud(Pc, Nn, rho)
+    A64 = oracle.compute_A_grid(oracle.geom_disks(Pc, Nn, rho), dirs, kmag, x, oracle.DOF_AFFINE)
+    area = (np.pi * rho * rho).sum()
+    xmax = rho.max() * kmag.max() * 1.1
+    assert xmax > 100, xmax
+    ctx = P.Context.default()
+    for series in (True, False):
+        ctx.set_series(series)
+        try:
+            A = P.compute_form_factor(g, kg, x, "affine", precision="fast")
+        finally:
+            ctx.set_series(True)
+        e = np.abs(A - A64).max() / area
+        print(f"series={series}: max|dA| / area {e:.2e}, x up to {xmax:.0f}")
+        assert e < 5e-7, (series, e)
+        assert wrel(A, A64, wq) < 1e-6
