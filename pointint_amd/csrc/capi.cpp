@@ -345,6 +345,19 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     d.qbeg = nser[slot] * Mh;
     d.kser = kser[slot];
     for (int k = 0; k < 3; ++k) d.ctr[k] = ctr[slot][k];
+    // derivative series: affine triangles away from a singular A (its
+    // reconstruction divides by |A^T d|^2 and |cof(A)^T d|^2)
+    const bool aff_tri = nser[slot] > 0 && g.type == PII_GEOM_TRIANGLE &&
+                         kinds[i] == PII_DOF_AFFINE;
+    for (int k = 0; k < 3; ++k) d.rctr[k] = aff_tri ? g.ref_c[k] : d.ctr[k];
+    if (aff_tri) {
+      const double* A = dofs[i] + 3;
+      const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                         A[2] * (A[3] * A[7] - A[4] * A[6]);
+      double fro2 = 0.0;
+      for (int k = 0; k < 9; ++k) fro2 += A[k] * A[k];
+      d.dser = std::abs(det) >= 1e-6 * std::pow(fro2 / 3.0, 1.5) ? 1 : 0;
+    }
     d.mrec = (nser[slot] && g.type == PII_GEOM_TRIANGLE) ? w.mrec.as<double>() + moff[slot]
                                                           : nullptr;
   }

@@ -20,8 +20,11 @@ struct ObjDesc {
   // Low-|k| moment series (triangles on a shell-structured k-grid): the first
   // nser shells come from the series (of order nord for triangles),
   // representative nodes q >= qbeg = nser*Mh from the per-face kernel.
-  int nser, qbeg, nord, pad1;
+  // dser: the affine derivative columns of those shells also come from a
+  // series (affine triangle objects with a well-conditioned A).
+  int nser, qbeg, nord, dser;
   double ctr[3];       // expansion centre of the series (world frame)
+  double rctr[3];      // the same centre in the reference frame (ctr = A rctr + t)
   double kser;         // largest |k| of the object's series shells
   const double* verts;        // nv*3 reference-frame vertices
   const int32_t* faces;       // n*3

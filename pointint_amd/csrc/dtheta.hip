@@ -28,6 +28,7 @@
 #include "device_common.hpp"
 #include "pii_internal.hpp"
 #include "tri_derivs.hpp"
+#include "dtheta_columns.hpp"
 
 namespace pii {
 
@@ -140,37 +141,16 @@ __device__ __forceinline__ void j1ox_and_prime(double x, double& f, double& fp) 
   fp = (J0 - 2.0 * J1 * inv) * inv;
 }
 
-// The 12 columns X_t = sqrt(w) dA/dtheta_t of node q from the sums v = {A,
-// dA/dA_rc (row-major)} (re, im interleaved): dA/dt_t = i k_t A.  Called by
-// whole waves (the column maxima are wave reductions).
-__device__ __forceinline__ void write_columns(const double (&v)[20], int q, int Qh, double kx,
-                                              double ky, double kz, const double* krep, int col,
-                                              float2* __restrict__ Xt, int ldX,
-                                              unsigned* __restrict__ colmax, int lane) {
-  const int qc = q < Qh ? q : Qh - 1;
-  const double sw = (q < Qh) ? sqrt(krep[7 * (size_t)Qh + qc]) : 0.0;
-  const double kv[3] = {kx, ky, kz};
-  float2* X = Xt + (size_t)(12 * col) * ldX;
-  for (int t = 0; t < 12; ++t) {
-    double re, im;
-    if (t < 3) {  // i k_t A
-      re = -kv[t] * v[1];
-      im = kv[t] * v[0];
-    } else {
-      re = v[2 + 2 * (t - 3)];
-      im = v[3 + 2 * (t - 3)];
-    }
-    const float2 x = make_float2((float)(re * sw), (float)(im * sw));
-    if (q < ldX) X[(size_t)t * ldX + q] = (q < Qh) ? x : make_float2(0.f, 0.f);
-    float mval = (q < Qh) ? fmaxf(fabsf(x.x), fabsf(x.y)) : 0.0f;
-    for (int off = 32; off > 0; off >>= 1) mval = fmaxf(mval, __shfl_xor(mval, off));
-    if (lane == 0) atomicMax(colmax + 12 * col + t, __float_as_uint(mval));
-  }
+// First node of the per-element derivative kernels: the derivative series
+// covers the nodes below qbeg of its objects.
+__device__ __forceinline__ int dtheta_first_node(const ObjDesc& d) {
+  return (d.type == PII_GEOM_TRIANGLE && d.dser) ? d.qbeg : 0;
 }
 
-// One workgroup per (object, 64 nodes[, element split]).  Split launches
-// store the 20 sums per node to part[((z * nobj + obj) * nblk * 64 + node) *
-// 20 + i] for k_dtheta_finish.
+// One workgroup per (object, 64 nodes from dtheta_first_node[, element
+// split]).  Split launches store the 20 sums per node to part[((z * nobj +
+// obj) * nblk * 64 + node) * 20 + i] (node counted from the first node) for
+// k_dtheta_finish.
 __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh,
     const float* __restrict__ drec, const long long* __restrict__ doff, float2* __restrict__ Xt,
@@ -180,7 +160,10 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
   const ObjDesc& d = objs[bo.obj];
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int q = bo.blk * kLanes + lane;
+  // objects with a derivative series start after its shells (uniform exit)
+  const int q0 = dtheta_first_node(d) + bo.blk * kLanes;
+  if (q0 >= ldX) return;
+  const int q = q0 + lane;
   const int qc = q < Qh ? q : Qh - 1;
   const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
   const float hx = (float)krep[3 * (size_t)Qh + qc], hy = (float)krep[4 * (size_t)Qh + qc],
@@ -364,7 +347,9 @@ __global__ void __launch_bounds__(kLanes) k_dtheta_finish(
     const double* __restrict__ part, int nsplit) {
   const int obj = blockIdx.y, blk = blockIdx.x, lane = threadIdx.x;
   const ObjDesc& d = objs[obj];
-  const int q = blk * kLanes + lane;
+  const int q0 = dtheta_first_node(d) + blk * kLanes;
+  if (q0 >= ldX) return;
+  const int q = q0 + lane;
   const int qc = q < Qh ? q : Qh - 1;
   const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
   double v[20];

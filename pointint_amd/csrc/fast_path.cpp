@@ -165,19 +165,58 @@ static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_
   PII_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(long long) * N, hipMemcpyHostToDevice, s));
   launch_dtheta_records(s, b.d_objs, b.d_eprefix, N, b.total_e, d_drec, d_off);
   PII_HIP(hipGetLastError());
-  // two workgroups per CU at this kernel's register and LDS use: split
-  // few-object launches to about two rounds of workgroups
-  int max_patches = 0, max_surf = 0;
+  // Derivative series (objects with dser): records, offsets of their records
+  // and shell values, then the three kernels.
+  std::vector<long long> ds_off(2 * (size_t)N, 0);  // record offsets, value offsets
+  size_t ds_rec = 0, ds_val = 0;
+  int max_qbeg = 0, max_dfaces = 0;
+  for (int slot = 0; slot < N; ++slot) {
+    const ObjDesc& d = b.h_objs[slot];
+    if (!d.dser) continue;
+    ds_off[slot] = (long long)ds_rec;
+    ds_off[N + slot] = (long long)ds_val;
+    ds_rec += align_up(dseries_record_doubles(d.n), 32);
+    ds_val += dseries_value_count(d.qbeg);
+    max_qbeg = std::max(max_qbeg, d.qbeg);
+    max_dfaces = std::max(max_dfaces, d.n);
+  }
+  const bool dseries = max_qbeg > 0 && b.Mh > 0;
+  int ds_split = 1;
+  if (dseries) {
+    const int dirblk = (b.Mh + 63) / 64;
+    ds_split = split_count(dirblk * N * 5, max_dfaces, 256, 1024);
+    const size_t rec_b = align_up(sizeof(double) * ds_rec, 256);
+    c.w.dsrec.ensure(rec_b + sizeof(long long) * 2 * N);
+    c.w.dspart.ensure(sizeof(double) * dseries_part_doubles(N, b.Mh, ds_split));
+    c.w.dsval.ensure(sizeof(double2) * ds_val);
+    PII_HIP(hipMemcpyAsync(static_cast<char*>(c.w.dsrec.p) + rec_b, ds_off.data(),
+                           sizeof(long long) * 2 * N, hipMemcpyHostToDevice, s));
+  }
+  // per-element kernel: two workgroups per CU at its register and LDS use;
+  // split few-object launches to about two rounds of the workgroups that have
+  // nodes (objects with a derivative series start at qbeg)
+  int max_patches = 0, max_surf = 0, base = 0;
   for (const ObjDesc& d : b.h_objs) {
     if (d.type == PII_GEOM_TRIANGLE) max_patches = std::max(max_patches, d.npatch);
     else max_surf = std::max(max_surf, d.n);
+    base += (ldX - (d.type == PII_GEOM_TRIANGLE && d.dser ? d.qbeg : 0) + 63) / 64;
   }
-  const int base = ((ldX + 63) / 64) * N;
   const int nsplit = std::max(split_count(base, max_patches, 2, 1024),
                               split_count(base, max_surf, 256, 1024));
   const size_t need = dtheta_split_doubles(N, ldX, nsplit);
   if (need) c.w.split.ensure(sizeof(double) * need);
   c.phase_begin(3);
+  if (dseries) {
+    const size_t rec_b = align_up(sizeof(double) * ds_rec, 256);
+    const long long* d_dsoff =
+        reinterpret_cast<const long long*>(static_cast<char*>(c.w.dsrec.p) + rec_b);
+    launch_dseries_records(s, b.d_objs, b.d_eprefix, N, b.total_e, c.w.dsrec.as<double>(),
+                           d_dsoff);
+    launch_dseries(s, b.d_objs, N, kg.rep.as<double>(), Qh, b.Mh, c.w.dsrec.as<double>(), d_dsoff,
+                   c.w.dspart.as<double>(), ds_split, c.w.dsval.as<double2>(), d_dsoff + N,
+                   max_qbeg, X + (size_t)na * ldX, ldX, colmax + na);
+    PII_HIP(hipGetLastError());
+  }
   launch_dtheta_affine(s, b.d_objs, N, kg.rep.as<double>(), Qh, d_drec, d_off,
                        X + (size_t)na * ldX, ldX, colmax + na, nsplit,
                        need ? c.w.split.as<double>() : nullptr);

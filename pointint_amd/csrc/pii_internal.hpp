@@ -134,6 +134,7 @@ struct GeometryDev {
 struct Workspace {
   DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw, mrec, gam;
   DevBuf dth, Xt, cmax2, H;  // Gauss-Newton Hessian: records, dA/dtheta columns, maxima, H
+  DevBuf dsrec, dspart, dsval;  // derivative series: face records, family sums, shell values
   DevBuf split;              // partial sums of element-split launches (few objects)
 };
 
@@ -229,6 +230,19 @@ void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const dou
                           int Qh, const float* d_drec, const long long* d_doff, float2* d_Xt,
                           int ldX, unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
 size_t dtheta_split_doubles(int N, int ldX, int nsplit);
+// Derivative series of affine triangle objects (ObjDesc::dser): per-face
+// reference records, the five families of per-direction sums and the 12
+// columns on the series shells (nodes q < qbeg); the per-element kernels then
+// start at qbeg for those objects.
+void launch_dseries_records(hipStream_t s, const ObjDesc* d_objs, const int* d_eprefix, int N,
+                            int total_e, double* d_out, const long long* d_off);
+size_t dseries_record_doubles(int nfaces);
+size_t dseries_part_doubles(int N, int Mh, int nsplit);
+size_t dseries_value_count(int qbeg);
+void launch_dseries(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
+                    int Mh, const double* d_srec, const long long* d_soff, double* d_part,
+                    int nsplit, double2* d_sval, const long long* d_voff, int max_qbeg,
+                    float2* d_Xt, int ldX, unsigned* d_colmax);
 // Sustained issue rate of one probe launch (blocks x 256 threads, iters
 // rounds): kind 0 FP64 FMA flop/s, 1 FP32 FMA flop/s, 2 FP32 hardware sines/s.
 // d_sink: blocks doubles (written only under a condition that never holds).
