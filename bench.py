@@ -43,11 +43,10 @@ FLOP_PER_SERIES_NODE = 2.0 * SERIES_ORDER + 12.0
 # Per disk x k-node of the per-element kernel: about 25 FMA-class operations
 # (50 flop), J1(x)/x (about 15 FMA, 30 flop), one sincos, one sqrt, one
 # reciprocal.  Per disk x direction of the series: 2 dot products and the
-# factors (15 flop), the w^j/j! sequence (2 x 56) and, for a nominal 6 terms
-# of the J1 factor, sum_{m<6} (57 - 2m) FMAs (624 flop).
+# element factors (15 flop), then per order the three-term recurrence (two
+# multiplies, one FMA) and the sum into H_n: 5 flop x 56.
 FLOP_PER_DISK_K = 83.0
-FLOP_PER_SURFEL_DIR = 15.0 + 2.0 * SERIES_ORDER + 2.0 * sum(SERIES_ORDER + 1 - 2 * m
-                                                           for m in range(6))
+FLOP_PER_SURFEL_DIR = 15.0 + 5.0 * SERIES_ORDER
 PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 (= non-packed FP32) peak
 # HBM bytes per launch of the dominant kernel on the default workload, from
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB device copy
