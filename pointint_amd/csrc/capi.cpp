@@ -41,6 +41,27 @@ void DevBuf::ensure(size_t n) {
   bytes = want;
 }
 
+void HostBuf::release() {
+  if (p) {
+    (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+}
+
+void HostBuf::ensure(size_t n) {
+  if (n <= bytes && p) return;
+  release();
+  if (n == 0) return;
+  const size_t want = std::max<size_t>(n, 4096);
+  if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    throw Error(PII_ERR_NOMEM, "hipHostMalloc of " + std::to_string(want) + " bytes failed");
+  }
+  bytes = want;
+}
+
 void Context::phase_begin(int p) {
   if (!profiling) return;
   if (!ev[p][0]) {
@@ -162,6 +183,28 @@ static void world_bounds(const GeometryDev& g, int kind, const double* dofs, dou
     return;
   }
   double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  if (g.dof_identity) {
+    // indexed mesh: the DoF vector holds every vertex once, contiguously
+    for (int v = 0; v < g.nv; ++v)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], dofs[3 * (size_t)v + k]);
+        hi[k] = std::max(hi[k], dofs[3 * (size_t)v + k]);
+      }
+    if (!(lo[0] <= hi[0])) {
+      c[0] = c[1] = c[2] = 0.0;
+      R = 0.0;
+      return;
+    }
+    for (int k = 0; k < 3; ++k) c[k] = 0.5 * (lo[k] + hi[k]);
+    double r2 = 0.0;
+    for (int v = 0; v < g.nv; ++v) {
+      const double x = dofs[3 * (size_t)v] - c[0], y = dofs[3 * (size_t)v + 1] - c[1],
+                   z = dofs[3 * (size_t)v + 2] - c[2];
+      r2 = std::max(r2, x * x + y * y + z * z);
+    }
+    R = std::sqrt(r2) * (1.0 + 1e-12);
+    return;
+  }
   for (int v = 0; v < g.nv; ++v) {
     const int dv = g.h_dofvid[v];
     if (dv < 0) continue;  // only in zero-area faces
@@ -268,17 +311,28 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   order.resize(N);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-  // DoF vectors, concatenated.
+  // Descriptors, prefix sums and the concatenated DoF vectors share one
+  // pinned staging area and one device buffer: a single DMA upload per call.
   std::vector<size_t> dof_off(N);
   size_t total_dofs = 0;
   for (int i = 0; i < N; ++i) {
     dof_off[i] = total_dofs;
     total_dofs += align_up((size_t)ndofs[i], 32);
   }
-  h_dofs.assign(total_dofs, 0.0);
-  for (int i = 0; i < N; ++i)
-    std::memcpy(h_dofs.data() + dof_off[i], dofs[i], sizeof(double) * ndofs[i]);
-  upload(w.dofs, h_dofs.data(), total_dofs, s);
+  const size_t obj_bytes = align_up(sizeof(ObjDesc) * N, 256);
+  const size_t pre_bytes = align_up(sizeof(int) * (N + 1), 256);
+  const size_t dofs_at = obj_bytes + 2 * pre_bytes;
+  const size_t stage_bytes = dofs_at + sizeof(double) * std::max<size_t>(total_dofs, 1);
+  w.stage.ensure(stage_bytes);
+  w.misc.ensure(stage_bytes);
+  char* hs = w.stage.as<char>();
+  double* h_dofs = reinterpret_cast<double*>(hs + dofs_at);
+  for (int i = 0; i < N; ++i) {
+    std::memcpy(h_dofs + dof_off[i], dofs[i], sizeof(double) * ndofs[i]);
+    const size_t end = (i + 1 < N) ? dof_off[i + 1] : total_dofs;
+    std::fill(h_dofs + dof_off[i] + ndofs[i], h_dofs + end, 0.0);
+  }
+  double* d_dofs = reinterpret_cast<double*>(static_cast<char*>(w.misc.p) + dofs_at);
   // World vertices and element records.
   std::vector<size_t> wv_off(N), rec_off(N);
   size_t total_w = 0, total_rec = 0;
@@ -337,7 +391,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     d.patch_faces = g.patch_faces.as<uint32_t>();
     d.patch_floc = g.patch_floc.as<uint32_t>();
     d.el = g.el.as<double>();
-    d.dofs = w.dofs.as<double>() + dof_off[i];
+    d.dofs = d_dofs + dof_off[i];
     d.world = w.world.as<double>() + wv_off[slot];
     d.rec = w.faces.as<double>() + rec_off[slot];
     d.nser = nser[slot];
@@ -366,14 +420,10 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     qbeg_min = h_objs[0].qbeg;
     for (int slot = 1; slot < N; ++slot) qbeg_min = std::min(qbeg_min, h_objs[slot].qbeg);
   }
-  // Descriptor table + prefix sums in one upload.
-  const size_t obj_bytes = align_up(sizeof(ObjDesc) * N, 256);
-  const size_t pre_bytes = align_up(sizeof(int) * (N + 1), 256);
-  h_blob.assign(obj_bytes + 2 * pre_bytes, 0);
-  std::memcpy(h_blob.data(), h_objs.data(), sizeof(ObjDesc) * N);
-  std::memcpy(h_blob.data() + obj_bytes, h_vprefix.data(), sizeof(int) * (N + 1));
-  std::memcpy(h_blob.data() + obj_bytes + pre_bytes, h_eprefix.data(), sizeof(int) * (N + 1));
-  upload(w.misc, h_blob.data(), h_blob.size(), s);
+  std::memcpy(hs, h_objs.data(), sizeof(ObjDesc) * N);
+  std::memcpy(hs + obj_bytes, h_vprefix.data(), sizeof(int) * (N + 1));
+  std::memcpy(hs + obj_bytes + pre_bytes, h_eprefix.data(), sizeof(int) * (N + 1));
+  PII_HIP(hipMemcpyAsync(w.misc.p, hs, stage_bytes, hipMemcpyHostToDevice, s));
   d_objs = reinterpret_cast<const ObjDesc*>(w.misc.p);
   d_vprefix = reinterpret_cast<const int*>(static_cast<char*>(w.misc.p) + obj_bytes);
   d_eprefix = reinterpret_cast<const int*>(static_cast<char*>(w.misc.p) + obj_bytes + pre_bytes);
@@ -648,6 +698,7 @@ int pii_geometry_create_mesh(pii_context* ctx, const double* V, int nv, const in
     d.h_dofvid.resize(nv);
     std::iota(d.h_dofvid.begin(), d.h_dofvid.end(), 0);
     d.ndof_verts = nv;
+    d.dof_identity = true;
     finish_triangles(ctx, d);
     *out = g.release();
   });

@@ -1,6 +1,7 @@
 // Fast-path drivers: form factors over the representative k-nodes (one per
 // antipodal pair when the grid is symmetric), then the Gram.
 #include <algorithm>
+#include <cstring>
 
 #include "pii_capi_internal.hpp"
 
@@ -335,8 +336,10 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
                      c.w.B.as<float2>(), ldX, ksplit, d_gw, d_goff);
   c.phase_end(3);
   PII_HIP(hipGetLastError());
-  std::vector<double> gw(std::max(goff[N], 1));
-  PII_HIP(hipMemcpyAsync(gw.data(), d_gw, sizeof(double) * goff[N], hipMemcpyDeviceToHost, s));
+  // world-vertex gradients come back through pinned staging (one DMA copy)
+  c.w.hout.ensure(sizeof(double) * std::max(goff[N], 1));
+  const double* gw = c.w.hout.as<double>();
+  PII_HIP(hipMemcpyAsync(c.w.hout.p, d_gw, sizeof(double) * goff[N], hipMemcpyDeviceToHost, s));
   if (V_host)
     PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N,
                            hipMemcpyDeviceToHost, s));
@@ -346,9 +349,11 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
     const int i = b.order[slot];
     const GeometryDev& g = *geoms[i];
     const ObjDesc& d = b.h_objs[slot];
-    const double* gv = gw.data() + goff[slot];
+    const double* gv = gw + goff[slot];
     double* out = grads[i];
-    if (d.dof_kind == PII_DOF_TRIMESH) {
+    if (d.dof_kind == PII_DOF_TRIMESH && g.dof_identity) {
+      std::memcpy(out, gv, sizeof(double) * 3 * (size_t)g.nv);
+    } else if (d.dof_kind == PII_DOF_TRIMESH) {
       std::fill(out, out + 3 * g.ndof_verts, 0.0);
       for (int v = 0; v < g.nv; ++v) {
         const int dv = g.h_dofvid[v];

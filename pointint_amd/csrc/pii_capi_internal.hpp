@@ -53,8 +53,6 @@ struct Batch {
   std::vector<int> order;         // slot -> caller index (largest first)
   std::vector<ObjDesc> h_objs;    // per slot
   std::vector<int> h_vprefix, h_eprefix;
-  std::vector<double> h_dofs;
-  std::vector<char> h_blob;
   const ObjDesc* d_objs = nullptr;
   const int* d_vprefix = nullptr;
   const int* d_eprefix = nullptr;

@@ -116,6 +116,7 @@ struct GeometryDev {
   double ref_R = 0.0;
   double ref_smax = 0.0;
   int ndof_verts = 0;                // DoF vertex count for TRIMESH DoFs
+  bool dof_identity = false;         // DoF vertex v is geometry vertex v (indexed meshes)
   DevBuf verts;                      // nv*3 FP64 (AoS)
   DevBuf faces;                      // n*3 int32
   DevBuf dofvid;                     // nv int32
@@ -130,12 +131,30 @@ struct GeometryDev {
   DevBuf el;
 };
 
+// Pinned host staging (hipHostMalloc), grow-only: the upload of a call's DoFs
+// and descriptors is one DMA copy from here.  Each workspace (and so each
+// stream) has its own; a call rewrites it only after its stream finished the
+// previous call's copy.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() { release(); }
+  void release();
+  void ensure(size_t n);
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
 // Per-call device workspaces (grow-only, reused across calls on one stream).
 struct Workspace {
-  DevBuf world, faces, J, X, part, V, misc, dofs, colmax, B, C, gw, mrec, gam;
+  DevBuf world, faces, J, X, part, V, misc, colmax, B, C, gw, mrec, gam;
   DevBuf dth, Xt, cmax2, H;  // Gauss-Newton Hessian: records, dA/dtheta columns, maxima, H
   DevBuf dsrec, dspart, dsval;  // derivative series: face records, family sums, shell values
   DevBuf split;              // partial sums of element-split launches (few objects)
+  HostBuf stage;             // pinned: descriptors, prefix sums and DoFs of a call
+  HostBuf hout;              // pinned: per-vertex gradients coming back
 };
 
 struct Context {
