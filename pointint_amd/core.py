@@ -344,7 +344,7 @@ def volume_matrix_vjp(geoms: Sequence[Geometry], kgrid: KGrid, dL_dV, dofs=None,
     ctx = geoms[0].ctx
     kinds, dl = _dof_args(geoms, dofs, dof_kinds)
     G = N.f64(dL_dV).reshape(n, n)
-    grads = [np.zeros(d.shape[0]) for d in dl]
+    grads = [np.empty(d.shape[0]) for d in dl]  # every entry is written by the call
     gh = (C.c_void_p * n)(*[g._hv for g in geoms])
     kh = (C.c_int * n)(*kinds)
     dh = (C.c_void_p * n)(*[d.ctypes.data for d in dl])

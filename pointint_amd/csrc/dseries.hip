@@ -392,6 +392,11 @@ void launch_dseries_records(hipStream_t s, const ObjDesc* d_objs, const int* d_e
 
 size_t dseries_record_doubles(int nfaces) { return (size_t)kRecDser * nfaces; }
 
+int dseries_slots() {
+  static std::atomic<int> cache[16];
+  return resident_workgroups(k_dseries_tri, kLanes * kWaves, cache);
+}
+
 size_t dseries_part_doubles(int N, int Mh, int nsplit) {
   return (size_t)kFamilies * nsplit * N * ((Mh + kLanes - 1) / kLanes) * (P + 1) * kLanes;
 }

@@ -373,6 +373,11 @@ void launch_dtheta_records(hipStream_t s, const ObjDesc* d_objs, const int* d_ep
                      d_eprefix, N, total_e, d_out, d_off);
 }
 
+int dtheta_slots() {
+  static std::atomic<int> cache[16];
+  return resident_workgroups(k_dtheta_affine, kLanes * kWaves, cache);
+}
+
 size_t dtheta_split_doubles(int N, int ldX, int nsplit) {
   return nsplit > 1 ? (size_t)nsplit * N * ((ldX + kLanes - 1) / kLanes) * kLanes * 20 : 0;
 }

@@ -1,20 +1,39 @@
 // Fast-path drivers: form factors over the representative k-nodes (one per
 // antipodal pair when the grid is symmetric), then the Gram.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "pii_capi_internal.hpp"
 
 namespace pii {
 
-// Element split for launches with few workgroups: as many splits as keep the
-// launch within about `target` workgroups, each split keeping at least
-// min_per elements.
-static int split_count(int base_wgs, int max_elems, int min_per, int target) {
-  constexpr int kFewWGs = 256;
-  if (base_wgs <= 0 || base_wgs >= kFewWGs || max_elems <= 0) return 1;
-  const int want = std::max(1, target / base_wgs);
-  return std::max(1, std::min(want, max_elems / min_per));
+// Element split that keeps a launch's rounds of workgroups full: base_wgs
+// workgroups of equal work on `slots` resident workgroups finish in
+// ceil(base/slots) rounds, the last one partly empty.  Returns the smallest
+// split s (at most smax) whose base*s workgroups fill at least 90 % of their
+// rounds, else the s that fills them best.  A few objects on a large grid
+// split many ways; hundreds of objects not at all.
+static int split_rounds(int base_wgs, int slots, int smax) {
+  if (base_wgs <= 0 || slots <= 0 || smax <= 1) return 1;
+  int best = 1;
+  double best_eff = 0.0;
+  for (int s = 1; s <= smax; ++s) {
+    const double n = (double)base_wgs * s / slots;
+    const double eff = n / std::ceil(n);
+    if (eff >= 0.9) return s;
+    if (eff > best_eff + 1e-12) {
+      best = s;
+      best_eff = eff;
+    }
+  }
+  return best;
+}
+
+// Largest split that keeps at least min_per elements per workgroup (and at
+// most 64 splits).
+static int max_split(int max_elems, int min_per) {
+  return std::max(1, std::min(64, max_elems / std::max(1, min_per)));
 }
 
 void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
@@ -41,11 +60,18 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
   }
   // splits and their partial-sum buffer (one buffer, used in launch order)
   const int dirblk = (b.Mh + 63) / 64;
-  const int s_tri = tri_series ? split_count(dirblk * N, ser_tri_faces, 256, 512) : 1;
-  const int s_surf = surfel_series ? split_count(dirblk * N, ser_surf_elems, 256, 512) : 1;
+  const int s_tri = tri_series ? split_rounds(dirblk * N, series_tri_slots(),
+                                              max_split(ser_tri_faces, 256)) : 1;
+  const int s_surf = surfel_series ? split_rounds(dirblk * N, series_surfel_slots(),
+                                                  max_split(ser_surf_elems, 256)) : 1;
+  // workgroups of the per-element kernel that have nodes (objects start at
+  // their own first direct node)
   const int span = (out_mode == 0 ? kg.Qh : ldX) - b.qbeg_min;
-  const int s_ff = std::max(split_count(((span + 63) / 64) * N, max_patches, 2, 512),
-                            split_count(((span + 63) / 64) * N, max_surf, 256, 512));
+  int ff_wgs = 0;
+  for (const ObjDesc& d : b.h_objs)
+    ff_wgs += std::max(0, ((out_mode == 0 ? kg.Qh : ldX) - d.qbeg + 63) / 64);
+  const int s_ff = split_rounds(ff_wgs, ff_fast_slots(),
+                                std::max(max_split(max_patches, 2), max_split(max_surf, 256)));
   const size_t need = std::max({series_split_doubles(N, b.Mh, s_tri),
                                 series_split_doubles(N, b.Mh, s_surf),
                                 ff_split_doubles(N, std::max(span, 0), s_ff)});
@@ -185,7 +211,7 @@ static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_
   int ds_split = 1;
   if (dseries) {
     const int dirblk = (b.Mh + 63) / 64;
-    ds_split = split_count(dirblk * N * 5, max_dfaces, 256, 1024);
+    ds_split = split_rounds(dirblk * N * 5, dseries_slots(), max_split(max_dfaces, 256));
     const size_t rec_b = align_up(sizeof(double) * ds_rec, 256);
     c.w.dsrec.ensure(rec_b + sizeof(long long) * 2 * N);
     c.w.dspart.ensure(sizeof(double) * dseries_part_doubles(N, b.Mh, ds_split));
@@ -193,17 +219,16 @@ static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_
     PII_HIP(hipMemcpyAsync(static_cast<char*>(c.w.dsrec.p) + rec_b, ds_off.data(),
                            sizeof(long long) * 2 * N, hipMemcpyHostToDevice, s));
   }
-  // per-element kernel: two workgroups per CU at its register and LDS use;
-  // split few-object launches to about two rounds of the workgroups that have
-  // nodes (objects with a derivative series start at qbeg)
+  // per-element kernel: split so that the workgroups that have nodes
+  // (objects with a derivative series start at qbeg) fill their rounds
   int max_patches = 0, max_surf = 0, base = 0;
   for (const ObjDesc& d : b.h_objs) {
     if (d.type == PII_GEOM_TRIANGLE) max_patches = std::max(max_patches, d.npatch);
     else max_surf = std::max(max_surf, d.n);
     base += (ldX - (d.type == PII_GEOM_TRIANGLE && d.dser ? d.qbeg : 0) + 63) / 64;
   }
-  const int nsplit = std::max(split_count(base, max_patches, 2, 1024),
-                              split_count(base, max_surf, 256, 1024));
+  const int nsplit = split_rounds(base, dtheta_slots(),
+                                  std::max(max_split(max_patches, 2), max_split(max_surf, 256)));
   const size_t need = dtheta_split_doubles(N, ldX, nsplit);
   if (need) c.w.split.ensure(sizeof(double) * need);
   c.phase_begin(3);

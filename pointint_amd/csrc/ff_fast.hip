@@ -440,6 +440,11 @@ size_t ff_split_doubles(int N, int span, int nsplit) {
   return nsplit > 1 ? 2 * (size_t)nsplit * N * ldP : 0;
 }
 
+int ff_fast_slots() {
+  static std::atomic<int> cache[16];
+  return resident_workgroups(k_ff_fast, kLanes * kWaves, cache);
+}
+
 void launch_ff_fast(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                     int qbeg_min, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                     unsigned* d_colmax, int nsplit, double* d_part) {

@@ -366,6 +366,16 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
 
 }  // namespace
 
+int series_tri_slots() {
+  static std::atomic<int> cache[16];
+  return resident_workgroups(k_series_tri, kLanes * kWaves, cache);
+}
+
+int series_surfel_slots() {
+  static std::atomic<int> cache[16];
+  return resident_workgroups(k_series_surfel<true>, kLanes * kWaves, cache);
+}
+
 size_t series_split_doubles(int N, int Mh, int nsplit) {
   return nsplit > 1 ? (size_t)nsplit * N * ((Mh + kLanes - 1) / kLanes) * (P + 1) * kLanes : 0;
 }

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -147,6 +149,29 @@ struct HostBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Workgroups of `kernel` (block threads, no dynamic LDS) resident at once on
+// the current device: compute units x the occupancy the runtime reports for
+// the kernel's registers and LDS.  Computed once per device id (cache[16]).
+template <class K>
+int resident_workgroups(K kernel, int block, std::atomic<int>* cache) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  int v = cache[dev].load(std::memory_order_relaxed);
+  if (v > 0) return v;
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  v = std::max(1, cus) * std::max(1, per_cu);
+  cache[dev].store(v, std::memory_order_relaxed);
+  return v;
+}
+
 // Per-call device workspaces (grow-only, reused across calls on one stream).
 struct Workspace {
   DevBuf world, faces, J, X, part, V, misc, colmax, B, C, gw, mrec, gam;
@@ -249,6 +274,12 @@ void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const dou
                           int Qh, const float* d_drec, const long long* d_doff, float2* d_Xt,
                           int ldX, unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
 size_t dtheta_split_doubles(int N, int ldX, int nsplit);
+// Resident workgroups of the element-splittable kernels (resident_workgroups).
+int ff_fast_slots();
+int series_tri_slots();
+int series_surfel_slots();
+int dtheta_slots();
+int dseries_slots();
 // Derivative series of affine triangle objects (ObjDesc::dser): per-face
 // reference records, the five families of per-direction sums and the 12
 // columns on the series shells (nodes q < qbeg); the per-element kernels then
