@@ -1,6 +1,6 @@
 // Fast form factor for gfx950: FP64 dot products, phases, range reduction and
-// small-span Taylor branch; FP32 for the phasor polynomials and the complex
-// combination; FP64 accumulation across patches.
+// small-span Taylor branch; the hardware FP32 sine/cosine on the reduced
+// phases, FP32 for the complex combination; FP64 accumulation across patches.
 //
 // Triangles (meshes and triangle soups) use a cancellation-free form of the
 // reference's element transform (cpp/src/element_functions.cpp:63-70 with the
@@ -99,7 +99,8 @@ __device__ __forceinline__ float sinc_from(float h, float sin_h) {
 // (g2/2) e^{i u0} with e^{i u0} = Q0^2.  |h_n| <= (n+1) H^n for span H, so the
 // truncation error is bounded by the first omitted term 2(NT+1) H^NT / (NT+2)!.
 template <int NT>
-__device__ __forceinline__ float2 tri_taylor(double al, double be, float g2, float2 Q0) {
+__device__ __forceinline__ void tri_taylor(double al, double be, float g2, float2 Q0,
+                                           float2& acc) {
   constexpr double c[12] = {1.0, 1.0 / 3.0, -1.0 / 12.0, -1.0 / 60.0, 1.0 / 360.0,
                             1.0 / 2520.0, -1.0 / 20160.0, -1.0 / 181440.0, 1.0 / 1814400.0,
                             1.0 / 19958400.0, -1.0 / 239500800.0, -1.0 / 3113510400.0};
@@ -114,27 +115,31 @@ __device__ __forceinline__ float2 tri_taylor(double al, double be, float g2, flo
   const float px = Q0.x * Q0.x - Q0.y * Q0.y;
   const float py = 2.0f * Q0.x * Q0.y;
   const float tr = (float)re * (0.5f * g2), ti = (float)im * (0.5f * g2);
-  return make_float2(px * tr - py * ti, px * ti + py * tr);
+  acc.x = fmaf(px, tr, fmaf(-py, ti, acc.x));
+  acc.y = fmaf(px, ti, fmaf(py, tr, acc.y));
 }
 
 // Main branch: middle node m and the other two (p, q); see the file comment.
-// The node choice is made with selects.  Half spans and the full span are
-// formed in FP32 from the FP32-rounded alpha, beta: their absolute error,
-// 6e-8 |k| diam, changes the result by a relative 6e-8 at most, because the
-// span D itself is at least that large here.
-__device__ __forceinline__ float2 tri_main(double al, double be, float g2, float2 Q0, float2 Q1,
-                                           float2 Q2) {
+// The formula is symmetric in p and q, so only the middle node matters: it
+// is node 0 when a and b differ in sign, else the one of nodes 1, 2 whose
+// phase is smaller in magnitude.  u_m is the median of (0, a, b)
+// (v_med3); Q_p = Q1 when m = 0 (else Q0), Q_q = Q1 when m = 2 (else Q2);
+// everything by selects.  Half spans and the span are formed in FP32 from
+// the FP32-rounded alpha, beta: their absolute error, 6e-8 |k| diam, changes
+// the result by a relative 6e-8 at most, because the span D itself is at
+// least that large here.
+__device__ __forceinline__ void tri_main(double al, double be, float g2, float2 Q0, float2 Q1,
+                                         float2 Q2, float2& acc) {
   const float a = (float)al, b = (float)be;
-  // relative phases u0 = 0, u1 = a, u2 = b; m is the middle one
-  const bool m0 = (al * be <= 0.0);
-  const bool m1 = (!m0) & (fabsf(a) < fabsf(b));
-  const bool m01 = m0 | m1;
-  const float um = m0 ? 0.0f : (m1 ? a : b);
+  const bool m0 = a * b <= 0.0f;
+  const bool m2 = (!m0) & (fabsf(b) <= fabsf(a));
+  const float um = __builtin_amdgcn_fmed3f(0.0f, a, b);
   const float up = m0 ? a : 0.0f;
-  const float uq = m01 ? b : a;
+  const float uq = m2 ? a : b;
+  const bool m1 = !(m0 | m2);
   const float2 Qm = make_float2(m0 ? Q0.x : (m1 ? Q1.x : Q2.x), m0 ? Q0.y : (m1 ? Q1.y : Q2.y));
   const float2 Qp = make_float2(m0 ? Q1.x : Q0.x, m0 ? Q1.y : Q0.y);
-  const float2 Qq = make_float2(m01 ? Q2.x : Q1.x, m01 ? Q2.y : Q1.y);
+  const float2 Qq = make_float2(m2 ? Q1.x : Q2.x, m2 ? Q1.y : Q2.y);
   const float hp = 0.5f * (um - up);
   const float hq = 0.5f * (uq - um);
   const float D = uq - up;
@@ -148,19 +153,20 @@ __device__ __forceinline__ float2 tri_main(double al, double be, float g2, float
   const float ty = Qm.x * ny + Qm.y * nx;
   const float f = g2 * __builtin_amdgcn_rcpf(D);
   // -i * (tx + i ty) = ty - i tx
-  return make_float2(ty * f, -tx * f);
+  acc.x = fmaf(ty, f, acc.x);
+  acc.y = fmaf(-tx, f, acc.y);
 }
 
 // gamma * e^{i u0} * Phi(alpha, beta) with a per-lane branch on the span
 // max(0, a, b) - min(0, a, b) = max(|a|, |b|, |a - b|), taken in FP32 (both
 // sides of the threshold are accurate near it).  Lanes below kLaneTaylor
 // take the 7-term Taylor sum (first omitted term < 1e-8 there).
-__device__ __forceinline__ float2 tri_term(double al, double be, float g2, float2 Q0, float2 Q1,
-                                           float2 Q2) {
+__device__ __forceinline__ void tri_term(double al, double be, float g2, float2 Q0, float2 Q1,
+                                         float2 Q2, float2& acc) {
   const float a = (float)al, b = (float)be;
   const float H = fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(a - b));
-  if (H < kLaneTaylor) return tri_taylor<7>(al, be, g2, Q0);
-  return tri_main(al, be, g2, Q0, Q1, Q2);
+  if (H < kLaneTaylor) tri_taylor<7>(al, be, g2, Q0, acc);
+  else tri_main(al, be, g2, Q0, Q1, Q2, acc);
 }
 
 // J1(x)/x for x >= 4 in FP32 from the modulus and phase of J1 + i Y1:
@@ -257,7 +263,7 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
         tab[s][lane] = half_phasor(u);
       }
       __syncthreads();
-      float px = 0.f, py = 0.f;
+      float2 pacc = make_float2(0.f, 0.f);
       for (int f = wave; f < fcnt; f += kWaves) {
         const auto* r = rec + (size_t)kRecFastTri * (fbeg + f);
         const double al = kx * r[0] + ky * r[1] + kz * r[2];
@@ -267,28 +273,25 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
         const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
         const int code = (int)(loc >> 24);
         const float2 Q0 = tab[loc & 0xff][lane];
-        float2 t;
         // code <= cut[j]  =>  H <= kmax * 2^(code-64) <= thr[j] for every lane
         if (code <= cut[3]) {
-          if (code <= cut[0]) t = tri_taylor<4>(al, be, g2, Q0);
-          else if (code <= cut[1]) t = tri_taylor<5>(al, be, g2, Q0);
-          else if (code <= cut[2]) t = tri_taylor<6>(al, be, g2, Q0);
-          else t = tri_taylor<7>(al, be, g2, Q0);
+          if (code <= cut[0]) tri_taylor<4>(al, be, g2, Q0, pacc);
+          else if (code <= cut[1]) tri_taylor<5>(al, be, g2, Q0, pacc);
+          else if (code <= cut[2]) tri_taylor<6>(al, be, g2, Q0, pacc);
+          else tri_taylor<7>(al, be, g2, Q0, pacc);
         } else if (code <= cut[7]) {
-          if (code <= cut[4]) t = tri_taylor<8>(al, be, g2, Q0);
-          else if (code <= cut[5]) t = tri_taylor<9>(al, be, g2, Q0);
-          else if (code <= cut[6]) t = tri_taylor<10>(al, be, g2, Q0);
-          else t = tri_taylor<11>(al, be, g2, Q0);
+          if (code <= cut[4]) tri_taylor<8>(al, be, g2, Q0, pacc);
+          else if (code <= cut[5]) tri_taylor<9>(al, be, g2, Q0, pacc);
+          else if (code <= cut[6]) tri_taylor<10>(al, be, g2, Q0, pacc);
+          else tri_taylor<11>(al, be, g2, Q0, pacc);
         } else {
           const float2 Q1 = tab[(loc >> 8) & 0xff][lane];
           const float2 Q2 = tab[(loc >> 16) & 0xff][lane];
-          t = tri_term(al, be, g2, Q0, Q1, Q2);
+          tri_term(al, be, g2, Q0, Q1, Q2, pacc);
         }
-        px += t.x;
-        py += t.y;
       }
-      accx += (double)px;
-      accy += (double)py;
+      accx += (double)pacc.x;
+      accy += (double)pacc.y;
       __syncthreads();
     }
   } else {
