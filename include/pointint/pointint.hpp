@@ -36,6 +36,7 @@ namespace PoIntInt {
 
 // ---- minimal dense types standing in for the Eigen types of the reference ----
 using VectorXd = std::vector<double>;
+using VectorXcd = std::vector<std::complex<double>>;
 using Vector3d = std::array<double, 3>;
 
 struct MatrixXd {
@@ -199,6 +200,12 @@ double compute_intersection_volume_cuda(const Geometry& geom1, const Geometry& g
 std::complex<double> compute_Ak_cuda(const Geometry& geom, const Vector3d& k,
                                      const std::shared_ptr<DoFParameterization>& dof,
                                      const VectorXd& dofs, int blockSize = 256);
+
+// dA(k)/dtheta at one k-vector (cpp/include/compute_intersection_volume.hpp:128-135):
+// num_dofs complex values; zeros for |k| < 1e-10, as the reference.
+VectorXcd compute_Ak_gradient_cuda(const Geometry& geom, const Vector3d& k,
+                                   const std::shared_ptr<DoFParameterization>& dof,
+                                   const VectorXd& dofs, int blockSize = 256);
 
 // Divergence-theorem self volume with identity DoFs (compute_volume.hpp).
 double compute_volume_cpu(const Geometry& geom, bool enable_profiling = false);

@@ -156,6 +156,20 @@ int pii_geometry_info(const pii_geometry* g, int* type, int* num_elements, int* 
 int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const double* dofs,
                     int ndofs, const pii_kgrid* kg, int precision, double* out);
 
+/* dA(k_q)/dtheta for every node of kg in FP64 (the reference's element
+ * derivatives), written to out[2*Q*ndofs] as a row-major Q x ndofs complex
+ * matrix (interleaved re, im, host memory): out[2*(q*ndofs + t)].  ndofs as
+ * for pii_form_factor.  Replaces the CudaComputeAkGradientFunc plug-ins
+ * compute_Ak_gradient_{affine_triangle,affine_disk,affine_gaussian,
+ * triangle_mesh}_cuda_wrapper (cpp/include/dof/cuda/dof_cuda_interface.hpp:34-40,
+ * cpp/src/dof/cuda/affine_dof_cuda.cu:932-1329,
+ * cpp/src/dof/cuda/triangle_mesh_dof_cuda.cu:387-466) and, on a one-node
+ * grid, compute_Ak_gradient_cuda (cpp/include/compute_intersection_volume.hpp:128-135).
+ * Q x ndofs doubles of device and host memory: a testing-scale helper, as
+ * in the reference (the volume path contracts instead: pii_volume_matrix_vjp). */
+int pii_form_factor_jacobian(pii_context* ctx, const pii_geometry* g, int dof_kind,
+                             const double* dofs, int ndofs, const pii_kgrid* kg, double* out);
+
 /* ---- all-pairs volume matrix ----------------------------------------------- */
 /* V[n*n] (row-major, symmetric) for n objects, each with its own DoF kind and
  * DoF vector.  Replaces compute_intersection_volume_matrix_cuda with

@@ -302,6 +302,19 @@ def compute_form_factor(geom: Geometry, kgrid: KGrid, dofs=None, dof_kind="affin
     return out[0::2] + 1j * out[1::2]
 
 
+def compute_form_factor_jacobian(geom: Geometry, kgrid: KGrid, dofs=None,
+                                 dof_kind="affine") -> np.ndarray:
+    """dA(k_q)/dtheta for every node (complex128, Q x ndofs) in FP64 with the
+    reference's element derivatives (its gradient plug-ins' output)."""
+    kinds, dl = _dof_args([geom], None if dofs is None else [dofs], [dof_kind])
+    D = dl[0].shape[0]
+    out = np.empty(2 * kgrid.Q * D)
+    N.check(N.load().pii_form_factor_jacobian(geom.ctx.handle, geom.handle, kinds[0],
+                                              N.ptr(dl[0]), D, kgrid.handle, N.ptr(out)))
+    out = out.reshape(kgrid.Q, D, 2)
+    return out[..., 0] + 1j * out[..., 1]
+
+
 def _batch_args(geoms, dofs, dof_kinds):
     """Pointer arrays of a batched call (numpy-built: per-object ctypes work
     would cost more than the GPU step for a few hundred objects).  Returns the

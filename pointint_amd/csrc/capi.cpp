@@ -864,6 +864,26 @@ int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const
   });
 }
 
+int pii_form_factor_jacobian(pii_context* ctx, const pii_geometry* g, int dof_kind,
+                             const double* dofs, int ndofs, const pii_kgrid* kg, double* out) {
+  return guarded([&] {
+    require(ctx && g && kg && dofs && out, "pii_form_factor_jacobian: NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    const GeometryDev* gp = &g->g;
+    Batch b;
+    b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
+    const int Q = kg->k.Q;
+    const size_t bytes = sizeof(double2) * (size_t)Q * ndofs;
+    ctx->c.w.J.ensure(bytes);
+    launch_jacobian(ctx->c.stream, b.d_objs, gp->type, dof_kind, gp->n, kg->k.full.as<double>(),
+                    Q, ndofs, ctx->c.w.J.as<double2>());
+    PII_HIP(hipGetLastError());
+    PII_HIP(hipMemcpyAsync(out, ctx->c.w.J.p, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
 int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
                       const int* dof_kinds, const double* const* dofs, const int* ndofs,
                       const pii_kgrid* kg, int ff_precision, int gram_mode, double* V) {

@@ -274,6 +274,10 @@ void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const dou
                           int Qh, const float* d_drec, const long long* d_doff, float2* d_Xt,
                           int ldX, unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
 size_t dtheta_split_doubles(int N, int ldX, int nsplit);
+// dA(k_q)/dtheta of one object (d_obj[0]) at every node of the full grid,
+// FP64, row-major Q x D complex.
+void launch_jacobian(hipStream_t s, const ObjDesc* d_obj, int type, int dof_kind, int n_elems,
+                     const double* d_kfull, int Q, int D, double2* d_out);
 // Resident workgroups of the element-splittable kernels (resident_workgroups).
 int ff_fast_slots();
 int series_tri_slots();

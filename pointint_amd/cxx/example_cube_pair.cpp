@@ -1,8 +1,9 @@
 // Example C++ caller: the two-cube configuration through the reference-named
 // C++ API (Lebedev 302 x 32 k-grid, unit cube and a copy shifted by
 // (0.3, 0.2, 0.1)).  Prints the 2 x 2 volume matrix, the pairwise volume and
-// its gradient w.r.t. the second cube's affine DoFs, and the translation
-// corner of the pair's Gauss-Newton cross Hessian.
+// its gradient w.r.t. the second cube's affine DoFs, the translation corner
+// of the pair's Gauss-Newton cross Hessian, and A(k) with dA(k)/dtheta of the
+// second cube at one k under an affine map.
 //
 //   usage: example_cube_pair <lebedev_029.txt> [fp64|fast]
 #include <cstdio>
@@ -51,5 +52,16 @@ int main(int argc, char** argv) {
     for (int b = 0; b < 3; ++b) std::printf(" %.15e", ph.hessian_cross(a, b));
   std::printf("\n");
   std::printf("self %.15f\n", compute_volume_cpu(geoms[0]));
+  // single-k derivatives of the second cube: A(k) and dA(k)/dtheta under a
+  // non-trivial affine map
+  const Vector3d k = {1.3, -0.7, 2.1};
+  const VectorXd th = {0.05, -0.02, 0.1, 1.1, 0.1, 0.0, -0.05, 0.9, 0.2, 0.0, 0.1, 1.05};
+  const std::complex<double> ak = compute_Ak_cuda(geoms[1], k, aff, th);
+  const VectorXcd gk = compute_Ak_gradient_cuda(geoms[1], k, aff, th);
+  if (gk.size() != 12) return 1;
+  std::printf("ak %.15e %.15e\n", ak.real(), ak.imag());
+  std::printf("gradk");
+  for (int t = 0; t < 12; ++t) std::printf(" %.15e %.15e", gk[t].real(), gk[t].imag());
+  std::printf("\n");
   return 0;
 }

@@ -523,6 +523,35 @@ std::complex<double> compute_Ak_cuda(const Geometry& geom, const Vector3d& k,
   return {out[0], out[1]};
 }
 
+// cpp/src/compute_intersection_volume.cu:558-626: the gradient plug-in on a
+// one-node grid (|k| < 1e-10 gives zeros without a device call)
+VectorXcd compute_Ak_gradient_cuda(const Geometry& geom, const Vector3d& k,
+                                   const std::shared_ptr<DoFParameterization>& dof,
+                                   const VectorXd& dofs, int /*blockSize*/) {
+  const int nd = (int)dofs.size();
+  VectorXcd zero(nd, {0.0, 0.0});
+  pii_context* ctx = context();
+  if (!ctx) return zero;
+  const double km = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  if (km < 1e-10) return zero;
+  CallObjects co;
+  const double dir[3] = {k[0] / km, k[1] / km, k[2] / km};
+  const double w = 1.0;
+  if (!report(pii_kgrid_create(ctx, 1, dir, &km, &w, &co.kg), "compute_Ak_gradient_cuda"))
+    return zero;
+  pii_geometry* g = upload(ctx, geom);
+  if (!g) return zero;
+  co.geoms.push_back(g);
+  std::vector<double> out(2 * (size_t)nd);
+  if (!report(pii_form_factor_jacobian(ctx, g, dof ? dof->dof_kind() : PII_DOF_AFFINE,
+                                       dofs.data(), nd, co.kg, out.data()),
+              "compute_Ak_gradient_cuda"))
+    return zero;
+  VectorXcd grad(nd);
+  for (int t = 0; t < nd; ++t) grad[t] = {out[2 * t], out[2 * t + 1]};
+  return grad;
+}
+
 // cpp/src/compute_volume.cpp:61-73 with cpp/src/dof/affine_dof.cpp:324-384 (identity DoFs)
 double compute_volume_cpu(const Geometry& g, bool) {
   double v = 0.0;

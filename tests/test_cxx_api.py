@@ -19,7 +19,8 @@ def test_cxx_library_exports_reference_names():
                  "PoIntInt::compute_intersection_volume_cuda", "PoIntInt::compute_Ak_cuda",
                  "PoIntInt::build_kgrid", "PoIntInt::load_lebedev_txt",
                  "PoIntInt::make_triangle_mesh", "PoIntInt::make_point_cloud",
-                 "PoIntInt::gauss_legendre_interval", "PoIntInt::compute_volume_cpu"):
+                 "PoIntInt::gauss_legendre_interval", "PoIntInt::compute_volume_cpu",
+                 "PoIntInt::compute_Ak_gradient_cuda"):
         assert name in out, name
 
 
@@ -39,6 +40,11 @@ def _run(mode):
             vals["self"] = float(parts[1])
         elif parts[0] == "hcross_tt":
             vals["hcross_tt"] = np.array([float(x) for x in parts[1:10]]).reshape(3, 3)
+        elif parts[0] == "ak":
+            vals["ak"] = float(parts[1]) + 1j * float(parts[2])
+        elif parts[0] == "gradk":
+            x = np.array([float(v) for v in parts[1:25]])
+            vals["gradk"] = x[0::2] + 1j * x[1::2]
     return vals
 
 
@@ -63,3 +69,11 @@ def test_cxx_example_cube_pair(oracle, mode, tol):
     c = wq * (z["J"][:, 0] * np.conj(z["J"][:, 1])).real / (8 * np.pi ** 3)
     H_tt = (k * c[:, None]).T @ k
     assert np.allclose(v["hcross_tt"], H_tt, rtol=1e-4, atol=1e-9 * np.abs(H_tt).max())
+    # single-k A and dA/dtheta of the second cube under an affine map, against
+    # the oracle (FP64 on both sides)
+    th = np.array([0.05, -0.02, 0.1, 1.1, 0.1, 0.0, -0.05, 0.9, 0.2, 0.0, 0.1, 1.05])
+    kk = np.array([1.3, -0.7, 2.1])
+    a64 = oracle.compute_A(g1, kk, th, oracle.DOF_AFFINE)
+    g64 = np.asarray(oracle.compute_A_gradient(g1, kk, th, oracle.DOF_AFFINE)).ravel()
+    assert abs(v["ak"] - a64) <= 1e-12 * abs(a64)
+    assert np.max(np.abs(v["gradk"] - g64)) <= 1e-11 * np.max(np.abs(g64))
