@@ -267,7 +267,9 @@ __device__ __forceinline__ void dseries_eval_body(const ObjDesc& d, BlockObj bo,
                                  (size_t)(P + 1) * kLanes;
   for (int z = wave; z < nsplit; z += kWaves) {
     const double* o = src + (size_t)z * stride;
-    for_chunked<PP>([&](int n) { H[n] += o[n * kLanes + lane]; });
+    // all orders of a split in flight at once (a latency-bound sum)
+#pragma unroll
+    for (int n = 0; n <= PP; ++n) H[n] += o[n * kLanes + lane];
   }
   sum_waves_123<PP>(H, red, wave, lane);
   if (wave == 0)

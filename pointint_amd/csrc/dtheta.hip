@@ -273,28 +273,40 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
   write_columns(v, q, Qh, kx, ky, kz, krep, d.col, Xt, ldX, colmax, lane);
 }
 
-// Finishes a split launch: sums the nsplit partials of each node in split
-// order and writes its columns.  One wave per (object, 64 nodes).
-__global__ void __launch_bounds__(kLanes) k_dtheta_finish(
+// Finishes a split launch: each of the four waves sums every fourth split of
+// the nodes' partials (splits in increasing order), the four sums are added
+// in the fixed order ((w0 + w1) + w2) + w3 through LDS, and wave 0 writes the
+// columns.  One workgroup per (object, 64 nodes).
+__global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_finish(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh,
     float2* __restrict__ Xt, int ldX, unsigned* __restrict__ colmax,
     const double* __restrict__ part, int nsplit) {
-  const int obj = blockIdx.y, blk = blockIdx.x, lane = threadIdx.x;
+  __shared__ double red[kWaves][20][kLanes];
+  const int obj = blockIdx.y, blk = blockIdx.x;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const ObjDesc& d = objs[obj];
   const int q0 = dtheta_first_node(d) + blk * kLanes;
-  if (q0 >= ldX) return;
+  if (q0 >= ldX) return;  // uniform over the workgroup
   const int q = q0 + lane;
-  const int qc = q < Qh ? q : Qh - 1;
-  const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
   double v[20];
 #pragma unroll
   for (int i = 0; i < 20; ++i) v[i] = 0.0;
   const size_t stride = (size_t)gridDim.y * gridDim.x * kLanes * 20;
   const double* src = part + ((size_t)obj * gridDim.x * kLanes + (size_t)blk * kLanes + lane) * 20;
-  for (int z = 0; z < nsplit; ++z) {
+  for (int z = wave; z < nsplit; z += kWaves) {
 #pragma unroll
     for (int i = 0; i < 20; ++i) v[i] += src[(size_t)z * stride + i];
   }
+#pragma unroll
+  for (int i = 0; i < 20; ++i) red[wave][i][lane] = v[i];
+  __syncthreads();
+  if (wave != 0) return;
+  const int qc = q < Qh ? q : Qh - 1;
+  const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
+#pragma unroll
+  for (int i = 0; i < 20; ++i) v[i] = ((red[0][i][lane] + red[1][i][lane]) + red[2][i][lane]) +
+                                      red[3][i][lane];
   write_columns(v, q, Qh, kx, ky, kz, krep, d.col, Xt, ldX, colmax, lane);
 }
 
@@ -326,8 +338,8 @@ void launch_dtheta_affine(hipStream_t s, const ObjDesc* d_objs, int N, const dou
   hipLaunchKernelGGL(k_dtheta_affine, dim3(nblk, N, split ? nsplit : 1), dim3(kLanes * kWaves), 0,
                      s, d_objs, d_krep, Qh, d_drec, d_doff, d_Xt, ldX, d_colmax, part);
   if (split)
-    hipLaunchKernelGGL(k_dtheta_finish, dim3(nblk, N), dim3(kLanes), 0, s, d_objs, d_krep, Qh,
-                       d_Xt, ldX, d_colmax, part, nsplit);
+    hipLaunchKernelGGL(k_dtheta_finish, dim3(nblk, N), dim3(kLanes * kWaves), 0, s, d_objs,
+                       d_krep, Qh, d_Xt, ldX, d_colmax, part, nsplit);
 }
 
 }  // namespace pii

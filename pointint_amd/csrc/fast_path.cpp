@@ -11,9 +11,10 @@ namespace pii {
 // Element split that keeps a launch's rounds of workgroups full: base_wgs
 // workgroups of equal work on `slots` resident workgroups finish in
 // ceil(base/slots) rounds, the last one partly empty.  Returns the smallest
-// split s (at most smax) whose base*s workgroups fill at least 90 % of their
-// rounds, else the s that fills them best.  A few objects on a large grid
-// split many ways; hundreds of objects not at all.
+// split s (at most smax) that gives at least two rounds (so that workgroups
+// finishing early are followed by others) filled to at least 90 %, else the
+// one that fills its rounds best.  A few objects on a large grid split many
+// ways; hundreds of objects not at all.
 static int split_rounds(int base_wgs, int slots, int smax) {
   if (base_wgs <= 0 || slots <= 0 || smax <= 1) return 1;
   int best = 1;
@@ -21,10 +22,11 @@ static int split_rounds(int base_wgs, int slots, int smax) {
   for (int s = 1; s <= smax; ++s) {
     const double n = (double)base_wgs * s / slots;
     const double eff = n / std::ceil(n);
-    if (eff >= 0.9) return s;
-    if (eff > best_eff + 1e-12) {
+    if (n >= 2.0 && eff >= 0.9) return s;
+    const double score = std::min(n, 2.0) / 2.0 * eff;  // rounds below two count against
+    if (score > best_eff + 1e-12) {
       best = s;
-      best_eff = eff;
+      best_eff = score;
     }
   }
   return best;

@@ -335,7 +335,9 @@ __device__ __forceinline__ void series_finish_body(
   const double* src = part + ((size_t)bo.obj * gridDim.x + bo.blk) * (size_t)(P + 1) * kLanes;
   for (int z = wave; z < nsplit; z += kWaves) {
     const double* o = src + (size_t)z * stride;
-    for_chunked<PP>([&](int n) { H[n] += o[n * kLanes + lane]; });
+    // all orders of a split in flight at once (a latency-bound sum)
+#pragma unroll
+    for (int n = 0; n <= PP; ++n) H[n] += o[n * kLanes + lane];
   }
   reduce_and_evaluate<PP, kKind>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh,
                                  out_mode, Aout, ldA, Xout, ldX, colmax);
