@@ -500,7 +500,7 @@ def main():
     if not args.no_cpu:
         t_full, nq, t_s, cores = cpu_baseline(wl, kg, args.cpu_seconds)
         cpu = {"value": pairs / t_full, "unit": "pairs/s", "cores": cores, "kind": "port",
-               "sample": f"FP64 oracle on {nq} of {kg.Q} k-nodes (every {kg.Q // nq}th), all "
+               "sample": f"FP64 oracle on {nq} of {kg.Q} k-nodes (every {round(kg.Q / nq)}th), all "
                          f"{N} objects, {t_s:.1f} s, extrapolated x{kg.Q / nq:.1f}; "
                          f"host {cpu_model()}",
                "ms_per_step_extrapolated": t_full * 1e3}
