@@ -178,14 +178,18 @@ int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
                       const int* dof_kinds, const double* const* dofs, const int* ndofs,
                       const pii_kgrid* kg, int ff_precision, int gram_mode, double* V);
 
-/* Gauss-Newton Hessian blocks of the volume matrix for AffineDoF objects
- * (triangles, disks, Gaussians):
+/* Gauss-Newton Hessian blocks of the volume matrix:
  *   H_ij = (1/8 pi^3) sum_q w_q Re( dA_i/dtheta_i (dA_j/dtheta_j)^H ),
- * every ordered pair (i, j), i == j included.  H is (12n x 12n) row-major;
- * block (i, j) occupies rows 12i..12i+11 and columns 12j..12j+11, DoF order
- * [t, A row-major].  gram_mode as for pii_volume_matrix.  V (may be NULL)
- * receives the volume matrix of the same call.  Other DoF kinds return
- * PII_ERR_UNSUPPORTED.
+ * every ordered pair (i, j), i == j included.  H is (D x D) row-major with
+ * D = sum_i ndofs[i]; block (i, j) occupies rows o_i..o_i+ndofs[i]-1 and
+ * columns o_j..o_j+ndofs[j]-1, o_i = sum_{i' < i} ndofs[i'] (12 i when every
+ * object has AffineDoF), in each DoF vector's own order.  gram_mode as for
+ * pii_volume_matrix.  V (may be NULL) receives the volume matrix of the same
+ * call.  AffineDoF objects (triangles, disks, Gaussians) take the fast
+ * derivative kernels when every object of the call has AffineDoF;
+ * TriangleMeshDoF objects (and every object of a call that has one) take the
+ * FP64 per-node Jacobian of pii_form_factor_jacobian.  D plus n (with V) is
+ * limited to 16384 (PII_ERR_INVALID above).
  * Replaces the HESSIAN flag of compute_intersection_volume_matrix_cuda/cpu
  * (cpp/src/compute_intersection_volume_multi_object.cpp:236-292), whose
  * hessian_ij[i][j] is block (i, j) and whose hessian_ii / hessian_jj are zero

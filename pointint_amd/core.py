@@ -370,14 +370,20 @@ def volume_matrix_vjp(geoms: Sequence[Geometry], kgrid: KGrid, dL_dV, dofs=None,
 
 
 def volume_matrix_hessian(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None,
-                          gram: str = "fp16x3", return_volume: bool = False):
-    """Gauss-Newton Hessian blocks of V for AffineDoF objects: H (12n x 12n),
-    block (i, j) = (1/8 pi^3) sum_q w_q Re(dA_i/dtheta_i (dA_j/dtheta_j)^H);
-    with return_volume also V of the same call."""
+                          gram: str = "fp16x3", return_volume: bool = False, dof_kinds=None):
+    """Gauss-Newton Hessian blocks of V: H (D x D), D = sum of the DoF
+    counts, block (i, j) = (1/8 pi^3) sum_q w_q Re(dA_i/dtheta_i
+    (dA_j/dtheta_j)^H) at the rows and columns of objects i and j in order
+    (12 per AffineDoF object, 3 per vertex for "trimesh"); with return_volume
+    also V of the same call.  dof_kinds defaults to AffineDoF for every
+    object."""
     n = len(geoms)
     ctx = geoms[0].ctx
-    gh, kh, dh, nh, _keep = _batch_args(geoms, dofs, None if dofs is None else ["affine"] * n)
-    H = np.empty((12 * n, 12 * n))
+    if dof_kinds is None and dofs is not None:
+        dof_kinds = ["affine"] * n
+    gh, kh, dh, nh, _keep = _batch_args(geoms, dofs, dof_kinds)
+    D = int(nh.sum())
+    H = np.empty((D, D))
     V = np.empty((n, n)) if return_volume else None
     N.check(N.load().pii_volume_matrix_hessian(ctx.handle, n, gh.ctypes.data, kh.ctypes.data,
                                                dh.ctypes.data, nh.ctypes.data, kgrid.handle,

@@ -936,18 +936,14 @@ int pii_volume_matrix_hessian(pii_context* ctx, int n, const pii_geometry* const
     require(ctx && kg && H, "pii_volume_matrix_hessian: NULL argument");
     require(n > 0 && geoms && dof_kinds && dofs && ndofs,
             "pii_volume_matrix_hessian: bad arrays");
-    for (int i = 0; i < n; ++i) {
-      require(geoms[i] && dofs[i], "pii_volume_matrix_hessian: NULL object");
-      if (dof_kinds[i] != PII_DOF_AFFINE)
-        throw Error(PII_ERR_UNSUPPORTED, "Gauss-Newton Hessian supports AffineDoF objects only");
-    }
+    for (int i = 0; i < n; ++i) require(geoms[i] && dofs[i], "pii_volume_matrix_hessian: NULL object");
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     PII_HIP(hipSetDevice(ctx->c.device));
     std::vector<const GeometryDev*> gs(n);
     for (int i = 0; i < n; ++i) gs[i] = &geoms[i]->g;
     Batch b;
     b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
-    volume_matrix_hessian(ctx->c, b, kg->k, gram_mode, H, V);
+    volume_matrix_hessian(ctx->c, b, kg->k, gram_mode, ndofs, H, V);
   });
 }
 

@@ -151,6 +151,30 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
   PII_HIP(hipStreamSynchronize(s));
 }
 
+// Gram of the D FP32 columns in c.w.Xt (leading dimension ldX, column
+// maxima colmax) into c.w.H (D x D, FP64): entry (a, b) = (1/8 pi^3)
+// sum_q w_q Re(x_a conj x_b) over the full grid.
+static void column_gram(Context& c, const KGridDev& kg, int gram_mode, int D, int ldX,
+                        unsigned* colmax) {
+  hipStream_t s = c.stream;
+  c.w.H.ensure(sizeof(double) * (size_t)D * D);
+  c.phase_begin(2);
+  if (gram_mode == PII_GRAM_FP64) {
+    const int nsplit = gram_fp64_splits(kg.Qh, D);
+    c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
+    launch_gram_fp64_x(s, c.w.Xt.as<float2>(), kg.Qh, D, ldX, c.w.part.as<double>(), nsplit,
+                       c.w.H.as<double>());
+  } else {
+    const int K = 2 * ldX;
+    const int nsplit = gram_split_splits(K, D);
+    c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
+    launch_gram_split(s, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.Xt.as<float>(), D, K, colmax,
+                      c.w.part.as<double>(), nsplit, c.w.H.as<double>());
+  }
+  c.phase_end(2);
+  PII_HIP(hipGetLastError());
+}
+
 // Gram of the columns [sqrt(w) A_0..A_{N-1} | sqrt(w) dA_i/dtheta_t] of
 // AffineDoF objects: the first N columns (when with_A) come from the fast
 // form-factor path, the 12 N derivative columns (12 col + t) from the
@@ -250,22 +274,60 @@ static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_
                        need ? c.w.split.as<double>() : nullptr);
   c.phase_end(3);
   PII_HIP(hipGetLastError());
-  c.w.H.ensure(sizeof(double) * (size_t)D * D);
-  c.phase_begin(2);
-  if (gram_mode == PII_GRAM_FP64) {
-    const int nsplit = gram_fp64_splits(Qh, D);
-    c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
-    launch_gram_fp64_x(s, X, Qh, D, ldX, c.w.part.as<double>(), nsplit, c.w.H.as<double>());
-  } else {
-    const int K = 2 * ldX;
-    const int nsplit = gram_split_splits(K, D);
-    c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
-    launch_gram_split(s, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.Xt.as<float>(), D, K, colmax,
-                      c.w.part.as<double>(), nsplit, c.w.H.as<double>());
-  }
-  c.phase_end(2);
-  PII_HIP(hipGetLastError());
+  column_gram(c, kg, gram_mode, D, ldX, colmax);
   return D;
+}
+
+// Gram of the derivative columns for any mix of DoF kinds: the same column
+// set as affine_gram ([sqrt(w) A | sqrt(w) dA_i/dtheta_i], object i's D_i =
+// ndofs[i] columns in caller order), with every derivative column from the
+// FP64 Jacobian kernels (the reference's per-node element derivatives) on the
+// representative nodes.  This is the route for TriangleMeshDoF objects
+// (D_i = 3 x DoF vertices); all-affine batches take affine_gram.
+static int general_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                        bool with_A, const int* ndofs) {
+  const int N = b.N, Qh = kg.Qh;
+  const int na = with_A ? N : 0;
+  if (gram_mode != PII_GRAM_FP64 && gram_mode != PII_GRAM_BF16X3 && gram_mode != PII_GRAM_FP16X3)
+    throw Error(PII_ERR_INVALID, "unknown Gram mode");
+  std::vector<long long> first(N);  // first derivative column, caller order
+  long long D = na;
+  int maxd = 0;
+  for (int i = 0; i < N; ++i) {
+    first[i] = D;
+    D += ndofs[i];
+    maxd = std::max(maxd, ndofs[i]);
+  }
+  if (D > kMaxDerivativeColumns)
+    throw Error(PII_ERR_INVALID, "derivative Gram: " + std::to_string(D) +
+                                     " columns exceed the limit of " +
+                                     std::to_string(kMaxDerivativeColumns));
+  hipStream_t s = c.stream;
+  const int ldX = (int)align_up((size_t)Qh, 64);
+  c.w.Xt.ensure(sizeof(float2) * (size_t)ldX * D);
+  c.w.cmax2.ensure(sizeof(unsigned) * (size_t)D);
+  float2* X = c.w.Xt.as<float2>();
+  unsigned* colmax = c.w.cmax2.as<unsigned>();
+  PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * D, s));
+  if (with_A) {
+    c.phase_begin(1);
+    launch_form_factors_fast(s, b, kg, 1, nullptr, 0, X, ldX, colmax, c.w, &c);
+    c.phase_end(1);
+    PII_HIP(hipGetLastError());
+  }
+  c.w.J.ensure(sizeof(double2) * (size_t)ldX * std::max(maxd, 1));
+  c.phase_begin(3);
+  for (int slot = 0; slot < N; ++slot) {
+    const ObjDesc& d = b.h_objs[slot];
+    const long long col = first[d.col];
+    launch_jacobian_columns(s, b.d_objs + slot, d.dof_kind, d.n, kg.rep.as<double>(), Qh,
+                            ndofs[d.col], c.w.J.as<double2>(), ldX, X + (size_t)col * ldX, ldX,
+                            colmax + col);
+    PII_HIP(hipGetLastError());
+  }
+  c.phase_end(3);
+  column_gram(c, kg, gram_mode, (int)D, ldX, colmax);
+  return (int)D;
 }
 
 void volume_matrix_affine_all(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
@@ -295,8 +357,25 @@ void volume_matrix_affine_all(Context& c, const Batch& b, const KGridDev& kg, in
 }
 
 void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
-                           double* H_host, double* V_host) {
-  volume_matrix_affine_all(c, b, kg, gram_mode, V_host, nullptr, H_host);
+                           const int* ndofs, double* H_host, double* V_host) {
+  bool all_affine = true;
+  for (const ObjDesc& d : b.h_objs) all_affine = all_affine && d.dof_kind == PII_DOF_AFFINE;
+  if (all_affine) {
+    volume_matrix_affine_all(c, b, kg, gram_mode, V_host, nullptr, H_host);
+    return;
+  }
+  const int N = b.N;
+  const int na = V_host ? N : 0;
+  const int D = general_gram(c, b, kg, gram_mode, V_host != nullptr, ndofs);
+  const int DH = D - na;
+  const double* G = c.w.H.as<double>();
+  PII_HIP(hipMemcpy2DAsync(H_host, sizeof(double) * DH, G + (size_t)na * D + na,
+                           sizeof(double) * D, sizeof(double) * DH, DH, hipMemcpyDeviceToHost,
+                           c.stream));
+  if (V_host)
+    PII_HIP(hipMemcpy2DAsync(V_host, sizeof(double) * N, G, sizeof(double) * D,
+                             sizeof(double) * N, N, hipMemcpyDeviceToHost, c.stream));
+  PII_HIP(hipStreamSynchronize(c.stream));
 }
 
 void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geoms,

@@ -6,7 +6,9 @@
 // compute_Ak_gradient_triangle_mesh_cuda_wrapper,
 // cpp/src/dof/cuda/triangle_mesh_dof_cuda.cu:387-466), with the maths of
 // cpp/src/dof/affine_dof.cpp:172-322 and cpp/src/dof/triangle_mesh_dof.cpp:76-143.
-// Output: row-major Q x D complex, out[q * D + t].
+// Output: row-major Q x D complex, out[q * D + t], or (for the Gram of
+// derivative columns) column-major, out[t * ld + q], on the representative
+// nodes, then scaled by sqrt(w) into FP32 columns.
 //
 // AffineDoF: one lane per node streams the object's elements (wave-uniform,
 // so through the scalar cache) and keeps the 12 complex sums in registers.
@@ -51,9 +53,10 @@ __device__ __forceinline__ cplx cexpi(double p) {
 }
 
 // AffineDoF (dofs [t, A row-major]): one lane per node q.
+template <bool kColMajor>
 __global__ void __launch_bounds__(256) k_jacobian_affine(const ObjDesc* __restrict__ objs,
                                                          const double* __restrict__ kfull, int Q,
-                                                         double2* __restrict__ out) {
+                                                         int ld, double2* __restrict__ out) {
   const ObjDesc& d = objs[0];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= Q) return;
@@ -176,18 +179,23 @@ __global__ void __launch_bounds__(256) k_jacobian_affine(const ObjDesc* __restri
       }
     }
   }
-  for (int i = 0; i < 12; ++i) out[(size_t)q * 12 + i] = make_double2(g[i].x, g[i].y);
+  for (int i = 0; i < 12; ++i)
+    out[kColMajor ? (size_t)i * ld + q : (size_t)q * 12 + i] = make_double2(g[i].x, g[i].y);
 }
 
 // TriangleMeshDoF (dofs: the world positions of the DoF vertices): one lane
-// per (node, face).
+// per (node, face).  Row-major output: the lanes of a wave take consecutive
+// faces of one node; column-major: consecutive nodes of one face, so that
+// their atomics go to consecutive addresses of one column.
+template <bool kColMajor>
 __global__ void __launch_bounds__(256) k_jacobian_mesh(const ObjDesc* __restrict__ objs,
                                                        const double* __restrict__ kfull, int Q,
-                                                       int D, double* __restrict__ out) {
+                                                       int D, int ld, double* __restrict__ out) {
   const ObjDesc& d = objs[0];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)Q * d.n) return;
-  const int q = (int)(i / d.n), f = (int)(i % d.n);
+  const int q = kColMajor ? (int)(i % Q) : (int)(i / d.n);
+  const int f = kColMajor ? (int)(i / Q) : (int)(i % d.n);
   const double km = kfull[6 * (size_t)Q + q];
   if (km < 1e-10) return;
   int dv[3];
@@ -217,7 +225,6 @@ __global__ void __launch_bounds__(256) k_jacobian_mesh(const ObjDesc* __restrict
   ref_Phi_grad(ke1, ke2, dPa, dPb);
   // coefficients of kappa k: i psi - dPa - dPb, dPa, dPb
   const cplx coef[3] = {{-psi.y - dPa.x - dPb.x, psi.x - dPa.y - dPb.y}, dPa, dPb};
-  double* o = out + 2 * ((size_t)q * D);
   for (int j = 0; j < 3; ++j) {
     const double dk[3] = {kh[1] * m[j][2] - kh[2] * m[j][1], kh[2] * m[j][0] - kh[0] * m[j][2],
                           kh[0] * m[j][1] - kh[1] * m[j][0]};
@@ -225,26 +232,70 @@ __global__ void __launch_bounds__(256) k_jacobian_mesh(const ObjDesc* __restrict
       // T (psi dkappa_c + kappa k_c coef_j)
       const double ux = psi.x * dk[c] + kap * kv[c] * coef[j].x;
       const double uy = psi.y * dk[c] + kap * kv[c] * coef[j].y;
-      atomicAdd(o + 2 * (3 * (size_t)dv[j] + c), T.x * ux - T.y * uy);
-      atomicAdd(o + 2 * (3 * (size_t)dv[j] + c) + 1, T.x * uy + T.y * ux);
+      const size_t t = 3 * (size_t)dv[j] + c;
+      double* o = out + 2 * (kColMajor ? t * ld + q : (size_t)q * D + t);
+      atomicAdd(o, T.x * ux - T.y * uy);
+      atomicAdd(o + 1, T.x * uy + T.y * ux);
     }
   }
 }
 
+// X column t = sqrt(w_q) J[:, t] in FP32 for the representative nodes q < Qh,
+// zero up to ldX, and colmax[t] = the largest |component| of the column (as
+// the form-factor columns).  Grid (ldX / 256 rounded up, D), whole waves.
+__global__ void __launch_bounds__(256) k_jacobian_to_x(const double2* __restrict__ J, int ldJ,
+                                                       const double* __restrict__ krep, int Qh,
+                                                       float2* __restrict__ X, int ldX,
+                                                       unsigned* __restrict__ colmax) {
+  const int t = blockIdx.y;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  float m = 0.0f;
+  if (q < ldX) {
+    float2 x = make_float2(0.0f, 0.0f);
+    if (q < Qh) {
+      const double sw = sqrt(krep[7 * (size_t)Qh + q]);
+      const double2 v = J[(size_t)t * ldJ + q];
+      x = make_float2((float)(v.x * sw), (float)(v.y * sw));
+      m = fmaxf(fabsf(x.x), fabsf(x.y));
+    }
+    X[(size_t)t * ldX + q] = x;
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(colmax + t, __float_as_uint(m));
+}
+
 }  // namespace
+
+void launch_jacobian_columns(hipStream_t s, const ObjDesc* d_obj, int dof_kind, int n_elems,
+                             const double* d_krep, int Qh, int D, double2* d_J, int ldJ,
+                             float2* d_X, int ldX, unsigned* d_colmax) {
+  if (Qh <= 0 || D <= 0) return;
+  if (dof_kind == PII_DOF_AFFINE) {
+    hipLaunchKernelGGL(k_jacobian_affine<true>, dim3((Qh + 255) / 256), dim3(256), 0, s, d_obj,
+                       d_krep, Qh, ldJ, d_J);
+  } else {
+    PII_HIP(hipMemsetAsync(d_J, 0, sizeof(double2) * (size_t)ldJ * D, s));
+    const long long total = (long long)Qh * n_elems;
+    if (total > 0)
+      hipLaunchKernelGGL(k_jacobian_mesh<true>, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                         0, s, d_obj, d_krep, Qh, D, ldJ, reinterpret_cast<double*>(d_J));
+  }
+  hipLaunchKernelGGL(k_jacobian_to_x, dim3((ldX + 255) / 256, D), dim3(256), 0, s, d_J, ldJ,
+                     d_krep, Qh, d_X, ldX, d_colmax);
+}
 
 void launch_jacobian(hipStream_t s, const ObjDesc* d_obj, int type, int dof_kind, int n_elems,
                      const double* d_kfull, int Q, int D, double2* d_out) {
   if (Q <= 0 || D <= 0) return;
   if (dof_kind == PII_DOF_AFFINE) {
-    hipLaunchKernelGGL(k_jacobian_affine, dim3((Q + 255) / 256), dim3(256), 0, s, d_obj, d_kfull,
-                       Q, d_out);
+    hipLaunchKernelGGL(k_jacobian_affine<false>, dim3((Q + 255) / 256), dim3(256), 0, s, d_obj,
+                       d_kfull, Q, 0, d_out);
   } else {
     PII_HIP(hipMemsetAsync(d_out, 0, sizeof(double2) * (size_t)Q * D, s));
     const long long total = (long long)Q * n_elems;
     if (total > 0)
-      hipLaunchKernelGGL(k_jacobian_mesh, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                         d_obj, d_kfull, Q, D, reinterpret_cast<double*>(d_out));
+      hipLaunchKernelGGL(k_jacobian_mesh<false>, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                         0, s, d_obj, d_kfull, Q, D, 0, reinterpret_cast<double*>(d_out));
   }
   (void)type;
 }

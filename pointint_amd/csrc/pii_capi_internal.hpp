@@ -85,11 +85,12 @@ void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, doubl
 void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
                         double* V_host);
 
-// Gauss-Newton Hessian blocks H (12N x 12N, row-major; block (i, j) rows
-// 12 i.., columns 12 j..) of affine-DoF objects; V (may be null) of the same
-// call.
+// Gauss-Newton Hessian blocks H (D x D, D = sum of ndofs, row-major; block
+// (i, j) rows from sum_{i' < i} ndofs[i'], columns likewise, caller order);
+// V (may be null) of the same call.  Affine-only batches use the fast
+// derivative kernels, others the FP64 Jacobian kernels.
 void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
-                           double* H_host, double* V_host);
+                           const int* ndofs, double* H_host, double* V_host);
 
 // V (N x N), grad[(i N + j) 12 + t] = dV_ij/dtheta_i,t (half for i == j) and
 // the Gauss-Newton H of affine-DoF objects from one Gram over the A and

@@ -278,6 +278,15 @@ size_t dtheta_split_doubles(int N, int ldX, int nsplit);
 // FP64, row-major Q x D complex.
 void launch_jacobian(hipStream_t s, const ObjDesc* d_obj, int type, int dof_kind, int n_elems,
                      const double* d_kfull, int Q, int D, double2* d_out);
+// The same Jacobian on the representative nodes (krep, Qh of them) as FP32
+// columns sqrt(w) dA/dtheta_t of X (leading dimension ldX, column maxima
+// colmax), through the FP64 column-major scratch d_J (ldJ >= Qh, D columns).
+void launch_jacobian_columns(hipStream_t s, const ObjDesc* d_obj, int dof_kind, int n_elems,
+                             const double* d_krep, int Qh, int D, double2* d_J, int ldJ,
+                             float2* d_X, int ldX, unsigned* d_colmax);
+// Most columns (A columns and derivative columns together) of one derivative
+// Gram: its result alone is D x D doubles (2 GiB at the limit).
+constexpr long long kMaxDerivativeColumns = 16384;
 // Resident workgroups of the element-splittable kernels (resident_workgroups).
 int ff_fast_slots();
 int series_tri_slots();

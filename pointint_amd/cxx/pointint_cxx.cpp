@@ -294,17 +294,24 @@ KGrid build_kgrid(const std::vector<std::array<double, 3>>& leb_dirs,
 // hessian_ij[i][j] = block (i, j) of the (12n x 12n) Gauss-Newton matrix;
 // hessian_ii / hessian_jj are zero in that form
 // (compute_intersection_volume_multi_object.cu:186-227).
-static void fill_hessian_blocks(IntersectionVolumeMatrixResult& res, int n,
+static void fill_hessian_blocks(IntersectionVolumeMatrixResult& res, const std::vector<int>& nd,
                                 const std::vector<double>& H) {
-  const int D = 12 * n;
-  res.hessian_ii.assign(n, std::vector<MatrixXd>(n, MatrixXd::Zero(12, 12)));
-  res.hessian_jj.assign(n, std::vector<MatrixXd>(n, MatrixXd::Zero(12, 12)));
-  res.hessian_ij.assign(n, std::vector<MatrixXd>(n, MatrixXd(12, 12)));
+  const int n = (int)nd.size();
+  std::vector<size_t> o(n + 1, 0);
+  for (int i = 0; i < n; ++i) o[i + 1] = o[i] + nd[i];
+  const size_t D = o[n];
+  res.hessian_ii.assign(n, std::vector<MatrixXd>(n));
+  res.hessian_jj.assign(n, std::vector<MatrixXd>(n));
+  res.hessian_ij.assign(n, std::vector<MatrixXd>(n));
   for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j)
-      for (int a = 0; a < 12; ++a)
-        for (int c = 0; c < 12; ++c)
-          res.hessian_ij[i][j](a, c) = H[(size_t)(12 * i + a) * D + 12 * j + c];
+    for (int j = 0; j < n; ++j) {
+      res.hessian_ii[i][j] = MatrixXd::Zero(nd[i], nd[i]);
+      res.hessian_jj[i][j] = MatrixXd::Zero(nd[j], nd[j]);
+      MatrixXd& b = res.hessian_ij[i][j];
+      b = MatrixXd(nd[i], nd[j]);
+      for (int a = 0; a < nd[i]; ++a)
+        for (int c = 0; c < nd[j]; ++c) b(a, c) = H[(o[i] + a) * D + o[j] + c];
+    }
 }
 
 IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
@@ -403,7 +410,7 @@ IntersectionVolumeMatrixResult volume_matrix_resident(
           res.grad_matrix[i][j].assign(G.begin() + ((size_t)i * n + j) * 12,
                                        G.begin() + ((size_t)i * n + j + 1) * 12);
     }
-    if (needs_hessian(flags)) fill_hessian_blocks(res, n, H);
+    if (needs_hessian(flags)) fill_hessian_blocks(res, nd, H);
     return res;
   }
   if (!report(pii_volume_matrix(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(), kg,
@@ -446,14 +453,15 @@ IntersectionVolumeMatrixResult volume_matrix_resident(
   if (needs_hessian(flags)) {
     // Gauss-Newton blocks (compute_intersection_volume_multi_object.cpp:236-292):
     // hessian_ij[i][j] from the device Gram of dA/dtheta columns; hessian_ii /
-    // hessian_jj are zero in this form.  AffineDoF objects only.
-    const int D = 12 * n;
-    std::vector<double> H((size_t)D * D);
+    // hessian_jj are zero in this form.
+    size_t D = 0;
+    for (int i = 0; i < n; ++i) D += nd[i];
+    std::vector<double> H(D * D);
     if (!report(pii_volume_matrix_hessian(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(),
                                           kg, g_gram, H.data(), nullptr),
                 "hessian"))
       return res;
-    fill_hessian_blocks(res, n, H);
+    fill_hessian_blocks(res, nd, H);
   }
   return res;
 }

@@ -1,9 +1,10 @@
-"""GPU Gauss-Newton Hessian blocks (AffineDoF) against the FP64 oracle.
+"""GPU Gauss-Newton Hessian blocks (AffineDoF and TriangleMeshDoF) against
+the FP64 oracle.
 
 Reference: the HESSIAN branch of the reference's matrix driver
 (cpp/src/compute_intersection_volume_multi_object.cpp:236-292),
 H_ij = (1/8 pi^3) sum_q w_q Re(dA_i/dtheta_i (dA_j/dtheta_j)^H), assembled here
-from the oracle's per-node affine gradients (`oracle.compute_A_gradient`,
+from the oracle's per-node gradients (`oracle.compute_A_gradient`,
 itself pinned against central differences in test_oracle_pins.py).
 Gate: ||H - H64||_F / ||H64||_F < 1e-5, and every diagonal block to 1e-4 of
 its own norm.
@@ -69,17 +70,62 @@ def test_hessian_matches_oracle(oracle, gram):
     assert np.max(np.abs(V - Vo) / dd) < 1e-5
 
 
-def test_hessian_rejects_vertex_dofs():
+@pytest.mark.parametrize("gram", ["fp64", "fp16x3"])
+def test_hessian_vertex_dofs(oracle, gram):
+    """TriangleMeshDoF objects (3 DoFs per vertex) next to an AffineDoF
+    Gaussian splat: every block of H, including the mixed (mesh, affine)
+    blocks, against H assembled from the oracle's per-node gradients
+    (triangle_mesh_dof.cpp:76-143, affine_dof.cpp:172-322); gate as above."""
+    rng = np.random.default_rng(61)
     V1, F1 = W.icosphere(1)
+    Va = W.star_perturb(V1, rng, 0.1) * 0.45
+    Vb = V1 * 0.35 + np.array([0.2, -0.1, 0.05])
+    Pc, Nn, _ = W.sphere_pointcloud(80)
+    sig = rng.uniform(0.03, 0.06, size=80)
+    wt = rng.uniform(0.5, 1.5, size=80) * 0.01
+    objs = [(P.make_triangle_mesh(Va, F1), oracle.geom_mesh(Va, F1)),
+            (P.make_triangle_mesh(Vb, F1), oracle.geom_mesh(Vb, F1)),
+            (P.make_gaussian_splat(Pc * 0.3, Nn, sig, wt),
+             oracle.geom_gaussians(Pc * 0.3, Nn, sig, wt))]
+    A = W.random_rotation(rng) @ np.diag(rng.uniform(0.8, 1.2, 3))
+    dofs = [(Va + 0.01 * rng.standard_normal(Va.shape)).ravel(),
+            (Vb + 0.01 * rng.standard_normal(Vb.shape)).ravel(),
+            np.concatenate([rng.uniform(-0.1, 0.1, 3), A.ravel()])]
+    kinds = ["trimesh", "trimesh", "affine"]
+    okinds = [oracle.DOF_MESH, oracle.DOF_MESH, oracle.DOF_AFFINE]
+    d, w = oracle.load_lebedev(P.lebedev_path(9))
+    dirs, kmag, wq = oracle.build_kgrid(d, w, 8)
+    kg = P.KGrid(dirs, kmag, wq)
+    K = dirs * kmag[:, None]
+    Gd = np.concatenate([np.stack([np.asarray(oracle.compute_A_gradient(og, K[q], x, kd)).ravel()
+                                   for q in range(len(kmag))])
+                         for (_, og), x, kd in zip(objs, dofs, okinds)], axis=1)
+    H64 = ((Gd.T * wq) @ np.conj(Gd)).real * oracle.SCALE
+    H, V = P.volume_matrix_hessian([g for g, _ in objs], kg, dofs, gram=gram,
+                                   return_volume=True, dof_kinds=kinds)
+    D = sum(x.size for x in dofs)
+    assert H.shape == (D, D)
+    assert np.allclose(H, H.T, rtol=0, atol=1e-12 * np.abs(H).max())
+    e = np.linalg.norm(H - H64) / np.linalg.norm(H64)
+    print(f"{gram}: ||H - H64|| / ||H64|| = {e:.2e}")
+    assert e < 1e-5, e
+    o = np.cumsum([0] + [x.size for x in dofs])
+    for i in range(3):
+        b = H[o[i]:o[i + 1], o[i]:o[i + 1]]
+        b64 = H64[o[i]:o[i + 1], o[i]:o[i + 1]]
+        assert np.linalg.norm(b - b64) / np.linalg.norm(b64) < 1e-4, i
+    Vo, _ = oracle.volume_matrix([og for _, og in objs], dirs, kmag, wq, dofs, okinds)
+    dd = np.sqrt(np.outer(np.diag(Vo), np.diag(Vo)))
+    assert np.max(np.abs(V - Vo) / dd) < 1e-5
+
+
+def test_hessian_rejects_too_many_columns():
+    """30,726 vertex DoFs: above the 16,384 columns of one derivative Gram."""
+    V5, F5 = W.icosphere(5)
     kg = P.kgrid_from_rule(9, 4)
-    g = P.make_triangle_mesh(V1, F1)
-    with pytest.raises(N.PiiError):
-        N.check(N.load().pii_volume_matrix_hessian(
-            g.ctx.handle, 1, np.array([g.handle.value], dtype=np.uintp).ctypes.data,
-            np.array([1], dtype=np.int32).ctypes.data,
-            np.array([V1.ravel().ctypes.data], dtype=np.uintp).ctypes.data,
-            np.array([V1.size], dtype=np.int32).ctypes.data, kg.handle, 0,
-            np.zeros((V1.size, V1.size)).ctypes.data, None))
+    g = P.make_triangle_mesh(V5, F5)
+    with pytest.raises(N.PiiError, match="limit"):
+        P.volume_matrix_hessian([g], kg, [V5.ravel()], dof_kinds=["trimesh"])
 
 
 @pytest.mark.parametrize("case", ["series", "near_singular"])
