@@ -39,7 +39,7 @@ constexpr int kWaves = 4;
 // phasors, then stream the patch's faces (a quarter each, wave-uniform, scalar
 // loads).  Per face and node the lane forms s = Re(F w conj B) and
 // r_j = kappa Re(G_j w conj B); the wave sums r_j k and s khat over its 64
-// nodes and adds them to the face's LDS accumulator.  At the end the
+// nodes (lane swaps, then DPP) and adds them to the face's LDS accumulator.  At the end the
 // accumulators become vertex gradients (cross products with the edges for
 // the dkappa/dv terms) added to the world-vertex gradient.
 __global__ void __launch_bounds__(256) k_adjoint_tri(const ObjDesc* __restrict__ objs,
@@ -114,13 +114,7 @@ __global__ void __launch_bounds__(256) k_adjoint_tri(const ObjDesc* __restrict__
       const float v[12] = {r0 * fx, r0 * fy, r0 * fz, r1 * fx, r1 * fy, r1 * fz,
                            r2 * fx, r2 * fy, r2 * fz,
                            sf * hx, sf * hy, sf * hz};
-      float t[12];
-#pragma unroll
-      for (int i = 0; i < 12; ++i) t[i] = wave_total(v[i]);
-      if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < 12; ++i) acc[f][i] += (double)t[i];
-      }
+      wave_totals12_add(v, acc[f], lane);
     }
   }
   __syncthreads();

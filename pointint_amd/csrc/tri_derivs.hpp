@@ -135,6 +135,49 @@ __device__ __forceinline__ float wave_total(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Adds the wave totals of 12 per-lane values to acc[0..11] (FP64, LDS; one
+// wave owns acc).  Two rounds of lane swaps fold the values pairwise:
+// v_permlane32_swap exchanges lanes 32-63 of its first operand with lanes
+// 0-31 of its second, so one add leaves value 2i's half sums in lanes 0-31
+// and value 2i+1's in lanes 32-63; v_permlane16_swap exchanges the odd rows
+// (16 lanes) of its first operand with the even rows of its second, so after
+// one more add row r of register j holds 16 partial sums of value
+// 4j + {0, 2, 1, 3}[r].  Four DPP steps (quad_perm [1,0,3,2], [2,3,0,1],
+// row_half_mirror, row_mirror) give every lane its row's total, and lanes 0,
+// 16, 32, 48 add the three registers' totals to their accumulators: 9 swaps,
+// 21 adds and 3 LDS atomics in place of 12 separate wave reductions.  Every
+// lane of the wave must be active.
+__device__ __forceinline__ void wave_totals12_add(const float (&v)[12], double* acc, int lane) {
+  float w[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[2 * i]),
+                                                    __float_as_uint(v[2 * i + 1]), false, false);
+    w[i] = __uint_as_float(s[0]) + __uint_as_float(s[1]);
+  }
+  float z[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(w[2 * j]),
+                                                    __float_as_uint(w[2 * j + 1]), false, false);
+    z[j] = __uint_as_float(s[0]) + __uint_as_float(s[1]);
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) z[j] += dpp_move<0xb1, 0xf>(z[j]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) z[j] += dpp_move<0x4e, 0xf>(z[j]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) z[j] += dpp_move<0x141, 0xf>(z[j]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) z[j] += dpp_move<0x140, 0xf>(z[j]);
+  if ((lane & 15) == 0) {
+    const int r = lane >> 4;
+    const int o = r == 1 ? 2 : (r == 2 ? 1 : r);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) atomicAdd(acc + 4 * j + o, (double)z[j]);
+  }
+}
+
 // sinc'(h) = (cos h - sinc h) / h from cos h and sinc h; odd polynomial below
 // |h| = kSincPoly (truncation < 1e-9 there).  Both evaluated, then selected.
 __device__ __forceinline__ float dsinc_from(float h, float cos_h, float sinc_h) {

@@ -2,8 +2,9 @@
 // C++ API (Lebedev 302 x 32 k-grid, unit cube and a copy shifted by
 // (0.3, 0.2, 0.1)).  Prints the 2 x 2 volume matrix, the pairwise volume and
 // its gradient w.r.t. the second cube's affine DoFs, the translation corner
-// of the pair's Gauss-Newton cross Hessian, and A(k) with dA(k)/dtheta of the
-// second cube at one k under an affine map.
+// of the pair's Gauss-Newton cross Hessian, the whole cross Hessian under
+// vertex DoFs, and A(k) with dA(k)/dtheta of the second cube at one k under
+// an affine map.
 //
 //   usage: example_cube_pair <lebedev_029.txt> [fp64|fast]
 #include <cstdio>
@@ -50,6 +51,17 @@ int main(int argc, char** argv) {
   std::printf("hcross_tt");
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b) std::printf(" %.15e", ph.hessian_cross(a, b));
+  std::printf("\n");
+  // the same cross block under vertex DoFs (3 per cube corner)
+  auto md = std::make_shared<TriangleMeshDoF>(V.rows());
+  auto pm = compute_intersection_volume_cuda(geoms[0], geoms[1], md, md, V.data, V2.data, kg,
+                                             ComputationFlags::VOLUME_ONLY |
+                                                 ComputationFlags::HESSIAN);
+  const int dm = 3 * V.rows();
+  if (pm.hessian_cross.rows() != dm || pm.hessian_cross.cols() != dm) return 1;
+  std::printf("hcross_mesh");
+  for (int a = 0; a < dm; ++a)
+    for (int b = 0; b < dm; ++b) std::printf(" %.15e", pm.hessian_cross(a, b));
   std::printf("\n");
   std::printf("self %.15f\n", compute_volume_cpu(geoms[0]));
   // single-k derivatives of the second cube: A(k) and dA(k)/dtheta under a

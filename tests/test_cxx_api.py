@@ -40,6 +40,8 @@ def _run(mode):
             vals["self"] = float(parts[1])
         elif parts[0] == "hcross_tt":
             vals["hcross_tt"] = np.array([float(x) for x in parts[1:10]]).reshape(3, 3)
+        elif parts[0] == "hcross_mesh":
+            vals["hcross_mesh"] = np.array([float(x) for x in parts[1:]]).reshape(24, 24)
         elif parts[0] == "ak":
             vals["ak"] = float(parts[1]) + 1j * float(parts[2])
         elif parts[0] == "gradk":
@@ -69,6 +71,18 @@ def test_cxx_example_cube_pair(oracle, mode, tol):
     c = wq * (z["J"][:, 0] * np.conj(z["J"][:, 1])).real / (8 * np.pi ** 3)
     H_tt = (k * c[:, None]).T @ k
     assert np.allclose(v["hcross_tt"], H_tt, rtol=1e-4, atol=1e-9 * np.abs(H_tt).max())
+    # the cross block under vertex DoFs, (1/8 pi^3) sum_q w_q Re(dA_0 dA_1^H),
+    # from the oracle's per-node vertex gradients on every node
+    g0 = oracle.geom_mesh(z["V0"], z["F"])
+    x0, x1 = z["V0"].ravel(), (z["V0"] + z["shift"]).ravel()
+    d0 = np.stack([np.asarray(oracle.compute_A_gradient(g0, k[q], x0, oracle.DOF_MESH)).ravel()
+                   for q in range(len(kmag))])
+    d1 = np.stack([np.asarray(oracle.compute_A_gradient(g1, k[q], x1, oracle.DOF_MESH)).ravel()
+                   for q in range(len(kmag))])
+    Hm = ((d0.T * wq) @ np.conj(d1)).real / (8 * np.pi ** 3)
+    e = np.linalg.norm(v["hcross_mesh"] - Hm) / np.linalg.norm(Hm)
+    print(f"{mode}: vertex-DoF cross Hessian error {e:.2e}")
+    assert e < max(tol, 1e-6), e
     # single-k A and dA/dtheta of the second cube under an affine map, against
     # the oracle (FP64 on both sides)
     th = np.array([0.05, -0.02, 0.1, 1.1, 0.1, 0.0, -0.05, 0.9, 0.2, 0.0, 0.1, 1.05])
