@@ -139,7 +139,10 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
         tab[s][lane] = half_phasor_a(u);
       }
       __syncthreads();
-      float pA[2] = {0.f, 0.f}, pg[9][2] = {};
+      // per-patch sums: kappa F, S_c = sum_f kappa_f sum_j G_j vref_j,c and
+      // M = sum_f F_f N_f; the node's k and [khat]_x, the same for every
+      // face, are applied once per patch (dA/dA_rc = k_r S_c + ([khat]_x M)_rc)
+      float pA[2] = {0.f, 0.f}, pS[3][2] = {}, pM[9][2] = {};
       for (int f = wave; f < fcnt; f += kWaves) {
         const auto* r = rec + (size_t)kRecFastTri * (fbeg + f);
         const double al = kx * r[0] + ky * r[1] + kz * r[2];
@@ -161,24 +164,32 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_dtheta_affine(
         const auto* t = dr + (size_t)kRecDtheta * (fbeg + f);  // vref (9), N (9)
         pA[0] += kap * F.x;
         pA[1] += kap * F.y;
+        const float2 K0 = make_float2(kap * G0.x, kap * G0.y);
+        const float2 K1 = make_float2(kap * G1.x, kap * G1.y);
+        const float2 K2 = make_float2(kap * G2.x, kap * G2.y);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          // gv_c = sum_j G_j vref_j,c
-          const float gx = G0.x * t[c] + G1.x * t[3 + c] + G2.x * t[6 + c];
-          const float gy = G0.y * t[c] + G1.y * t[3 + c] + G2.y * t[6 + c];
-          // ([khat]_x N)_rc
-          const float D0 = fh[1] * t[9 + 6 + c] - fh[2] * t[9 + 3 + c];
-          const float D1 = fh[2] * t[9 + c] - fh[0] * t[9 + 6 + c];
-          const float D2 = fh[0] * t[9 + 3 + c] - fh[1] * t[9 + c];
-          const float Dr[3] = {D0, D1, D2};
+          // kappa sum_j G_j vref_j,c
+          pS[c][0] += K0.x * t[c] + K1.x * t[3 + c] + K2.x * t[6 + c];
+          pS[c][1] += K0.y * t[c] + K1.y * t[3 + c] + K2.y * t[6 + c];
+        }
 #pragma unroll
-          for (int rr = 0; rr < 3; ++rr) {
-            const float kk = kap * fk[rr];
-            pg[3 * rr + c][0] += kk * gx + F.x * Dr[rr];
-            pg[3 * rr + c][1] += kk * gy + F.y * Dr[rr];
-          }
+        for (int i = 0; i < 9; ++i) {
+          pM[i][0] += F.x * t[9 + i];
+          pM[i][1] += F.y * t[9 + i];
         }
       }
+      float pg[9][2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          // ([khat]_x M)_rc from column c of M
+          const float m0 = pM[c][e], m1 = pM[3 + c][e], m2 = pM[6 + c][e];
+          pg[c][e] = fk[0] * pS[c][e] + (fh[1] * m2 - fh[2] * m1);
+          pg[3 + c][e] = fk[1] * pS[c][e] + (fh[2] * m0 - fh[0] * m2);
+          pg[6 + c][e] = fk[2] * pS[c][e] + (fh[0] * m1 - fh[1] * m0);
+        }
       flush(pA, pg);
       __syncthreads();
     }
