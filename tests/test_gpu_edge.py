@@ -8,6 +8,8 @@
   others untouched; zero-area (repeated-vertex) faces contribute nothing;
   an object far from the origin (|t| = 40) keeps its accuracy (phases in
   FP64).
+* Shells at |k| = 0 and 1e-12 give exactly zero form factors (the
+  reference's 1e-10 cut) on every path, series included; the adjoint agrees.
 """
 import numpy as np
 import pytest
@@ -140,3 +142,65 @@ def test_disk_bessel_large_arguments(oracle):
         print(f"series={series}: max|dA| / area {e:.2e}, x up to {xmax:.0f}")
         assert e < 5e-7, (series, e)
         assert wrel(A, A64, wq) < 1e-6
+
+
+def test_zero_and_tiny_k_shells(oracle):
+    """A shell-structured grid whose first shells sit at |k| = 0 and 1e-12
+    (below the reference's 1e-10 cut, where A and dA/dtheta are 0:
+    triangle_mesh_dof.cpp:28, affine_dof.cpp:72,181) and 1e-9 (just above
+    it), then ordinary shells up to |k| = 30.  The moment series covers the
+    low shells, the per-element kernels the rest.  An open hemisphere (whose
+    A tends to khat . sum S != 0 as k -> 0, so only the cut gives 0), a
+    closed sphere, disks and Gaussians.  Form factors (fast and FP64) exactly
+    0 on the cut shells and within the usual gates elsewhere; V against the
+    oracle; the vertex-DoF adjoint against the oracle's VJP."""
+    rng = np.random.default_rng(91)
+    d, w = oracle.load_lebedev(P.lebedev_path(9))
+    shells = np.array([0.0, 1e-12, 1e-9, 0.3, 1.0, 3.0, 10.0, 30.0])
+    M = len(w)
+    dirs = np.tile(d, (len(shells), 1))
+    kmag = np.repeat(shells, M)
+    wq = np.tile(w, len(shells)) * np.repeat(rng.uniform(0.5, 1.5, len(shells)), M)
+    kg = P.KGrid(dirs, kmag, wq)
+    V2, F2 = W.icosphere(2)
+    keep = np.all(V2[F2][:, :, 2] >= -1e-12, axis=1)      # faces of the upper half
+    Fh = F2[keep]
+    Pc, Nn, _ = W.sphere_pointcloud(500)
+    rho = rng.uniform(0.02, 0.04, size=500)
+    sig = rng.uniform(0.02, 0.04, size=500)
+    wt = rng.uniform(0.5, 1.5, size=500) * 1e-2
+    objs = [(P.make_triangle_mesh(V2 * 0.3, Fh), oracle.geom_mesh(V2 * 0.3, Fh)),
+            (P.make_triangle_mesh(V2 * 0.35 + 0.1, F2), oracle.geom_mesh(V2 * 0.35 + 0.1, F2)),
+            (P.make_point_cloud(Pc * 0.3, Nn, rho), oracle.geom_disks(Pc * 0.3, Nn, rho)),
+            (P.make_gaussian_splat(Pc * 0.3, Nn, sig, wt),
+             oracle.geom_gaussians(Pc * 0.3, Nn, sig, wt))]
+    cut = kmag < 1e-10
+    Js = []
+    for g, og in objs:
+        A64 = oracle.compute_A_grid(og, dirs, kmag)
+        Js.append(A64)
+        assert np.all(A64[cut] == 0)
+        assert np.abs(A64[kmag == 1e-9]).max() > 0
+        for prec, gate in (("fast", 1e-6), ("fp64", 1e-12)):
+            A = P.compute_form_factor(g, kg, precision=prec)
+            assert np.all(A[cut] == 0), prec
+            e = wrel(A, A64, wq)
+            print(f"{prec}: {e:.2e}")
+            assert e < gate, (prec, e)
+    Vo, _ = oracle.volume_matrix([og for _, og in objs], dirs, kmag, wq)
+    V = P.compute_volume_matrix([g for g, _ in objs], kg, gram="fp16x3")
+    dd = np.sqrt(np.outer(np.diag(Vo), np.diag(Vo)))
+    assert np.max(np.abs(V - Vo) / dd) < 1e-5
+    # vertex-DoF adjoint of L = V_01 over the two meshes
+    ms = objs[:2]
+    xs = [(V2 * 0.3).ravel(), (V2 * 0.35 + 0.1).ravel()]
+    G = np.array([[0.0, 1.0], [0.0, 0.0]])
+    grads, _ = P.volume_matrix_vjp([g for g, _ in ms], kg, G, xs, ["trimesh"] * 2)
+    J = np.stack([oracle.compute_A_grid(og, dirs, kmag, x, oracle.DOF_MESH)
+                  for (_, og), x in zip(ms, xs)], axis=1)
+    C = G + G.T
+    for i in range(2):
+        r = oracle.vjp(ms[i][1], xs[i], oracle.DOF_MESH, dirs, kmag, wq, J @ C[:, i])
+        e = np.linalg.norm(grads[i] - r) / np.linalg.norm(r)
+        print(f"vjp {i}: {e:.2e}")
+        assert e < 1e-5, (i, e)
