@@ -27,6 +27,7 @@
 // gradients and adds them to the world-vertex gradient.
 #include "device_common.hpp"
 #include "pii_internal.hpp"
+#include "series_common.hpp"
 
 namespace pii {
 
@@ -90,38 +91,33 @@ __global__ void __launch_bounds__(kLanes) k_series_gamma(const ObjDesc* __restri
     out[P + 1 + m] = (double)(m + 3) * G[m + 1] * kGammaCoef.c[m + 1] * kInv8Pi3;
 }
 
-__global__ void __launch_bounds__(256) k_series_adjoint_tri(
-    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
-    const double* __restrict__ gam, double* __restrict__ gworld, const int* __restrict__ goff) {
-  const ObjDesc& d = objs[blockIdx.y];
-  if (d.nser <= 0) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= d.n) return;
-  const double* r = d.mrec + (size_t)kRecMoment * e;
-  const double x0 = r[0], y0 = r[1], z0 = r[2];
-  const double x1 = r[3], y1 = r[4], z1 = r[5];
-  const double x2 = r[6], y2 = r[7], z2 = r[8];
-  const double Sx = r[9], Sy = r[10], Sz = r[11];
-  double G0[3] = {0, 0, 0}, G1[3] = {0, 0, 0}, G2[3] = {0, 0, 0}, Sd[3] = {0, 0, 0};
-  const auto* kr = as_const(krep);
-  const auto* ga = as_const(gam) + (size_t)blockIdx.y * Mh * kGammaStride;
+// Direction loop of one face at series order PP (the object's order: its
+// forward series stops there too).  Returns the sums behind the face's
+// vertex gradients: Gj = sum_d (S.d) (dH/dw_j-series) d and Sd = sum_d gs d.
+template <int PP>
+__device__ __forceinline__ void series_adjoint_dirs(const PII_CONST_AS double* __restrict__ kr,
+                                                    int Qh, int Mh,
+                                                    const PII_CONST_AS double* __restrict__ ga,
+                                                    const double (&x)[9], const double (&S)[3],
+                                                    double (&G0)[3], double (&G1)[3],
+                                                    double (&G2)[3], double (&Sd)[3]) {
   for (int j = 0; j < Mh; ++j) {
     const double dx = kr[3 * (size_t)Qh + j], dy = kr[4 * (size_t)Qh + j],
                  dz = kr[5 * (size_t)Qh + j];
     const auto* g = ga + (size_t)j * kGammaStride;
-    const double w0 = dx * x0 + dy * y0 + dz * z0;
-    const double w1 = dx * x1 + dy * y1 + dz * z1;
-    const double w2 = dx * x2 + dy * y2 + dz * z2;
-    const double s = dx * Sx + dy * Sy + dz * Sz;
+    const double w0 = dx * x[0] + dy * x[1] + dz * x[2];
+    const double w1 = dx * x[3] + dy * x[4] + dz * x[5];
+    const double w2 = dx * x[6] + dy * x[7] + dz * x[8];
+    const double s = dx * S[0] + dy * S[1] + dz * S[2];
     double b = 1.0, hab = 1.0, habc = 1.0, D0 = 1.0, D1 = 1.0;
     double gs = g[0], g0 = g[1], g1 = g[1], gt = g[P + 1];
 #pragma unroll
-    for (int n = 1; n <= P; ++n) {
+    for (int n = 1; n <= PP; ++n) {
       b *= w1;
       hab = fma(w0, hab, b);
       habc = fma(w2, habc, hab);
       gs = fma(g[n], habc, gs);
-      if (n < P) {
+      if (n < PP) {
         D0 = fma(w0, D0, habc);
         D1 = fma(w1, D1, habc);
         g0 = fma(g[n + 1], D0, g0);
@@ -135,6 +131,27 @@ __global__ void __launch_bounds__(256) k_series_adjoint_tri(
     G2[0] = fma(sg2, dx, G2[0]); G2[1] = fma(sg2, dy, G2[1]); G2[2] = fma(sg2, dz, G2[2]);
     Sd[0] = fma(gs, dx, Sd[0]); Sd[1] = fma(gs, dy, Sd[1]); Sd[2] = fma(gs, dz, Sd[2]);
   }
+}
+
+__global__ void __launch_bounds__(256) k_series_adjoint_tri(
+    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
+    const double* __restrict__ gam, double* __restrict__ gworld, const int* __restrict__ goff) {
+  const ObjDesc& d = objs[blockIdx.y];
+  if (d.nser <= 0) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.n) return;
+  const double* r = d.mrec + (size_t)kRecMoment * e;
+  const double x[9] = {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8]};
+  const double S[3] = {r[9], r[10], r[11]};
+  double G0[3] = {0, 0, 0}, G1[3] = {0, 0, 0}, G2[3] = {0, 0, 0}, Sd[3] = {0, 0, 0};
+  const auto* kr = as_const(krep);
+  const auto* ga = as_const(gam) + (size_t)blockIdx.y * Mh * kGammaStride;
+#define PII_SADJ_BODY(PP) series_adjoint_dirs<PP>(kr, Qh, Mh, ga, x, S, G0, G1, G2, Sd)
+  PII_SERIES_SWITCH(PII_SADJ_BODY)
+#undef PII_SADJ_BODY
+  const double x0 = x[0], y0 = x[1], z0 = x[2];
+  const double x1 = x[3], y1 = x[4], z1 = x[5];
+  const double x2 = x[6], y2 = x[7], z2 = x[8];
   // area-vector terms: v0: (e1 - e2) x Sd / 2, v1: e2 x Sd / 2, v2: Sd x e1 / 2
   const double e1x = x1 - x0, e1y = y1 - y0, e1z = z1 - z0;
   const double e2x = x2 - x0, e2y = y2 - y0, e2z = z2 - z0;
