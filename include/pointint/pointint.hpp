@@ -18,9 +18,10 @@
 //     Vector3d (std::array<double,3>) with the accessors the drivers use.
 //   * double3 comes from HIP's vector types; <cuda_runtime.h> is never used.
 //   * The "_cuda" names are kept for drop-in compatibility; they run on HIP.
-//   * HESSIAN (Gauss-Newton blocks) is computed for AffineDoF objects; other
-//     DoF kinds print the error to std::cerr and leave the blocks empty, as
-//     the reference reports errors.
+//   * HESSIAN (Gauss-Newton blocks) is computed for AffineDoF and
+//     TriangleMeshDoF objects up to 16384 DoFs per call in all; above that
+//     the error goes to std::cerr and the blocks stay empty, as the reference
+//     reports errors.
 #pragma once
 
 #include <hip/hip_vector_types.h>
