@@ -217,6 +217,28 @@ int pii_volume_matrix_affine_derivatives(pii_context* ctx, int n, const pii_geom
                                          const double* const* dofs, const pii_kgrid* kg,
                                          int gram_mode, double* V, double* grad, double* H);
 
+/* The same one-pass outputs for any DoF kinds (AffineDoF: 12 DoFs,
+ * TriangleMeshDoF: 3 per DoF vertex), each output optional but not all:
+ *   V[n*n]           volume matrix;
+ *   grad[n * D]      D = sum_i ndofs[i]; object i's block is (n x ndofs[i])
+ *                    row-major at offset n * o_i, o_i = sum_{i' < i} ndofs[i'],
+ *                    entry (j, t) = dV_ij/dtheta_i,t (half the full
+ *                    derivative for i == j), the same layout as
+ *                    pii_volume_matrix_affine_derivatives when every object
+ *                    has AffineDoF;
+ *   H[D*D]           as pii_volume_matrix_hessian.
+ * Calls with only AffineDoF objects take the fast derivative kernels; others
+ * the FP64 per-node Jacobian of pii_form_factor_jacobian for every object.
+ * D plus n is limited to 16384 (PII_ERR_INVALID above).
+ * Replaces compute_intersection_volume_matrix_cuda with GRADIENT and/or
+ * HESSIAN for any registered DoF kind: dense dA/dtheta, then grad_matrix and
+ * the Gauss-Newton blocks from it (cpp/src/compute_intersection_volume_multi_object.cu:437-600,
+ * cpp/src/compute_intersection_volume_multi_object.cpp:154-292). */
+int pii_volume_matrix_derivatives(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                                  const int* dof_kinds, const double* const* dofs,
+                                  const int* ndofs, const pii_kgrid* kg, int gram_mode, double* V,
+                                  double* grad, double* H);
+
 /* Adjoint: gradients of L = sum_ij dL_dV[i*n+j] V_ij with respect to every
  * object's DoFs, written to grads[i] (ndofs[i] doubles), without forming
  * dA/dtheta (vector-Jacobian product over the k-nodes).  V (optional, may be

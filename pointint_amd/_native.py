@@ -51,6 +51,7 @@ SIGNATURES = {
     "pii_context_work": ([_P, _P, _I], _I),
     "pii_volume_matrix_hessian": ([_P, _I, _P, _P, _P, _P, _P, _I, _P, _P], _I),
     "pii_volume_matrix_affine_derivatives": ([_P, _I, _P, _P, _P, _I, _P, _P, _P], _I),
+    "pii_volume_matrix_derivatives": ([_P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P], _I),
     "pii_measure_issue_rate": ([_P, _I, _P], _I),
     "pii_kgrid_create": ([_P, _I, _P, _P, _P, _P], _I),
     "pii_kgrid_destroy": ([_P], _I),

@@ -393,13 +393,29 @@ def volume_matrix_hessian(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None,
 
 
 def volume_matrix_derivatives(geoms: Sequence[Geometry], kgrid: KGrid, dofs=None,
-                              gram: str = "fp16x3", hessian: bool = True):
-    """V (n x n), grad (n, n, 12) and, with hessian, the Gauss-Newton H
-    (12n x 12n) of AffineDoF objects from one pass.  grad[i, j] =
-    dV_ij/dtheta_i = (1/8 pi^3) sum_q w_q Re(dA_i/dtheta conj A_j), half the
-    full derivative for i == j (the matrix driver's grad_matrix convention)."""
+                              gram: str = "fp16x3", hessian: bool = True, dof_kinds=None):
+    """V (n x n), grad and, with hessian, the Gauss-Newton H (D x D, D = sum
+    of the DoF counts) from one pass.  grad[i][j] = dV_ij/dtheta_i =
+    (1/8 pi^3) sum_q w_q Re(dA_i/dtheta conj A_j), half the full derivative
+    for i == j (the matrix driver's grad_matrix convention): an (n, n, 12)
+    array for AffineDoF objects (dof_kinds None), else a list of (n, D_i)
+    arrays, one per object."""
     n = len(geoms)
     ctx = geoms[0].ctx
+    if dof_kinds is not None:
+        gh, kh, dh, nh, _keep = _batch_args(geoms, dofs, dof_kinds)
+        sizes = [int(x) for x in nh]
+        D = sum(sizes)
+        V = np.empty((n, n))
+        Gf = np.empty(n * D)
+        H = np.empty((D, D)) if hessian else None
+        N.check(N.load().pii_volume_matrix_derivatives(
+            ctx.handle, n, gh.ctypes.data, kh.ctypes.data, dh.ctypes.data, nh.ctypes.data,
+            kgrid.handle, _GRAM[gram], V.ctypes.data, Gf.ctypes.data,
+            H.ctypes.data if H is not None else None))
+        offs = np.cumsum([0] + sizes)
+        G = [Gf[n * offs[i]:n * offs[i + 1]].reshape(n, sizes[i]) for i in range(n)]
+        return (V, G, H) if hessian else (V, G)
     gh, _kh, dh, _nh, _keep = _batch_args(geoms, dofs, None if dofs is None else ["affine"] * n)
     V = np.empty((n, n))
     G = np.empty((n, n, 12))

@@ -947,6 +947,27 @@ int pii_volume_matrix_hessian(pii_context* ctx, int n, const pii_geometry* const
   });
 }
 
+int pii_volume_matrix_derivatives(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                                  const int* dof_kinds, const double* const* dofs,
+                                  const int* ndofs, const pii_kgrid* kg, int gram_mode, double* V,
+                                  double* grad, double* H) {
+  return guarded([&] {
+    require(ctx && kg, "pii_volume_matrix_derivatives: NULL argument");
+    require(V || grad || H, "pii_volume_matrix_derivatives: no output requested");
+    require(n > 0 && geoms && dof_kinds && dofs && ndofs,
+            "pii_volume_matrix_derivatives: bad arrays");
+    for (int i = 0; i < n; ++i)
+      require(geoms[i] && dofs[i], "pii_volume_matrix_derivatives: NULL object");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    std::vector<const GeometryDev*> gs(n);
+    for (int i = 0; i < n; ++i) gs[i] = &geoms[i]->g;
+    Batch b;
+    b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
+    volume_matrix_derivatives(ctx->c, b, kg->k, gram_mode, ndofs, V, grad, H);
+  });
+}
+
 int pii_volume_matrix_affine_derivatives(pii_context* ctx, int n, const pii_geometry* const* geoms,
                                          const double* const* dofs, const pii_kgrid* kg,
                                          int gram_mode, double* V, double* grad, double* H) {

@@ -356,6 +356,47 @@ void volume_matrix_affine_all(Context& c, const Batch& b, const KGridDev& kg, in
   }
 }
 
+void volume_matrix_derivatives(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                               const int* ndofs, double* V_host, double* grad_host,
+                               double* H_host) {
+  bool all_affine = true;
+  for (const ObjDesc& d : b.h_objs) all_affine = all_affine && d.dof_kind == PII_DOF_AFFINE;
+  if (all_affine) {  // same layouts with 12 DoFs per object
+    volume_matrix_affine_all(c, b, kg, gram_mode, V_host, grad_host, H_host);
+    return;
+  }
+  const int N = b.N;
+  const bool with_A = V_host || grad_host;
+  const int na = with_A ? N : 0;
+  const int D = general_gram(c, b, kg, gram_mode, with_A, ndofs);
+  const int DH = D - na;
+  const double* G = c.w.H.as<double>();
+  if (H_host)
+    PII_HIP(hipMemcpy2DAsync(H_host, sizeof(double) * DH, G + (size_t)na * D + na,
+                             sizeof(double) * D, sizeof(double) * DH, DH, hipMemcpyDeviceToHost,
+                             c.stream));
+  if (V_host)
+    PII_HIP(hipMemcpy2DAsync(V_host, sizeof(double) * N, G, sizeof(double) * D,
+                             sizeof(double) * N, N, hipMemcpyDeviceToHost, c.stream));
+  if (grad_host) {
+    // rows na.. (derivative columns), columns 0..N-1 (A columns): row r of
+    // the copy is dV_(i, .)/dtheta_(i, t) for derivative column r = o_i + t
+    std::vector<double> B((size_t)DH * N);
+    PII_HIP(hipMemcpy2DAsync(B.data(), sizeof(double) * N, G + (size_t)na * D, sizeof(double) * D,
+                             sizeof(double) * N, DH, hipMemcpyDeviceToHost, c.stream));
+    PII_HIP(hipStreamSynchronize(c.stream));
+    size_t o = 0;
+    for (int i = 0; i < N; ++i) {
+      const int Di = ndofs[i];
+      double* gi = grad_host + o * N;  // (N x D_i) block of object i
+      for (int j = 0; j < N; ++j)
+        for (int t = 0; t < Di; ++t) gi[(size_t)j * Di + t] = B[(o + t) * N + j];
+      o += Di;
+    }
+  }
+  PII_HIP(hipStreamSynchronize(c.stream));
+}
+
 void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
                            const int* ndofs, double* H_host, double* V_host) {
   bool all_affine = true;

@@ -92,6 +92,16 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
 void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
                            const int* ndofs, double* H_host, double* V_host);
 
+// V (N x N), per-pair gradients and Gauss-Newton blocks for any DoF kinds from
+// one Gram over the A and derivative columns (each output may be null).
+// grad: object i's (N x ndofs[i]) block at offset N sum_{i' < i} ndofs[i'],
+// entry (j, t) = dV_ij/dtheta_i,t (half the full derivative for i == j); H
+// as volume_matrix_hessian.  Affine-only batches take the fast derivative
+// kernels (the same layouts with 12 DoFs per object).
+void volume_matrix_derivatives(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
+                               const int* ndofs, double* V_host, double* grad_host,
+                               double* H_host);
+
 // V (N x N), grad[(i N + j) 12 + t] = dV_ij/dtheta_i,t (half for i == j) and
 // the Gauss-Newton H of affine-DoF objects from one Gram over the A and
 // dA/dtheta columns; every output may be null.

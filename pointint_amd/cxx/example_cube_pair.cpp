@@ -2,9 +2,9 @@
 // C++ API (Lebedev 302 x 32 k-grid, unit cube and a copy shifted by
 // (0.3, 0.2, 0.1)).  Prints the 2 x 2 volume matrix, the pairwise volume and
 // its gradient w.r.t. the second cube's affine DoFs, the translation corner
-// of the pair's Gauss-Newton cross Hessian, the whole cross Hessian under
-// vertex DoFs, and A(k) with dA(k)/dtheta of the second cube at one k under
-// an affine map.
+// of the pair's Gauss-Newton cross Hessian, the second cube's gradient and
+// the whole cross Hessian under vertex DoFs, and A(k) with dA(k)/dtheta of
+// the second cube at one k under an affine map.
 //
 //   usage: example_cube_pair <lebedev_029.txt> [fp64|fast]
 #include <cstdio>
@@ -55,10 +55,13 @@ int main(int argc, char** argv) {
   // the same cross block under vertex DoFs (3 per cube corner)
   auto md = std::make_shared<TriangleMeshDoF>(V.rows());
   auto pm = compute_intersection_volume_cuda(geoms[0], geoms[1], md, md, V.data, V2.data, kg,
-                                             ComputationFlags::VOLUME_ONLY |
-                                                 ComputationFlags::HESSIAN);
+                                             ComputationFlags::ALL);
   const int dm = 3 * V.rows();
   if (pm.hessian_cross.rows() != dm || pm.hessian_cross.cols() != dm) return 1;
+  if ((int)pm.grad_geom2.size() != dm) return 1;
+  std::printf("gmesh2");
+  for (int t = 0; t < dm; ++t) std::printf(" %.15e", pm.grad_geom2[t]);
+  std::printf("\n");
   std::printf("hcross_mesh");
   for (int a = 0; a < dm; ++a)
     for (int b = 0; b < dm; ++b) std::printf(" %.15e", pm.hessian_cross(a, b));

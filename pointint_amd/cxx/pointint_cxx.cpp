@@ -413,6 +413,43 @@ IntersectionVolumeMatrixResult volume_matrix_resident(
     if (needs_hessian(flags)) fill_hessian_blocks(res, nd, H);
     return res;
   }
+  size_t dtot = 0;
+  for (int i = 0; i < n; ++i) dtot += nd[i];
+  if ((needs_gradient(flags) || needs_hessian(flags)) && dtot + n <= 16384) {
+    // Other DoF kinds, as the reference's GRADIENT / HESSIAN branch: dense
+    // dA/dtheta, then grad_matrix and the Gauss-Newton blocks from one Gram
+    // (compute_intersection_volume_multi_object.cu:437-600).
+    std::vector<double> G(n * dtot), H;
+    if (needs_hessian(flags)) H.resize(dtot * dtot);
+    if (!report(pii_volume_matrix_derivatives(ctx, n, gc.data(), kinds.data(), dp.data(),
+                                              nd.data(), kg, g_gram, res.volume_matrix.data.data(),
+                                              G.data(), H.empty() ? nullptr : H.data()),
+                "compute_intersection_volume_matrix_cuda")) {
+      res.volume_matrix = MatrixXd::Zero(n, n);
+      return res;
+    }
+    print_profile();
+    if (g_ff != PII_FF_FAST &&
+        !report(pii_volume_matrix(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(), kg,
+                                  g_ff, g_gram, res.volume_matrix.data.data()),
+                "compute_intersection_volume_matrix_cuda")) {
+      res.volume_matrix = MatrixXd::Zero(n, n);
+      return res;
+    }
+    if (needs_gradient(flags)) {
+      res.grad_matrix.assign(n, std::vector<VectorXd>(n));
+      size_t o = 0;
+      for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+          const auto b = G.begin() + (ptrdiff_t)(n * o + (size_t)j * nd[i]);
+          res.grad_matrix[i][j].assign(b, b + nd[i]);
+        }
+        o += nd[i];
+      }
+    }
+    if (needs_hessian(flags)) fill_hessian_blocks(res, nd, H);
+    return res;
+  }
   if (!report(pii_volume_matrix(ctx, n, gc.data(), kinds.data(), dp.data(), nd.data(), kg,
                                 g_ff, g_gram, res.volume_matrix.data.data()),
               "compute_intersection_volume_matrix_cuda")) {
@@ -421,7 +458,8 @@ IntersectionVolumeMatrixResult volume_matrix_resident(
   }
   print_profile();
   if (needs_gradient(flags)) {
-    // grad_matrix[i][j] = dV_ij/dtheta_i from one VJP per unordered pair.
+    // More DoFs than one Gram takes: grad_matrix[i][j] = dV_ij/dtheta_i
+    // from one VJP per unordered pair.
     res.grad_matrix.assign(n, std::vector<VectorXd>(n));
     std::vector<double> G((size_t)n * n);
     std::vector<VectorXd> grads(n);

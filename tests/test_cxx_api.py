@@ -40,6 +40,8 @@ def _run(mode):
             vals["self"] = float(parts[1])
         elif parts[0] == "hcross_tt":
             vals["hcross_tt"] = np.array([float(x) for x in parts[1:10]]).reshape(3, 3)
+        elif parts[0] == "gmesh2":
+            vals["gmesh2"] = np.array([float(x) for x in parts[1:]])
         elif parts[0] == "hcross_mesh":
             vals["hcross_mesh"] = np.array([float(x) for x in parts[1:]]).reshape(24, 24)
         elif parts[0] == "ak":
@@ -80,6 +82,11 @@ def test_cxx_example_cube_pair(oracle, mode, tol):
     d1 = np.stack([np.asarray(oracle.compute_A_gradient(g1, k[q], x1, oracle.DOF_MESH)).ravel()
                    for q in range(len(kmag))])
     Hm = ((d0.T * wq) @ np.conj(d1)).real / (8 * np.pi ** 3)
+    # dV01/d(vertices of cube 2) = (1/8 pi^3) sum_q w_q Re(dA_1 conj A_0)
+    gm = ((d1.T * wq) @ np.conj(z["J"][:, 0])).real / (8 * np.pi ** 3)
+    eg = np.linalg.norm(v["gmesh2"] - gm) / np.linalg.norm(gm)
+    print(f"{mode}: vertex-DoF pair gradient error {eg:.2e}")
+    assert eg < max(tol, 1e-6), eg
     e = np.linalg.norm(v["hcross_mesh"] - Hm) / np.linalg.norm(Hm)
     print(f"{mode}: vertex-DoF cross Hessian error {e:.2e}")
     assert e < max(tol, 1e-6), e

@@ -70,12 +70,10 @@ def test_hessian_matches_oracle(oracle, gram):
     assert np.max(np.abs(V - Vo) / dd) < 1e-5
 
 
-@pytest.mark.parametrize("gram", ["fp64", "fp16x3"])
-def test_hessian_vertex_dofs(oracle, gram):
-    """TriangleMeshDoF objects (3 DoFs per vertex) next to an AffineDoF
-    Gaussian splat: every block of H, including the mixed (mesh, affine)
-    blocks, against H assembled from the oracle's per-node gradients
-    (triangle_mesh_dof.cpp:76-143, affine_dof.cpp:172-322); gate as above."""
+def _mixed_scene(oracle):
+    """Two TriangleMeshDoF meshes and an AffineDoF Gaussian splat on a small
+    grid, with the oracle's dense per-node gradients (Q x D) and form factors
+    (Q x 3)."""
     rng = np.random.default_rng(61)
     V1, F1 = W.icosphere(1)
     Va = W.star_perturb(V1, rng, 0.1) * 0.45
@@ -100,6 +98,18 @@ def test_hessian_vertex_dofs(oracle, gram):
     Gd = np.concatenate([np.stack([np.asarray(oracle.compute_A_gradient(og, K[q], x, kd)).ravel()
                                    for q in range(len(kmag))])
                          for (_, og), x, kd in zip(objs, dofs, okinds)], axis=1)
+    J = np.stack([oracle.compute_A_grid(og, dirs, kmag, x, kd)
+                  for (_, og), x, kd in zip(objs, dofs, okinds)], axis=1)
+    return objs, dofs, kinds, okinds, dirs, kmag, wq, kg, Gd, J
+
+
+@pytest.mark.parametrize("gram", ["fp64", "fp16x3"])
+def test_hessian_vertex_dofs(oracle, gram):
+    """TriangleMeshDoF objects (3 DoFs per vertex) next to an AffineDoF
+    Gaussian splat: every block of H, including the mixed (mesh, affine)
+    blocks, against H assembled from the oracle's per-node gradients
+    (triangle_mesh_dof.cpp:76-143, affine_dof.cpp:172-322); gate as above."""
+    objs, dofs, kinds, okinds, dirs, kmag, wq, kg, Gd, _ = _mixed_scene(oracle)
     H64 = ((Gd.T * wq) @ np.conj(Gd)).real * oracle.SCALE
     H, V = P.volume_matrix_hessian([g for g, _ in objs], kg, dofs, gram=gram,
                                    return_volume=True, dof_kinds=kinds)
@@ -117,6 +127,38 @@ def test_hessian_vertex_dofs(oracle, gram):
     Vo, _ = oracle.volume_matrix([og for _, og in objs], dirs, kmag, wq, dofs, okinds)
     dd = np.sqrt(np.outer(np.diag(Vo), np.diag(Vo)))
     assert np.max(np.abs(V - Vo) / dd) < 1e-5
+
+
+def test_derivatives_vertex_dofs(oracle):
+    """One pass for any DoF kinds (`pii_volume_matrix_derivatives`): V,
+    grad[i][j] = (1/8 pi^3) sum_q w_q Re(dA_i conj A_j) for every ordered
+    pair (half the full derivative on the diagonal, the matrix driver's
+    convention) and H, against the oracle's dense gradients and form
+    factors; the gradients also against the VJP of L = V_01 (an independent
+    route: the adjoint kernels)."""
+    objs, dofs, kinds, okinds, dirs, kmag, wq, kg, Gd, J = _mixed_scene(oracle)
+    n = len(objs)
+    V, G, H = P.volume_matrix_derivatives([g for g, _ in objs], kg, dofs, gram="fp16x3",
+                                          dof_kinds=kinds)
+    o = np.cumsum([0] + [x.size for x in dofs])
+    for i in range(n):
+        Gi64 = oracle.SCALE * ((Gd[:, o[i]:o[i + 1]].T * wq) @ np.conj(J)).real.T  # (n, D_i)
+        assert G[i].shape == (n, dofs[i].size)
+        e = np.linalg.norm(G[i] - Gi64) / np.linalg.norm(Gi64)
+        print(f"grad block {i}: {e:.2e}")
+        assert e < 1e-5, (i, e)
+    H64 = ((Gd.T * wq) @ np.conj(Gd)).real * oracle.SCALE
+    assert np.linalg.norm(H - H64) / np.linalg.norm(H64) < 1e-5
+    Vo, _ = oracle.volume_matrix([og for _, og in objs], dirs, kmag, wq, dofs, okinds)
+    dd = np.sqrt(np.outer(np.diag(Vo), np.diag(Vo)))
+    assert np.max(np.abs(V - Vo) / dd) < 1e-5
+    # dV_01/dvertex of both meshes through the adjoint
+    ms = [g for g, _ in objs[:2]]
+    grads, _ = P.volume_matrix_vjp(ms, kg, np.array([[0.0, 1.0], [0.0, 0.0]]), dofs[:2],
+                                   ["trimesh"] * 2)
+    for i, j in ((0, 1), (1, 0)):
+        r = grads[i]
+        assert np.linalg.norm(G[i][j] - r) / np.linalg.norm(r) < 1e-5, (i, j)
 
 
 def test_hessian_rejects_too_many_columns():
