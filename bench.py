@@ -192,11 +192,12 @@ def run_diff(args):
         "config": {"workload": "64 icospheres L4 (5,120 tris, vertex DoFs), Lebedev 590 x L=64 "
                                "(Q=37,760), loss sum_{i<j} V_ij", "objects": n, "Q": kg.Q},
         "phase_ms": tm, "overlapping_pair_fraction": frac_overlap,
-        "fd_check": {"vertices": args.fd_vertices, "h": h,
-                     "max_err_rel_to_max_grad": float(max(errs)),
-                     "median_err_rel_to_max_grad": float(np.median(errs)),
-                     "richardson_max_err_rel_to_max_grad": float(max(errs_r)),
-                     "richardson_median_err_rel_to_max_grad": float(np.median(errs_r))},
+        "fd_check": ({"vertices": args.fd_vertices, "h": h,
+                      "max_err_rel_to_max_grad": float(max(errs)),
+                      "median_err_rel_to_max_grad": float(np.median(errs)),
+                      "richardson_max_err_rel_to_max_grad": float(max(errs_r)),
+                      "richardson_median_err_rel_to_max_grad": float(np.median(errs_r))}
+                     if errs else None),
     }
     print(json.dumps(out))
 
