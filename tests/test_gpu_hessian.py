@@ -212,3 +212,20 @@ def test_derivative_series_against_oracle(oracle, case):
                             H64[12 * i:12 * i + 12, 12 * i:12 * i + 12])
         hi /= np.linalg.norm(H64[12 * i:12 * i + 12, 12 * i:12 * i + 12])
         assert gi < 1e-5 and hi < 1e-5, (case, i, gi, hi)
+
+
+def test_derivatives_entry_with_affine_objects(oracle):
+    """With only AffineDoF objects the general entry takes the fast derivative
+    kernels and returns the affine entry's layout: the same numbers, block by
+    block."""
+    rng = np.random.default_rng(51)
+    objs, dofs = _scene(oracle, rng)
+    kg = P.kgrid_from_rule(9, 8)
+    geoms = [g for g, _ in objs]
+    n = len(geoms)
+    V1, G1, H1 = P.volume_matrix_derivatives(geoms, kg, dofs, gram="fp16x3")
+    V2, G2, H2 = P.volume_matrix_derivatives(geoms, kg, dofs, gram="fp16x3",
+                                             dof_kinds=["affine"] * n)
+    assert np.array_equal(V1, V2) and np.array_equal(H1, H2)
+    for i in range(n):
+        assert np.array_equal(G2[i], G1[i])
