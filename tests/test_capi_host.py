@@ -55,3 +55,17 @@ def test_errors_surface_as_status_codes():
     with pytest.raises(N.PiiError):
         core.load_lebedev("/nonexistent/rule.txt")
     assert lib.pii_gauss_legendre(0, 0.0, 1.0, None, None) == N.PII_ERR_INVALID
+
+
+def test_matrix_entries_reject_null_arguments():
+    """Argument checks run before any device call: NULL context / grid or no
+    requested output give PII_ERR_INVALID with a message naming the entry."""
+    lib = N.load()
+    one = (ctypes.c_int * 1)(12)
+    rc = lib.pii_volume_matrix_derivatives(None, 1, None, one, None, one, None, 0, None, None,
+                                           None)
+    assert rc == N.PII_ERR_INVALID
+    assert b"pii_volume_matrix_derivatives" in lib.pii_last_error()
+    rc = lib.pii_volume_matrix_hessian(None, 1, None, one, None, one, None, 0, None, None)
+    assert rc == N.PII_ERR_INVALID
+    assert b"pii_volume_matrix_hessian" in lib.pii_last_error()
