@@ -156,6 +156,14 @@ int pii_geometry_info(const pii_geometry* g, int* type, int* num_elements, int* 
 int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const double* dofs,
                     int ndofs, const pii_kgrid* kg, int precision, double* out);
 
+/* The same A(k_q) written into caller-owned DEVICE memory d_out[2*Q] (device
+ * memory of the context's device, else PII_ERR_INVALID), synchronously: the
+ * contract of the CudaComputeAkFunc plug-ins, which write into a device
+ * double2* (cpp/include/dof/cuda/dof_cuda_interface.hpp:24-30). */
+int pii_form_factor_device(pii_context* ctx, const pii_geometry* g, int dof_kind,
+                           const double* dofs, int ndofs, const pii_kgrid* kg, int precision,
+                           double* d_out);
+
 /* dA(k_q)/dtheta for every node of kg in FP64 (the reference's element
  * derivatives), written to out[2*Q*ndofs] as a row-major Q x ndofs complex
  * matrix (interleaved re, im, host memory): out[2*(q*ndofs + t)].  ndofs as
@@ -169,6 +177,13 @@ int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const
  * in the reference (the volume path contracts instead: pii_volume_matrix_vjp). */
 int pii_form_factor_jacobian(pii_context* ctx, const pii_geometry* g, int dof_kind,
                              const double* dofs, int ndofs, const pii_kgrid* kg, double* out);
+
+/* The same Jacobian written into caller-owned DEVICE memory d_out[2*Q*ndofs]
+ * (row-major Q x ndofs complex), synchronously: the CudaComputeAkGradientFunc
+ * plug-in contract (cpp/include/dof/cuda/dof_cuda_interface.hpp:34-40). */
+int pii_form_factor_jacobian_device(pii_context* ctx, const pii_geometry* g, int dof_kind,
+                                    const double* dofs, int ndofs, const pii_kgrid* kg,
+                                    double* d_out);
 
 /* ---- all-pairs volume matrix ----------------------------------------------- */
 /* V[n*n] (row-major, symmetric) for n objects, each with its own DoF kind and

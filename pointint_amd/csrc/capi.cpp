@@ -836,32 +836,75 @@ int pii_geometry_info(const pii_geometry* g, int* type, int* num_elements, int* 
   });
 }
 
+// Caller memory for the device-output entries: device memory of the
+// context's device (the reference's plug-ins write into a device double2*).
+static void require_device_pointer(const void* p, int device, const char* who) {
+  hipPointerAttribute_t a{};
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) (void)hipGetLastError();
+  if (e != hipSuccess || a.type != hipMemoryTypeDevice || a.device != device)
+    throw Error(PII_ERR_INVALID, std::string(who) + ": out is not device memory of the context's device");
+}
+
+static void form_factor_to(pii_context* ctx, const pii_geometry* g, int dof_kind,
+                           const double* dofs, int ndofs, const pii_kgrid* kg, int precision,
+                           double* out, hipMemcpyKind kind) {
+  const GeometryDev* gp = &g->g;
+  Batch b;
+  if (precision == PII_FF_FP64) {
+    b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
+    const int Q = kg->k.Q;
+    const int ldJ = (int)align_up((size_t)Q, 64);
+    ctx->c.w.J.ensure(sizeof(double2) * (size_t)ldJ);
+    launch_ff_fp64(ctx->c.stream, b.d_objs, 1, kg->k.full.as<double>(), Q,
+                   ctx->c.w.J.as<double2>(), ldJ);
+    PII_HIP(hipGetLastError());
+    PII_HIP(hipMemcpyAsync(out, ctx->c.w.J.p, sizeof(double2) * Q, kind, ctx->c.stream));
+    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+  } else if (precision == PII_FF_FAST) {
+    b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, true, ctx->c.series_grid(kg->k));
+    form_factor_fast_full(ctx->c, b, kg->k, out, kind);
+  } else {
+    throw Error(PII_ERR_INVALID, "unknown form-factor precision");
+  }
+}
+
 int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const double* dofs,
                     int ndofs, const pii_kgrid* kg, int precision, double* out) {
   return guarded([&] {
     require(ctx && g && kg && dofs && out, "pii_form_factor: NULL argument");
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     PII_HIP(hipSetDevice(ctx->c.device));
-    const GeometryDev* gp = &g->g;
-    Batch b;
-    if (precision == PII_FF_FP64) {
-      b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
-      const int Q = kg->k.Q;
-      const int ldJ = (int)align_up((size_t)Q, 64);
-      ctx->c.w.J.ensure(sizeof(double2) * (size_t)ldJ);
-      launch_ff_fp64(ctx->c.stream, b.d_objs, 1, kg->k.full.as<double>(), Q,
-                     ctx->c.w.J.as<double2>(), ldJ);
-      PII_HIP(hipGetLastError());
-      PII_HIP(hipMemcpyAsync(out, ctx->c.w.J.p, sizeof(double2) * Q, hipMemcpyDeviceToHost,
-                             ctx->c.stream));
-      PII_HIP(hipStreamSynchronize(ctx->c.stream));
-    } else if (precision == PII_FF_FAST) {
-      b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, true, ctx->c.series_grid(kg->k));
-      form_factor_fast_full(ctx->c, b, kg->k, out);
-    } else {
-      throw Error(PII_ERR_INVALID, "unknown form-factor precision");
-    }
+    form_factor_to(ctx, g, dof_kind, dofs, ndofs, kg, precision, out, hipMemcpyDeviceToHost);
   });
+}
+
+int pii_form_factor_device(pii_context* ctx, const pii_geometry* g, int dof_kind,
+                           const double* dofs, int ndofs, const pii_kgrid* kg, int precision,
+                           double* d_out) {
+  return guarded([&] {
+    require(ctx && g && kg && dofs && d_out, "pii_form_factor_device: NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    require_device_pointer(d_out, ctx->c.device, "pii_form_factor_device");
+    form_factor_to(ctx, g, dof_kind, dofs, ndofs, kg, precision, d_out,
+                   hipMemcpyDeviceToDevice);
+  });
+}
+
+static void jacobian_to(pii_context* ctx, const pii_geometry* g, int dof_kind, const double* dofs,
+                        int ndofs, const pii_kgrid* kg, double* out, hipMemcpyKind kind) {
+  const GeometryDev* gp = &g->g;
+  Batch b;
+  b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
+  const int Q = kg->k.Q;
+  const size_t bytes = sizeof(double2) * (size_t)Q * ndofs;
+  ctx->c.w.J.ensure(bytes);
+  launch_jacobian(ctx->c.stream, b.d_objs, gp->type, dof_kind, gp->n, kg->k.full.as<double>(), Q,
+                  ndofs, ctx->c.w.J.as<double2>());
+  PII_HIP(hipGetLastError());
+  PII_HIP(hipMemcpyAsync(out, ctx->c.w.J.p, bytes, kind, ctx->c.stream));
+  PII_HIP(hipStreamSynchronize(ctx->c.stream));
 }
 
 int pii_form_factor_jacobian(pii_context* ctx, const pii_geometry* g, int dof_kind,
@@ -870,17 +913,19 @@ int pii_form_factor_jacobian(pii_context* ctx, const pii_geometry* g, int dof_ki
     require(ctx && g && kg && dofs && out, "pii_form_factor_jacobian: NULL argument");
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     PII_HIP(hipSetDevice(ctx->c.device));
-    const GeometryDev* gp = &g->g;
-    Batch b;
-    b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
-    const int Q = kg->k.Q;
-    const size_t bytes = sizeof(double2) * (size_t)Q * ndofs;
-    ctx->c.w.J.ensure(bytes);
-    launch_jacobian(ctx->c.stream, b.d_objs, gp->type, dof_kind, gp->n, kg->k.full.as<double>(),
-                    Q, ndofs, ctx->c.w.J.as<double2>());
-    PII_HIP(hipGetLastError());
-    PII_HIP(hipMemcpyAsync(out, ctx->c.w.J.p, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
-    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+    jacobian_to(ctx, g, dof_kind, dofs, ndofs, kg, out, hipMemcpyDeviceToHost);
+  });
+}
+
+int pii_form_factor_jacobian_device(pii_context* ctx, const pii_geometry* g, int dof_kind,
+                                    const double* dofs, int ndofs, const pii_kgrid* kg,
+                                    double* d_out) {
+  return guarded([&] {
+    require(ctx && g && kg && dofs && d_out, "pii_form_factor_jacobian_device: NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    require_device_pointer(d_out, ctx->c.device, "pii_form_factor_jacobian_device");
+    jacobian_to(ctx, g, dof_kind, dofs, ndofs, kg, d_out, hipMemcpyDeviceToDevice);
   });
 }
 

@@ -97,7 +97,8 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
   PII_HIP(hipGetLastError());
 }
 
-void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out_host) {
+void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out,
+                           hipMemcpyKind kind) {
   hipStream_t s = c.stream;
   const int N = b.N, Qh = kg.Qh, Q = kg.Q;
   const int ldA = (int)align_up((size_t)Qh, 64);
@@ -110,8 +111,7 @@ void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, doubl
   launch_expand(s, c.w.X.as<double2>(), ldA, kg.rep_index.as<int>(), kg.anti_index.as<int>(), Qh,
                 Q, N, c.w.J.as<double2>());
   PII_HIP(hipGetLastError());
-  PII_HIP(hipMemcpyAsync(out_host, c.w.J.p, sizeof(double2) * (size_t)Q * N,
-                         hipMemcpyDeviceToHost, s));
+  PII_HIP(hipMemcpyAsync(out, c.w.J.p, sizeof(double2) * (size_t)Q * N, kind, s));
   PII_HIP(hipStreamSynchronize(s));
 }
 

@@ -81,7 +81,10 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
                               unsigned* d_colmax, Workspace& w, Context* prof);
 
 // Fast path (FP32 transcendentals, FP64 phases and Taylor branch).
-void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out_host);
+// out: host memory (kind hipMemcpyDeviceToHost) or device memory
+// (hipMemcpyDeviceToDevice), Q x N complex.
+void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out,
+                           hipMemcpyKind kind);
 void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
                         double* V_host);
 
