@@ -13,6 +13,7 @@ from .core import (  # noqa: F401
     build_kgrid,
     build_kgrid_arrays,
     compute_form_factor,
+    compute_form_factor_into,
     compute_form_factor_jacobian,
     compute_volume_matrix,
     gauss_legendre,

@@ -302,6 +302,21 @@ def compute_form_factor(geom: Geometry, kgrid: KGrid, dofs=None, dof_kind="affin
     return out[0::2] + 1j * out[1::2]
 
 
+def compute_form_factor_into(out, geom: Geometry, kgrid: KGrid, dofs=None, dof_kind="affine",
+                             precision: str = "fast"):
+    """A(k_q) written into `out`, a contiguous float64 CUDA tensor of 2 Q
+    elements (interleaved re, im) on the context's device, without a host
+    copy (the plug-in functors' contract).  Returns `out`."""
+    if not (out.is_cuda and out.is_contiguous() and out.dtype.is_floating_point
+            and out.element_size() == 8 and out.numel() == 2 * kgrid.Q):
+        raise ValueError("out must be a contiguous float64 CUDA tensor of 2 Q elements")
+    kinds, dl = _dof_args([geom], None if dofs is None else [dofs], [dof_kind])
+    N.check(N.load().pii_form_factor_device(geom.ctx.handle, geom.handle, kinds[0],
+                                            dl[0].ctypes.data, dl[0].shape[0], kgrid.handle,
+                                            _FF[precision], out.data_ptr()))
+    return out
+
+
 def compute_form_factor_jacobian(geom: Geometry, kgrid: KGrid, dofs=None,
                                  dof_kind="affine") -> np.ndarray:
     """dA(k_q)/dtheta for every node (complex128, Q x ndofs) in FP64 with the

@@ -56,3 +56,17 @@ def test_jacobian_device_equals_host():
     # vertex-DoF columns are summed with FP64 atomics (order varies per run)
     d = dev.cpu().numpy()
     assert np.max(np.abs(d - host)) <= 1e-13 * np.max(np.abs(host))
+
+
+def test_python_form_factor_into_tensor():
+    import torch
+    V, F = W.icosphere(2)
+    g = P.make_triangle_mesh(V * 0.4, F)
+    kg = P.kgrid_from_rule(9, 8)
+    out = torch.empty(2 * kg.Q, dtype=torch.float64, device="cuda")
+    P.compute_form_factor_into(out, g, kg)
+    ref = P.compute_form_factor(g, kg)
+    got = out.cpu().numpy()
+    assert np.array_equal(got[0::2] + 1j * got[1::2], ref)
+    with pytest.raises(ValueError):
+        P.compute_form_factor_into(torch.empty(2 * kg.Q, dtype=torch.float32, device="cuda"), g, kg)
