@@ -432,6 +432,17 @@ def main():
         d = np.sqrt(np.abs(np.outer(np.diag(V64), np.diag(V64))))
         err = np.abs(V - V64) / d
         ovl = V64 > 1e-3 * d
+
+        def pair_err(Vx):
+            return float((np.abs(Vx - V64)[ovl] / np.abs(V64[ovl])).max())
+
+        # every Gram mode on the same fast form factors: the fp64 Gram shows
+        # the form factors' own share of the error
+        by_mode = {}
+        for g in ("fp64", "bf16x3", "fp16x3"):
+            Vg = step("fast", g)
+            by_mode[g] = {"max_rel_err_overlapping_pairs": pair_err(Vg),
+                          "max_err_scaled": float((np.abs(Vg - V64) / d).max())}
         ref = {
             "fp64_ms_per_step": (r1 - r0) * 1e3,
             "fp64_ff_ms": t64["form_factor"],
@@ -439,6 +450,8 @@ def main():
             "speedup_vs_fp64_gpu": (r1 - r0) * 1e3 / ms_step,
             "max_err_vs_fp64_scaled": float(err.max()),
             "max_rel_err_overlapping_pairs": float((err[ovl] * d[ovl] / np.abs(V64[ovl])).max()),
+            "overlapping_pairs": int(np.triu(ovl, 1).sum()),
+            "errors_by_gram_mode": by_mode,
         }
         print(f"[bench] fp64 reference: {ref['fp64_ms_per_step']:.2f} ms/step, "
               f"err {ref['max_err_vs_fp64_scaled']:.2e}", file=sys.stderr, flush=True)

@@ -258,7 +258,7 @@ static int series_order(double X) {
 
 void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
                   const GeometryDev* const* geoms, const int* kinds, const double* const* dofs,
-                  const int* ndofs, bool fast, const KGridDev* series_kg) {
+                  const int* ndofs, bool fast, const KGridDev* series_kg, const int* cols) {
   N = n;
   for (int i = 0; i < N; ++i) validate_object(*geoms[i], kinds[i], ndofs[i]);
   // Moment series plan per object: leading shells with |k| R <= kSeriesReach,
@@ -382,7 +382,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     d.nv = g.nv;
     d.npatch = g.npatch;
     d.dof_kind = kinds[i];
-    d.col = i;
+    d.col = cols ? cols[i] : i;
     d.verts = g.verts.as<double>();
     d.faces = g.faces.as<int32_t>();
     d.dofvid = g.dofvid.as<int32_t>();

@@ -97,6 +97,27 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
   PII_HIP(hipGetLastError());
 }
 
+void gram_split_run(hipStream_t s, Workspace& w, int fp16, const float2* d_X, int ldX, int ncols,
+                    const unsigned* d_colmax, std::vector<GramQuery>& qs, double* d_V) {
+  GramPlanHeader h{};
+  gram_plan_layout(qs, h);
+  const int K = 2 * ldX;
+  w.planes.ensure(gram_plane_bytes(fp16, K, ncols));
+  w.part.ensure(sizeof(double) * (size_t)std::max<long long>(h.part_doubles, 1));
+  const size_t plan_bytes = sizeof(GramPlanHeader) + sizeof(GramQuery) * qs.size();
+  w.gplan_h.ensure(plan_bytes);
+  w.gplan.ensure(plan_bytes);
+  std::memcpy(w.gplan_h.p, &h, sizeof(h));
+  std::memcpy(static_cast<char*>(w.gplan_h.p) + sizeof(h), qs.data(),
+              sizeof(GramQuery) * qs.size());
+  PII_HIP(hipMemcpyAsync(w.gplan.p, w.gplan_h.p, plan_bytes, hipMemcpyHostToDevice, s));
+  launch_split_planes(s, fp16, reinterpret_cast<const float*>(d_X), K, ncols, d_colmax,
+                      w.planes.p);
+  launch_gram_planes(s, fp16, w.planes.p, K, ncols, w.gplan.as<GramPlanHeader>(),
+                     gram_planes_grid(h.nitems, 1 << 20), h.nentries, w.part.as<double>(),
+                     d_colmax, d_V);
+}
+
 void form_factor_fast_full(Context& c, const Batch& b, const KGridDev& kg, double* out,
                            hipMemcpyKind kind) {
   hipStream_t s = c.stream;
@@ -136,11 +157,9 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
     launch_gram_fp64_x(s, c.w.X.as<float2>(), Qh, N, ldX, c.w.part.as<double>(), nsplit,
                        c.w.V.as<double>());
   } else if (gram_mode == PII_GRAM_BF16X3 || gram_mode == PII_GRAM_FP16X3) {
-    const int K = 2 * ldX;
-    const int nsplit = gram_split_splits(K, N);
-    c.w.part.ensure(sizeof(double) * (size_t)nsplit * N * N);
-    launch_gram_split(s, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.X.as<float>(), N, K, colmax,
-                      c.w.part.as<double>(), nsplit, c.w.V.as<double>());
+    std::vector<GramQuery> qs{gram_query(0, N, 2 * ldX)};
+    gram_split_run(s, c.w, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.X.as<float2>(), ldX, N,
+                   colmax, qs, c.w.V.as<double>());
   } else {
     throw Error(PII_ERR_INVALID, "unknown Gram mode");
   }
@@ -165,11 +184,9 @@ static void column_gram(Context& c, const KGridDev& kg, int gram_mode, int D, in
     launch_gram_fp64_x(s, c.w.Xt.as<float2>(), kg.Qh, D, ldX, c.w.part.as<double>(), nsplit,
                        c.w.H.as<double>());
   } else {
-    const int K = 2 * ldX;
-    const int nsplit = gram_split_splits(K, D);
-    c.w.part.ensure(sizeof(double) * (size_t)nsplit * D * D);
-    launch_gram_split(s, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.Xt.as<float>(), D, K, colmax,
-                      c.w.part.as<double>(), nsplit, c.w.H.as<double>());
+    std::vector<GramQuery> qs{gram_query(0, D, 2 * ldX)};
+    gram_split_run(s, c.w, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, c.w.Xt.as<float2>(), ldX, D,
+                   colmax, qs, c.w.H.as<double>());
   }
   c.phase_end(2);
   PII_HIP(hipGetLastError());

@@ -1,20 +1,38 @@
 // Split-precision Gram on the gfx950 matrix cores.
 //
-// V = X^T X / (8 pi^3) with X (K x N, column-major, K = 2 Qh) the FP32
-// weighted form factors sqrt(w) [Re A; Im A] written by the fast form-factor
-// kernel.  Each FP32 entry is split into a high and a low 16-bit piece,
-// x = hi + lo (+ O(2^-16 x) for BF16, O(2^-22 x) for FP16), and the product is
-// assembled from three MFMA products hi.hi + hi.lo + lo.hi with FP32
-// accumulation.  The FP32 accumulators are flushed into FP64 every 256 K
-// values (compensates the long FP32 sums), split-K slices are summed in FP64
-// by a deterministic second pass.
+// V = X^T X / (8 pi^3) with X (K x ncols, column-major, K = 2 ldX FP32
+// values per column: Re, Im of sqrt(w) A at each representative node) written
+// by the fast form-factor kernels.  Two passes:
 //
-// FP16 mode scales every column by a power of two 2^e_i (chosen from the
-// column's max |x| so that the max lands near 2^14) to keep the low pieces out
-// of the subnormal range; the reduction divides by 2^(e_i + e_j) exactly.
+// 1. k_split_planes: every FP32 value is split once into NP 16-bit pieces,
+//    x = p0 + p1 (+ p2), written as NP planes in the operand order of
+//    v_mfma_f32_16x16x32_{bf16,f16}: for each group of 16 columns and each
+//    step of 32 K values, 64 lanes x 8 values (lane l: column l % 16, K values
+//    8 (l / 16) .. + 7), 1 KiB per plane, so a wave loads one operand as one
+//    contiguous 1 KiB block.
+//      BF16 (PII_GRAM_BF16X3): three pieces (8 + 8 + 8 significant bits, the
+//        full FP32 significand; BF16 has the FP32 exponent range) and six
+//        products p0 p0' + p0 p1' + p1 p0' + p0 p2' + p2 p0' + p1 p1'; the
+//        dropped terms are below 2^-24 of |x x'|.
+//      FP16 (PII_GRAM_FP16X3): two pieces (11 + 11 bits) of the column scaled
+//        by 2^e (max |x| 2^e in [2^13, 2^14), keeping the low pieces out of
+//        the FP16 subnormals) and three products p0 p0' + p0 p1' + p1 p0'.
+// 2. k_gram_planes: 128 x 128 output tiles of the upper triangle per
+//    workgroup (8 waves of 32 x 64), split over K; the planes of a K step are
+//    staged in LDS (double-buffered) once per workgroup and read by every wave
+//    that needs them.  p0 p0' accumulates in one FP32 MFMA accumulator, the
+//    correction products in a second; the first is added into FP64 registers
+//    every 2 K steps (64 values), the second every 8, so that FP32 rounding
+//    of long sums stays below the rounding of X itself.  The FP64 partials of
+//    the K slices are summed in slice order by k_gram_planes_reduce
+//    (deterministic).
 //
-// Tile: 64 x 64 outputs per workgroup (4 waves, 32 x 32 each as 2 x 2 MFMA
-// 16x16x32 tiles), upper-triangular tile pairs only.
+// One launch serves a list of Grams ("queries"): query q owns columns
+// [c0, c0 + n) of the planes (c0 a multiple of 16) and its own slices, so
+// its result does not depend on the other queries of the launch.  The work
+// items (tile x slice of every query) and V entries are read from a device
+// plan; both kernels loop over them from a fixed grid, so a captured launch
+// can be replayed for another plan of the same capacity.
 #include "device_common.hpp"
 #include "pii_internal.hpp"
 
@@ -26,48 +44,33 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTile = 64;
-constexpr int kKStep = 32;       // K per MFMA 16x16x32
-constexpr int kFlush = 8;        // K steps between FP32 -> FP64 flushes (256 K values)
+constexpr int kTileCols = 128;             // output tile side
+constexpr int kTileGroups = kTileCols / 16;
+constexpr int kWaves = 8;                  // 4 x 2 waves of 32 x 64 outputs
+constexpr int kFlushMain = 2;              // K steps between FP64 flushes of p0 p0'
+constexpr int kFlushCorr = 8;              // ... of the correction products
 
-__device__ __forceinline__ float col_scale(const unsigned* colmax, int i, int N, int fp16) {
-  if (!fp16 || i >= N) return 1.0f;
-  const float m = __uint_as_float(colmax[i]);
-  if (!(m > 0.0f)) return 1.0f;
+__device__ __forceinline__ float col_scale(const unsigned* colmax, int c, int fp16) {
+  if (!fp16) return 1.0f;
+  const float m = __uint_as_float(colmax[c]);
+  if (!(m > 0.0f) || !(m < 3.0e38f)) return 1.0f;
   int e;
   frexpf(m, &e);                 // m = f * 2^e, f in [0.5, 1)
   return ldexpf(1.0f, 14 - e);   // max |x| * scale in [2^13, 2^14)
 }
 
 template <int FP16>
-struct Split;
+struct Piece;
 
 template <>
-struct Split<1> {
-  typedef half8 T;
-  static __device__ __forceinline__ void run(const float (&x)[8], float s, T& hi, T& lo) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const float v = x[i] * s;
-      const _Float16 h = (_Float16)v;
-      hi[i] = h;
-      lo[i] = (_Float16)(v - (float)h);
-    }
-  }
-  static __device__ __forceinline__ f32x4 mma(T a, T b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-  }
-};
-
-template <>
-struct Split<0> {
+struct Piece<0> {
   typedef bf16x8 T;
-  static __device__ __forceinline__ void run(const float (&x)[8], float, T& hi, T& lo) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const __bf16 h = (__bf16)x[i];
-      hi[i] = h;
-      lo[i] = (__bf16)(x[i] - (float)h);
+  static __device__ __forceinline__ void split(float x, __bf16* out, int np) {
+    float r = x;
+    for (int p = 0; p < np; ++p) {
+      const __bf16 h = (__bf16)r;
+      out[p] = h;
+      r = r - (float)h;      // exact: the residual fits the FP32 significand
     }
   }
   static __device__ __forceinline__ f32x4 mma(T a, T b, f32x4 c) {
@@ -75,150 +78,352 @@ struct Split<0> {
   }
 };
 
-template <int FP16>
-__global__ void __launch_bounds__(256) k_gram_split(const float* __restrict__ X, int N, int ldK,
-                                                    const unsigned* __restrict__ colmax,
-                                                    double* __restrict__ part, int kslice) {
-  typedef Split<FP16> S;
-  typedef typename S::T T;
-  const int nt = (N + kTile - 1) / kTile;
-  int t = blockIdx.x, ti = 0;
-  while (t >= nt - ti) { t -= nt - ti; ++ti; }
-  const int tj = ti + t;
-  const int split = blockIdx.y;
-  const int kbeg = split * kslice;
-  const int kend = min(ldK, kbeg + kslice);
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int l16 = lane & 15, kq = lane >> 4;
-  const int row0 = ti * kTile + (wave >> 1) * 32;
-  const int col0 = tj * kTile + (wave & 1) * 32;
-
-  int ri[2], cj[2];
-  float rs[2], cs[2];
-  const float* pa[2];
-  const float* pb[2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    ri[a] = row0 + a * 16 + l16;
-    cj[a] = col0 + a * 16 + l16;
-    rs[a] = col_scale(colmax, ri[a], N, FP16);
-    cs[a] = col_scale(colmax, cj[a], N, FP16);
-    pa[a] = X + (size_t)min(ri[a], N - 1) * ldK + kq * 8;
-    pb[a] = X + (size_t)min(cj[a], N - 1) * ldK + kq * 8;
+template <>
+struct Piece<1> {
+  typedef half8 T;
+  static __device__ __forceinline__ void split(float x, _Float16* out, int np) {
+    float r = x;
+    for (int p = 0; p < np; ++p) {
+      const _Float16 h = (_Float16)r;
+      out[p] = h;
+      r = r - (float)h;
+    }
   }
-  const bool rv[2] = {ri[0] < N, ri[1] < N};
-  const bool cv[2] = {cj[0] < N, cj[1] < N};
+  static __device__ __forceinline__ f32x4 mma(T a, T b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
 
-  f32x4 acc[2][2];
-  double acc64[2][2][4];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
-    }
+template <int FP16> constexpr int pieces() { return FP16 ? 2 : 3; }
 
-  int step = 0;
-  for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
-    T ahi[2], alo[2], bhi[2], blo[2];
+// Planes: np x G groups x S steps x 64 lanes x 16 bytes.  One thread per
+// (group, step, lane) chunk in output order (coalesced 16-byte stores).
+template <int FP16>
+__global__ void __launch_bounds__(256) k_split_planes(const float* __restrict__ X, int ldK,
+                                                      int ncols, int G, int S,
+                                                      const unsigned* __restrict__ colmax,
+                                                      uint4* __restrict__ planes) {
+  constexpr int NP = pieces<FP16>();
+  typedef typename std::conditional<FP16 != 0, _Float16, __bf16>::type E;
+  const size_t nchunk = (size_t)G * S * 64;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nchunk;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(idx & 63);
+    const size_t gs = idx >> 6;
+    const int s = (int)(gs % S);
+    const int g = (int)(gs / S);
+    const int c = g * 16 + (lane & 15);
+    const int k = s * 32 + 8 * (lane >> 4);
+    float x[8];
+    if (c < ncols) {
+      const float4 a = *reinterpret_cast<const float4*>(X + (size_t)c * ldK + k);
+      const float4 b = *reinterpret_cast<const float4*>(X + (size_t)c * ldK + k + 4);
+      const float sc = col_scale(colmax, c, FP16);
+      x[0] = a.x * sc; x[1] = a.y * sc; x[2] = a.z * sc; x[3] = a.w * sc;
+      x[4] = b.x * sc; x[5] = b.y * sc; x[6] = b.z * sc; x[7] = b.w * sc;
+    } else {
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      float xa[8], xb[8];
-      const float4 a0 = *reinterpret_cast<const float4*>(pa[a] + k0);
-      const float4 a1 = *reinterpret_cast<const float4*>(pa[a] + k0 + 4);
-      const float4 b0 = *reinterpret_cast<const float4*>(pb[a] + k0);
-      const float4 b1 = *reinterpret_cast<const float4*>(pb[a] + k0 + 4);
-      xa[0] = rv[a] ? a0.x : 0.f; xa[1] = rv[a] ? a0.y : 0.f; xa[2] = rv[a] ? a0.z : 0.f;
-      xa[3] = rv[a] ? a0.w : 0.f; xa[4] = rv[a] ? a1.x : 0.f; xa[5] = rv[a] ? a1.y : 0.f;
-      xa[6] = rv[a] ? a1.z : 0.f; xa[7] = rv[a] ? a1.w : 0.f;
-      xb[0] = cv[a] ? b0.x : 0.f; xb[1] = cv[a] ? b0.y : 0.f; xb[2] = cv[a] ? b0.z : 0.f;
-      xb[3] = cv[a] ? b0.w : 0.f; xb[4] = cv[a] ? b1.x : 0.f; xb[5] = cv[a] ? b1.y : 0.f;
-      xb[6] = cv[a] ? b1.z : 0.f; xb[7] = cv[a] ? b1.w : 0.f;
-      S::run(xa, rs[a], ahi[a], alo[a]);
-      S::run(xb, cs[a], bhi[a], blo[a]);
+      for (int e = 0; e < 8; ++e) x[e] = 0.0f;
     }
+    E pc[NP][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      E t[NP];
+      Piece<FP16>::split(x[e], t, NP);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) pc[p][e] = t[p];
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      planes[(size_t)p * nchunk + idx] = *reinterpret_cast<const uint4*>(pc[p]);
+  }
+}
+
+__device__ __forceinline__ void decode_tile(int t, int nt, int& ti, int& tj) {
+  ti = 0;
+  while (t >= nt - ti) {
+    t -= nt - ti;
+    ++ti;
+  }
+  tj = ti + t;
+}
+
+__device__ __forceinline__ int find_query(const GramQuery* q, int nq, long long key,
+                                          long long GramQuery::*field) {
+  int lo = 0, hi = nq - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (q[mid].*field <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int FP16>
+__global__ void __launch_bounds__(kWaves * 64) k_gram_planes(const uint4* __restrict__ planes,
+                                                             int G, int S,
+                                                             const GramPlanHeader* __restrict__ plan,
+                                                             double* __restrict__ part) {
+  constexpr int NP = pieces<FP16>();
+  typedef Piece<FP16> Pc;
+  typedef typename Pc::T T;
+  // [buffer][side][group][piece][lane]
+  __shared__ uint4 lds[2][2][kTileGroups][NP][64];
+  const GramQuery* qs = reinterpret_cast<const GramQuery*>(plan + 1);
+  const int nq = plan->nq;
+  const long long nitems = plan->nitems;
+  const size_t nchunk = (size_t)G * S * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;   // rows 32 wr.., cols 64 wc..
+
+  for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int qi = find_query(qs, nq, item, &GramQuery::item0);
+    const GramQuery q = qs[qi];
+    const int local = (int)(item - q.item0);
+    const int t = local / q.nslice, slice = local - t * q.nslice;
+    const int nt = (q.n + kTileCols - 1) / kTileCols;
+    int ti, tj;
+    decode_tile(t, nt, ti, tj);
+    const bool diag = ti == tj;
+    const int s0 = slice * q.slice_len;
+    const int s1 = min(S, s0 + q.slice_len);
+    const int r0 = ti * kTileCols, cc0 = tj * kTileCols;   // query-local
+    const int gA = (q.c0 + r0) >> 4, gB = (q.c0 + cc0) >> 4;
+    // this wave's block intersects the query (and, on a diagonal tile, the
+    // upper triangle)
+    const bool active = (r0 + 32 * wr < q.n) && (cc0 + 64 * wc < q.n) &&
+                        !(diag && 32 * wr >= 64 * wc + 64);
+
+    // chunks of one K step this thread stages: (side, group, piece, lane)
+    constexpr int kChunks = 2 * kTileGroups * NP * 64;
+    constexpr int kPerThread = kChunks / (kWaves * 64);
+    uint4 reg[kPerThread];
+    auto fetch = [&](int s) {
+#pragma unroll
+      for (int r = 0; r < kPerThread; ++r) {
+        const int ch = tid + r * kWaves * 64;
+        const int ln = ch & 63;
+        const int p = (ch >> 6) % NP;
+        const int g = (ch >> 6) / NP % kTileGroups;
+        const int side = (ch >> 6) / NP / kTileGroups;
+        const int col = (side ? cc0 : r0) + 16 * g + (ln & 15);   // query-local
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (col < q.n && !(diag && side)) {
+          const int gg = (side ? gB : gA) + g;
+          v = planes[(size_t)p * nchunk + ((size_t)gg * S + s) * 64 + ln];
+        }
+        reg[r] = v;
+      }
+    };
+    auto stage = [&](int buf) {
+#pragma unroll
+      for (int r = 0; r < kPerThread; ++r) {
+        const int ch = tid + r * kWaves * 64;
+        (&lds[buf][0][0][0][0])[ch] = reg[r];
+      }
+    };
+
+    f32x4 accm[2][4], accc[2][4];
+    double acc64[2][4][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        acc[a][b] = S::mma(ahi[a], bhi[b], acc[a][b]);
-        acc[a][b] = S::mma(ahi[a], blo[b], acc[a][b]);
-        acc[a][b] = S::mma(alo[a], bhi[b], acc[a][b]);
+      for (int b = 0; b < 4; ++b) {
+        accm[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        accc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
       }
-    if (++step == kFlush) {
-      step = 0;
+    const int sideB = diag ? 0 : 1;
+
+    if (s0 < s1) {
+      fetch(s0);
+      stage(0);
+    }
+    __syncthreads();
+    int nm = 0, nc = 0;
+    for (int s = s0; s < s1; ++s) {
+      const int buf = (s - s0) & 1;
+      const bool more = s + 1 < s1;
+      if (more) fetch(s + 1);
+      if (active) {
+        T A[2][NP];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            const uint4 v = lds[buf][0][2 * wr + a][p][lane];
+            A[a][p] = *reinterpret_cast<const T*>(&v);
+          }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          T B[NP];
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            const uint4 v = lds[buf][sideB][4 * wc + b][p][lane];
+            B[p] = *reinterpret_cast<const T*>(&v);
+          }
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            accm[a][b] = Pc::mma(A[a][0], B[0], accm[a][b]);
+            accc[a][b] = Pc::mma(A[a][0], B[1], accc[a][b]);
+            accc[a][b] = Pc::mma(A[a][1], B[0], accc[a][b]);
+            if (NP == 3) {
+              accc[a][b] = Pc::mma(A[a][0], B[NP - 1], accc[a][b]);
+              accc[a][b] = Pc::mma(A[a][NP - 1], B[0], accc[a][b]);
+              accc[a][b] = Pc::mma(A[a][1], B[1], accc[a][b]);
+            }
+          }
+        }
+        if (++nm == kFlushMain || !more) {
+          nm = 0;
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)accm[a][b][r];
+              accm[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+        }
+        if (++nc == kFlushCorr || !more) {
+          nc = 0;
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)accc[a][b][r];
+              accc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+        }
+      }
+      if (more) stage(buf ^ 1);
+      __syncthreads();
+    }
+    if (active) {
+      double* P = part + q.part + (size_t)slice * q.n * q.n;
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];
-          acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
+          for (int r = 0; r < 4; ++r) {
+            // MFMA 16x16 output layout: lane holds column lane % 16, rows 4 (lane / 16) + r
+            const int i = r0 + 32 * wr + 16 * a + 4 * (lane >> 4) + r;
+            const int j = cc0 + 64 * wc + 16 * b + (lane & 15);
+            if (i < q.n && j < q.n && i <= j) P[(size_t)i * q.n + j] = acc64[a][b][r];
+          }
     }
   }
-  double* P = part + (size_t)split * N * N;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        // MFMA 16x16 output layout: lane holds column l16, rows 4*kq + r.
-        const int i = row0 + a * 16 + 4 * kq + r;
-        const int j = col0 + b * 16 + l16;
-        if (i < N && j < N) P[(size_t)i * N + j] = acc64[a][b][r] + (double)acc[a][b][r];
-      }
 }
 
-__global__ void __launch_bounds__(256) k_gram_split_reduce(const double* __restrict__ part,
-                                                           int nsplit, int N,
-                                                           const unsigned* __restrict__ colmax,
-                                                           int fp16, double scale,
-                                                           double* __restrict__ V) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)N * N) return;
-  const int i = (int)(idx / N), j = (int)(idx % N);
-  if (i > j) return;
-  double s = 0.0;
-  for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * N * N + idx];
-  const double si = (double)col_scale(colmax, i, N, fp16);
-  const double sj = (double)col_scale(colmax, j, N, fp16);
-  s *= scale / (si * sj);
-  V[(size_t)i * N + j] = s;
-  V[(size_t)j * N + i] = s;
+// V of every query: the upper triangle summed over the slices in order,
+// scaled and mirrored.
+__global__ void __launch_bounds__(256) k_gram_planes_reduce(const double* __restrict__ part,
+                                                            const GramPlanHeader* __restrict__ plan,
+                                                            const unsigned* __restrict__ colmax,
+                                                            int fp16, double scale,
+                                                            double* __restrict__ V) {
+  const GramQuery* qs = reinterpret_cast<const GramQuery*>(plan + 1);
+  const int nq = plan->nq;
+  const long long total = plan->nentries;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int qi = find_query(qs, nq, e, &GramQuery::entry0);
+    const GramQuery q = qs[qi];
+    const long long loc = e - q.entry0;
+    const int i = (int)(loc / q.n), j = (int)(loc % q.n);
+    if (i > j) continue;
+    const double* P = part + q.part + (size_t)i * q.n + j;
+    const size_t stride = (size_t)q.n * q.n;
+    double s = 0.0;
+    for (int k = 0; k < q.nslice; ++k) s += P[k * stride];
+    const double si = (double)col_scale(colmax, q.c0 + i, fp16);
+    const double sj = (double)col_scale(colmax, q.c0 + j, fp16);
+    s *= scale / (si * sj);
+    V[q.v + (size_t)i * q.n + j] = s;
+    V[q.v + (size_t)j * q.n + i] = s;
+  }
 }
 
 }  // namespace
 
-int gram_split_splits(int K, int N) {
-  const int nt = (N + kTile - 1) / kTile;
-  const int tiles = nt * (nt + 1) / 2;
-  int s = (1024 + tiles - 1) / tiles;               // >= ~1024 workgroups
-  s = std::max(1, std::min(s, K / (kKStep * kFlush * 2)));  // >= 512 K per slice
-  return s;
+int gram_planes_pieces(int fp16) { return fp16 ? 2 : 3; }
+
+size_t gram_plane_bytes(int fp16, int K, int ncols) {
+  const size_t G = (size_t)(ncols + 15) / 16, S = (size_t)K / 32;
+  return (size_t)gram_planes_pieces(fp16) * G * S * 64 * 16;
 }
 
-void launch_gram_split(hipStream_t s, int fp16, const float* d_X, int N, int ldK,
-                       const unsigned* d_colmax, double* d_part, int nsplit, double* d_V) {
-  if (N <= 0) return;
-  const int nt = (N + kTile - 1) / kTile;
-  const int tiles = nt * (nt + 1) / 2;
-  const int unit = kKStep * kFlush;
-  int kslice = (ldK + nsplit - 1) / nsplit;
-  kslice = (kslice + unit - 1) / unit * unit;
+GramQuery gram_query(int c0, int n, int K) {
+  GramQuery q{};
+  q.c0 = c0;
+  q.n = n;
+  const int S = K / 32;
+  const int nt = (n + kTileCols - 1) / kTileCols;
+  q.ntiles = nt * (nt + 1) / 2;
+  // enough slices for about 512 workgroups, at least 8 K steps each, and the
+  // partials' traffic (8 n^2 bytes per slice) at most about half of the
+  // planes' (2 NP K n bytes)
+  long long ns = (512 + q.ntiles - 1) / q.ntiles;
+  ns = std::min<long long>(ns, std::max(1, S / 8));
+  ns = std::min<long long>(ns, std::max<long long>(1, (3LL * K) / (8LL * n)));
+  ns = std::max<long long>(ns, 1);
+  q.slice_len = (int)((S + ns - 1) / ns);
+  q.nslice = (S + q.slice_len - 1) / q.slice_len;
+  return q;
+}
+
+void gram_plan_layout(std::vector<GramQuery>& qs, GramPlanHeader& h) {
+  long long item = 0, part = 0, entry = 0, v = 0;
+  for (GramQuery& q : qs) {
+    q.item0 = item;
+    q.part = part;
+    q.entry0 = entry;
+    q.v = v;
+    item += (long long)q.ntiles * q.nslice;
+    part += (long long)q.nslice * q.n * q.n;
+    entry += (long long)q.n * q.n;
+    v += (long long)q.n * q.n;
+  }
+  h.nq = (int)qs.size();
+  h.nitems = item;
+  h.nentries = entry;
+  h.part_doubles = part;
+}
+
+int gram_planes_grid(long long nitems, int cap) {
+  return (int)std::max<long long>(1, std::min<long long>(nitems, cap));
+}
+
+void launch_split_planes(hipStream_t s, int fp16, const float* d_X, int ldK, int ncols,
+                         const unsigned* d_colmax, void* d_planes) {
+  const int G = (ncols + 15) / 16, S = ldK / 32;
+  const size_t nchunk = (size_t)G * S * 64;
+  const unsigned grid = (unsigned)std::min<size_t>((nchunk + 255) / 256, 16384);
   if (fp16)
-    hipLaunchKernelGGL(k_gram_split<1>, dim3(tiles, nsplit), dim3(256), 0, s, d_X, N, ldK,
-                       d_colmax, d_part, kslice);
+    hipLaunchKernelGGL(k_split_planes<1>, dim3(grid), dim3(256), 0, s, d_X, ldK, ncols, G, S,
+                       d_colmax, static_cast<uint4*>(d_planes));
   else
-    hipLaunchKernelGGL(k_gram_split<0>, dim3(tiles, nsplit), dim3(256), 0, s, d_X, N, ldK,
-                       d_colmax, d_part, kslice);
-  const size_t total = (size_t)N * N;
-  hipLaunchKernelGGL(k_gram_split_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                     d_part, nsplit, N, d_colmax, fp16, kInv8Pi3, d_V);
+    hipLaunchKernelGGL(k_split_planes<0>, dim3(grid), dim3(256), 0, s, d_X, ldK, ncols, G, S,
+                       d_colmax, static_cast<uint4*>(d_planes));
+}
+
+void launch_gram_planes(hipStream_t s, int fp16, const void* d_planes, int ldK, int ncols,
+                        const GramPlanHeader* d_plan, int grid_items, long long nentries_cap,
+                        double* d_part, const unsigned* d_colmax, double* d_V) {
+  const int G = (ncols + 15) / 16, S = ldK / 32;
+  if (fp16)
+    hipLaunchKernelGGL(k_gram_planes<1>, dim3(grid_items), dim3(kWaves * 64), 0, s,
+                       static_cast<const uint4*>(d_planes), G, S, d_plan, d_part);
+  else
+    hipLaunchKernelGGL(k_gram_planes<0>, dim3(grid_items), dim3(kWaves * 64), 0, s,
+                       static_cast<const uint4*>(d_planes), G, S, d_plan, d_part);
+  const unsigned rgrid =
+      (unsigned)std::max<long long>(1, std::min<long long>((nentries_cap + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_gram_planes_reduce, dim3(rgrid), dim3(256), 0, s, d_part, d_plan,
+                     d_colmax, fp16, kInv8Pi3, d_V);
 }
 
 }  // namespace pii

@@ -63,9 +63,10 @@ struct Batch {
   int nseries = 0, Mh = 0, qbeg_min = 0;
   // prof (may be null) records the pre-pass phase with HIP events.  series_kg
   // (fast path only, may be null) enables the moment series on that grid.
+  // cols (may be null: column i) gives object i's output column.
   void build(Workspace& w, hipStream_t s, Context* prof, int n, const GeometryDev* const* geoms,
              const int* kinds, const double* const* dofs, const int* ndofs, bool fast,
-             const KGridDev* series_kg = nullptr);
+             const KGridDev* series_kg = nullptr, const int* cols = nullptr);
   void build(Context& c, int n, const GeometryDev* const* geoms, const int* kinds,
              const double* const* dofs, const int* ndofs, bool fast,
              const KGridDev* series_kg = nullptr) {
@@ -79,6 +80,14 @@ struct Batch {
 void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
                               int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                               unsigned* d_colmax, Workspace& w, Context* prof);
+
+// Split-precision Gram (fp16: FP16x3, else BF16x3) of the column ranges qs
+// (from gram_query; offsets filled here) of X (FP32 complex, ldX nodes per
+// column, ncols columns, column maxima d_colmax): V of query q at d_V + q.v.
+// Enqueued on s; the plan goes up through w's pinned staging, which the next
+// call rewrites (the caller synchronises s between calls).
+void gram_split_run(hipStream_t s, Workspace& w, int fp16, const float2* d_X, int ldX, int ncols,
+                    const unsigned* d_colmax, std::vector<GramQuery>& qs, double* d_V);
 
 // Fast path (FP32 transcendentals, FP64 phases and Taylor branch).
 // out: host memory (kind hipMemcpyDeviceToHost) or device memory

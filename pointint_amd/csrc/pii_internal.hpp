@@ -178,6 +178,8 @@ struct Workspace {
   DevBuf dth, Xt, cmax2, H;  // Gauss-Newton Hessian: records, dA/dtheta columns, maxima, H
   DevBuf dsrec, dspart, dsval;  // derivative series: face records, family sums, shell values
   DevBuf split;              // partial sums of element-split launches (few objects)
+  DevBuf planes, gplan;      // split-precision Gram: 16-bit planes of X, device plan
+  HostBuf gplan_h;           // pinned: the Gram plan of a call
   HostBuf stage;             // pinned: descriptors, prefix sums and DoFs of a call
   HostBuf hout;              // pinned: per-vertex gradients coming back
 };
@@ -314,10 +316,44 @@ double probe_issue_rate(hipStream_t s, int kind, int blocks, int iters, void* d_
 void launch_expand(hipStream_t s, const double2* d_Arep, int ldA, const int* d_rep,
                    const int* d_anti, int Qh, int Q, int N, double2* d_Afull);
 
-// Split-precision MFMA Gram (fp16 != 0: FP16x3 with column scaling; else BF16x3).
-int gram_split_splits(int K, int N);
-void launch_gram_split(hipStream_t s, int fp16, const float* d_X, int N, int ldK,
-                       const unsigned* d_colmax, double* d_part, int nsplit, double* d_V);
+// Split-precision MFMA Gram (fp16 != 0: FP16x3, two FP16 pieces with column
+// scaling and three products; else BF16x3, three BF16 pieces and six
+// products).  One launch computes the Grams of a list of column ranges
+// ("queries") of the same planes; the device plan is a GramPlanHeader
+// followed by its nq GramQuery records.
+struct GramQuery {
+  int c0;            // first column (a multiple of 16)
+  int n;             // columns
+  int ntiles;        // 128 x 128 upper-triangle tiles
+  int nslice;        // K slices
+  int slice_len;     // K steps (32 values) per slice
+  int pad_;
+  long long item0;   // first work item (tile x slice) in the launch
+  long long entry0;  // first V entry (n x n per query) in the launch
+  long long part;    // offset of the query's partials (nslice n^2 doubles)
+  long long v;       // offset of the query's V (n^2 doubles)
+};
+struct GramPlanHeader {
+  int nq;
+  int pad_;
+  long long nitems, nentries, part_doubles;
+};
+int gram_planes_pieces(int fp16);
+// Bytes of the planes of ncols columns of K values (K a multiple of 32).
+size_t gram_plane_bytes(int fp16, int K, int ncols);
+// One query over columns [c0, c0 + n) with K values per column: tiles and
+// K slices (a function of n and K only, so a query's result does not depend
+// on the launch it is part of).
+GramQuery gram_query(int c0, int n, int K);
+// Offsets of the queries' work items, partials and outputs; fills h.
+void gram_plan_layout(std::vector<GramQuery>& qs, GramPlanHeader& h);
+// Workgroups of a launch over nitems work items (at most cap; each loops).
+int gram_planes_grid(long long nitems, int cap);
+void launch_split_planes(hipStream_t s, int fp16, const float* d_X, int ldK, int ncols,
+                         const unsigned* d_colmax, void* d_planes);
+void launch_gram_planes(hipStream_t s, int fp16, const void* d_planes, int ldK, int ncols,
+                        const GramPlanHeader* d_plan, int grid_items, long long nentries_cap,
+                        double* d_part, const unsigned* d_colmax, double* d_V);
 
 // Adjoint: Y = X C (complex columns mixed by a real N x N matrix), then the
 // per-face vertex VJP of triangle objects into world-vertex gradients

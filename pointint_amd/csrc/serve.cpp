@@ -90,9 +90,16 @@ struct Server {
 
   size_t footprint(int n, const std::vector<const GeometryDev*>& gs) const {
     const size_t ldX = align_up((size_t)kg->Qh, 64);
-    size_t b = (size_t)n * ldX * sizeof(float2) * 1;         // X columns
-    const int nsplit = gram_split_splits((int)(2 * ldX), n);
-    b += (size_t)n * n * sizeof(double) * (nsplit + 1);      // partials + V
+    const size_t ncol = align_up((size_t)n, 16);
+    size_t b = ncol * ldX * sizeof(float2);                   // X columns
+    if (gram_mode == PII_GRAM_FP64) {
+      b += (size_t)n * n * sizeof(double) * (gram_fp64_splits(kg->Qh, n) + 1);
+    } else {
+      const int fp16 = gram_mode == PII_GRAM_FP16X3 ? 1 : 0;
+      b += gram_plane_bytes(fp16, (int)(2 * ldX), (int)ncol);  // 16-bit planes
+      b += (size_t)n * n * sizeof(double) *
+           (gram_query(0, n, (int)(2 * ldX)).nslice + 1);       // partials + V
+    }
     for (const GeometryDev* g : gs) {
       b += (size_t)g->n * sizeof(double) * kRecFastTri;       // element records
       if (series) b += (size_t)g->n * sizeof(double) * kRecMoment;  // moment records
@@ -115,28 +122,39 @@ struct Server {
       }
     }
     const int N = (int)gs.size();
+    // Each query's columns start at a multiple of 16 (the Gram's column groups).
+    std::vector<int> cols(N);
+    std::vector<int> c0s;
+    int ncol = 0;
+    {
+      int k = 0;
+      for (Request* r : s.reqs) {
+        c0s.push_back(ncol);
+        for (int i = 0; i < r->n; ++i) cols[k++] = ncol + i;
+        ncol += (int)align_up((size_t)r->n, 16);
+      }
+    }
     s.batch = Batch();
     s.batch.build(s.w, s.stream, nullptr, N, gs.data(), kinds.data(), dp.data(), nd.data(), true,
-                  series ? kg : nullptr);
+                  series ? kg : nullptr, cols.data());
     const int Qh = kg->Qh;
     const int ldX = (int)align_up((size_t)Qh, 64);
-    s.w.X.ensure(sizeof(float2) * (size_t)ldX * N);
-    s.w.colmax.ensure(sizeof(unsigned) * (size_t)N);
+    s.w.X.ensure(sizeof(float2) * (size_t)ldX * ncol);
+    s.w.colmax.ensure(sizeof(unsigned) * (size_t)ncol);
     unsigned* colmax = s.w.colmax.as<unsigned>();
-    PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * N, s.stream));
+    PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * ncol, s.stream));
     launch_form_factors_fast(s.stream, s.batch, *kg, 1, nullptr, 0, s.w.X.as<float2>(), ldX,
                              colmax, s.w, nullptr);
     PII_HIP(hipGetLastError());
-    // One Gram per query over its column range.
     size_t vtot = 0, maxpart = 0;
     for (Request* r : s.reqs) {
       r->v_off = vtot;
       vtot += (size_t)r->n * r->n;
-      const int ns = gram_split_splits(2 * ldX, r->n);
-      maxpart = std::max(maxpart, (size_t)ns * r->n * r->n);
+      if (gram_mode == PII_GRAM_FP64)
+        maxpart = std::max(maxpart, (size_t)gram_fp64_splits(Qh, r->n) * r->n * r->n);
     }
     s.Vdev.ensure(sizeof(double) * std::max<size_t>(vtot, 1));
-    s.w.part.ensure(sizeof(double) * std::max<size_t>(maxpart, 1));
+    if (maxpart) s.w.part.ensure(sizeof(double) * maxpart);
     if (s.pinned_n < vtot) {
       if (s.pinned) (void)hipHostFree(s.pinned);
       s.pinned = nullptr;
@@ -145,21 +163,22 @@ struct Server {
       PII_HIP(hipHostMalloc((void**)&s.pinned, sizeof(double) * want, hipHostMallocDefault));
       s.pinned_n = want;
     }
-    int col = 0;
-    for (Request* r : s.reqs) {
-      const float* Xr = s.w.X.as<float>() + (size_t)col * 2 * ldX;
-      double* Vr = s.Vdev.as<double>() + r->v_off;
-      if (gram_mode == PII_GRAM_FP64) {
-        const int ns = gram_fp64_splits(Qh, r->n);
-        launch_gram_fp64_x(s.stream, reinterpret_cast<const float2*>(Xr), Qh, r->n, ldX,
-                           s.w.part.as<double>(), ns, Vr);
-      } else {
-        const int ns = gram_split_splits(2 * ldX, r->n);
-        launch_gram_split(s.stream, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, Xr, r->n, 2 * ldX,
-                          colmax + col, s.w.part.as<double>(), ns, Vr);
+    if (gram_mode == PII_GRAM_FP64) {
+      for (size_t qi = 0; qi < s.reqs.size(); ++qi) {
+        Request* r = s.reqs[qi];
+        const float2* Xr = s.w.X.as<float2>() + (size_t)c0s[qi] * ldX;
+        launch_gram_fp64_x(s.stream, Xr, Qh, r->n, ldX, s.w.part.as<double>(),
+                           gram_fp64_splits(Qh, r->n), s.Vdev.as<double>() + r->v_off);
+        PII_HIP(hipGetLastError());
       }
+    } else {
+      // one launch for the Grams of all queries of the batch
+      std::vector<GramQuery> qs;
+      for (size_t qi = 0; qi < s.reqs.size(); ++qi)
+        qs.push_back(gram_query(c0s[qi], s.reqs[qi]->n, 2 * ldX));
+      gram_split_run(s.stream, s.w, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, s.w.X.as<float2>(),
+                     ldX, ncol, colmax, qs, s.Vdev.as<double>());
       PII_HIP(hipGetLastError());
-      col += r->n;
     }
     PII_HIP(hipMemcpyAsync(s.pinned, s.Vdev.p, sizeof(double) * vtot, hipMemcpyDeviceToHost,
                            s.stream));
