@@ -284,21 +284,48 @@ int pii_gram(pii_context* ctx, const double* J, int n, const pii_kgrid* kg, int 
  * time, with per-call device allocation).  A server keeps the k-grid and the
  * geometries resident and answers volume-matrix queries -- a scene of n
  * resident triangle geometries, each under its own 12 affine DoFs -- by
- * continuous batching on its own HIP streams.  Queries are admitted against
- * hbm_budget bytes (the reservation of a query lasts from admission to
- * completion); pii_server_submit returns PII_ERR_BUDGET at once for a query
- * that does not fit.  pii_server_wait blocks up to timeout_s seconds (< 0:
- * no limit; PII_ERR_BUSY on timeout), copies V (n*n) and the latency
- * (completion - submission, ms), and releases the query.
- * pii_server_stats: submitted, admitted, refused, completed, batches,
- * objects, reserved bytes, budget bytes. */
+ * continuous batching on its own HIP streams: batches of queries run as one
+ * fused pass each (form factors of all their objects, one Gram launch over
+ * all their column ranges), replayed from hipGraphs captured per batch
+ * capacity (pii_server_set_graphs(srv, 0) launches the same kernels
+ * directly; the results are identical).
+ *
+ * Admission: a query is refused at once with PII_ERR_BUDGET when its HBM
+ * footprint does not fit hbm_budget bytes next to the queries admitted and
+ * not finished, or, with a latency target set (pii_server_set_latency_target,
+ * ms; 0 = none, the default), when its predicted completion -- time since
+ * its arrival plus the outstanding work ahead of it and its own, priced by
+ * the throughput the server measures -- exceeds the target.
+ *
+ * pii_server_submit_at takes the query's arrival time t_arrival (seconds on
+ * the clock of pii_clock_seconds; <= 0: now); latencies are measured from it.
+ * pii_server_wait blocks up to timeout_s seconds (< 0: no limit; PII_ERR_BUSY
+ * on timeout), copies V (n*n) and latency_ms[0] (completion - arrival, ms),
+ * and releases the query; pii_server_wait_times gives times[4] = arrival,
+ * submission, launch, completion (seconds, same clock) instead.
+ * pii_server_stats fills out[0..n): submitted, admitted, refused, completed,
+ * batches, objects, reserved bytes, budget bytes, refused by memory, refused
+ * by the latency target, batches launched from graphs, graphs captured, and
+ * device-clock sums over the batches of the pre-pass, moment-series,
+ * per-face form-factor and Gram phases (ms), the triangle x node evaluations
+ * of the per-face kernel, the triangle x direction passes of the series, and
+ * the current ms-per-triangle estimate of the latency model.
+ * A geometry (or k-grid) used by a queued or running query cannot be
+ * destroyed: pii_geometry_destroy returns PII_ERR_BUSY. */
 typedef struct pii_server pii_server;
+int pii_clock_seconds(double* t);
 int pii_server_create(pii_context* ctx, const pii_kgrid* kg, int gram_mode, double hbm_budget,
                       int max_batch_objects, pii_server** out);
 int pii_server_destroy(pii_server* srv);
+int pii_server_set_latency_target(pii_server* srv, double target_ms);
+int pii_server_set_graphs(pii_server* srv, int enable);
 int pii_server_submit(pii_server* srv, int n, const pii_geometry* const* geoms,
                       const double* affine_dofs, int64_t* id);
+int pii_server_submit_at(pii_server* srv, int n, const pii_geometry* const* geoms,
+                         const double* affine_dofs, double t_arrival, int64_t* id);
 int pii_server_wait(pii_server* srv, int64_t id, double timeout_s, double* V, double* latency_ms);
+int pii_server_wait_times(pii_server* srv, int64_t id, double timeout_s, double* V,
+                          double* times);
 int pii_server_stats(pii_server* srv, double* out, int n);
 
 #ifdef __cplusplus

@@ -69,10 +69,15 @@ SIGNATURES = {
     "pii_volume_matrix": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _P], _I),
     "pii_gram": ([_P, _P, _I, _P, _I, _P], _I),
     "pii_volume_matrix_vjp": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "pii_clock_seconds": ([_P], _I),
     "pii_server_create": ([_P, _P, _I, _D, _I, _P], _I),
     "pii_server_destroy": ([_P], _I),
+    "pii_server_set_latency_target": ([_P, _D], _I),
+    "pii_server_set_graphs": ([_P, _I], _I),
     "pii_server_submit": ([_P, _I, _P, _P, _P], _I),
+    "pii_server_submit_at": ([_P, _I, _P, _P, _D, _P], _I),
     "pii_server_wait": ([_P, C.c_int64, _D, _P, _P], _I),
+    "pii_server_wait_times": ([_P, C.c_int64, _D, _P, _P], _I),
     "pii_server_stats": ([_P, _P, _I], _I),
 }
 

@@ -258,8 +258,10 @@ static int series_order(double X) {
 
 void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
                   const GeometryDev* const* geoms, const int* kinds, const double* const* dofs,
-                  const int* ndofs, bool fast, const KGridDev* series_kg, const int* cols) {
+                  const int* ndofs, bool fast, const KGridDev* series_kg, const int* cols,
+                  int n_cap, bool enqueue_now) {
   N = n;
+  NC = std::max(n, n_cap);
   for (int i = 0; i < N; ++i) validate_object(*geoms[i], kinds[i], ndofs[i]);
   // Moment series plan per object: leading shells with |k| R <= kSeriesReach,
   // when at least kSeriesMinShells of them (below that the per-element
@@ -319,10 +321,13 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     dof_off[i] = total_dofs;
     total_dofs += align_up((size_t)ndofs[i], 32);
   }
-  const size_t obj_bytes = align_up(sizeof(ObjDesc) * N, 256);
-  const size_t pre_bytes = align_up(sizeof(int) * (N + 1), 256);
+  // Laid out for NC >= N descriptors (fixed offsets for a capacity, so that a
+  // captured upload and launches replay for any batch of up to NC objects).
+  const size_t obj_bytes = align_up(sizeof(ObjDesc) * NC, 256);
+  const size_t pre_bytes = align_up(sizeof(int) * (NC + 1), 256);
   const size_t dofs_at = obj_bytes + 2 * pre_bytes;
-  const size_t stage_bytes = dofs_at + sizeof(double) * std::max<size_t>(total_dofs, 1);
+  const size_t dof_room = std::max(total_dofs, (size_t)32 * (n_cap > 0 ? n_cap : 0));
+  stage_bytes = dofs_at + sizeof(double) * std::max<size_t>(dof_room, 1);
   w.stage.ensure(stage_bytes);
   w.misc.ensure(stage_bytes);
   char* hs = w.stage.as<char>();
@@ -421,14 +426,29 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     for (int slot = 1; slot < N; ++slot) qbeg_min = std::min(qbeg_min, h_objs[slot].qbeg);
   }
   std::memcpy(hs, h_objs.data(), sizeof(ObjDesc) * N);
-  std::memcpy(hs + obj_bytes, h_vprefix.data(), sizeof(int) * (N + 1));
-  std::memcpy(hs + obj_bytes + pre_bytes, h_eprefix.data(), sizeof(int) * (N + 1));
-  PII_HIP(hipMemcpyAsync(w.misc.p, hs, stage_bytes, hipMemcpyHostToDevice, s));
+  // empty descriptors past N: no elements, no nodes (every kernel skips them)
+  for (int slot = N; slot < NC; ++slot) {
+    ObjDesc& d = reinterpret_cast<ObjDesc*>(hs)[slot];
+    std::memset(&d, 0, sizeof(d));
+    d.type = PII_GEOM_TRIANGLE;
+    d.dof_kind = PII_DOF_AFFINE;
+    d.qbeg = 1 << 30;
+  }
+  int* hv = reinterpret_cast<int*>(hs + obj_bytes);
+  int* he = reinterpret_cast<int*>(hs + obj_bytes + pre_bytes);
+  std::memcpy(hv, h_vprefix.data(), sizeof(int) * (N + 1));
+  std::memcpy(he, h_eprefix.data(), sizeof(int) * (N + 1));
+  for (int slot = N + 1; slot <= NC; ++slot) {
+    hv[slot] = h_vprefix[N];
+    he[slot] = h_eprefix[N];
+  }
   d_objs = reinterpret_cast<const ObjDesc*>(w.misc.p);
   d_vprefix = reinterpret_cast<const int*>(static_cast<char*>(w.misc.p) + obj_bytes);
   d_eprefix = reinterpret_cast<const int*>(static_cast<char*>(w.misc.p) + obj_bytes + pre_bytes);
   total_v = h_vprefix[N];
   total_e = h_eprefix[N];
+  if (!enqueue_now) return;
+  PII_HIP(hipMemcpyAsync(w.misc.p, hs, stage_bytes, hipMemcpyHostToDevice, s));
   if (prof) {
     prof->begin_call();
     prof->phase_begin(0);
@@ -589,6 +609,7 @@ int pii_kgrid_create(pii_context* ctx, int Q, const double* dirs, const double* 
     PII_HIP(hipSetDevice(ctx->c.device));
     auto kg = std::make_unique<pii_kgrid>();
     kg->ctx = ctx;
+    kg->device = ctx->c.device;
     KGridDev& k = kg->k;
     k.Q = Q;
     k.h_dirs.assign(dirs, dirs + 3 * (size_t)Q);
@@ -628,7 +649,9 @@ int pii_kgrid_create(pii_context* ctx, int Q, const double* dirs, const double* 
 int pii_kgrid_destroy(pii_kgrid* kg) {
   return guarded([&] {
     if (!kg) return;
-    (void)hipSetDevice(kg->ctx->c.device);
+    if (kg->refs.load() > 0)
+      throw Error(PII_ERR_BUSY, "pii_kgrid_destroy: the k-grid is in use by a server");
+    (void)hipSetDevice(kg->device);
     delete kg;
   });
 }
@@ -689,6 +712,7 @@ int pii_geometry_create_mesh(pii_context* ctx, const double* V, int nv, const in
     PII_HIP(hipSetDevice(ctx->c.device));
     auto g = std::make_unique<pii_geometry>();
     g->ctx = ctx;
+    g->device = ctx->c.device;
     GeometryDev& d = g->g;
     d.type = PII_GEOM_TRIANGLE;
     d.n = nf;
@@ -712,6 +736,7 @@ int pii_geometry_create_tris(pii_context* ctx, const double* tri, const int32_t*
     PII_HIP(hipSetDevice(ctx->c.device));
     auto g = std::make_unique<pii_geometry>();
     g->ctx = ctx;
+    g->device = ctx->c.device;
     GeometryDev& d = g->g;
     d.type = PII_GEOM_TRIANGLE;
     d.n = nt;
@@ -780,6 +805,7 @@ static int create_surfels(pii_context* ctx, int type, const double* el, int n, p
     PII_HIP(hipSetDevice(ctx->c.device));
     auto g = std::make_unique<pii_geometry>();
     g->ctx = ctx;
+    g->device = ctx->c.device;
     GeometryDev& d = g->g;
     d.type = type;
     d.n = n;
@@ -822,7 +848,10 @@ int pii_geometry_create_gaussians(pii_context* ctx, const double* el, int n, pii
 int pii_geometry_destroy(pii_geometry* g) {
   return guarded([&] {
     if (!g) return;
-    (void)hipSetDevice(g->ctx->c.device);
+    if (g->refs.load() > 0)
+      throw Error(PII_ERR_BUSY, "pii_geometry_destroy: the geometry is in use by " +
+                                    std::to_string(g->refs.load()) + " serving queries");
+    (void)hipSetDevice(g->device);
     delete g;
   });
 }

@@ -112,7 +112,7 @@ void gram_split_run(hipStream_t s, Workspace& w, int fp16, const float2* d_X, in
               sizeof(GramQuery) * qs.size());
   PII_HIP(hipMemcpyAsync(w.gplan.p, w.gplan_h.p, plan_bytes, hipMemcpyHostToDevice, s));
   launch_split_planes(s, fp16, reinterpret_cast<const float*>(d_X), K, ncols, d_colmax,
-                      w.planes.p);
+                      w.planes.p, w.gplan.as<GramPlanHeader>());
   launch_gram_planes(s, fp16, w.planes.p, K, ncols, w.gplan.as<GramPlanHeader>(),
                      gram_planes_grid(h.nitems, 1 << 20), h.nentries, w.part.as<double>(),
                      d_colmax, d_V);

@@ -98,15 +98,21 @@ template <int FP16> constexpr int pieces() { return FP16 ? 2 : 3; }
 
 // Planes: np x G groups x S steps x 64 lanes x 16 bytes.  One thread per
 // (group, step, lane) chunk in output order (coalesced 16-byte stores).
+// Columns at or beyond the plan's ncols (when a plan is given) are not
+// converted: the planes are laid out for G groups, the launch covers the
+// columns the plan's queries use.
 template <int FP16>
 __global__ void __launch_bounds__(256) k_split_planes(const float* __restrict__ X, int ldK,
                                                       int ncols, int G, int S,
                                                       const unsigned* __restrict__ colmax,
+                                                      const GramPlanHeader* __restrict__ plan,
                                                       uint4* __restrict__ planes) {
   constexpr int NP = pieces<FP16>();
   typedef typename std::conditional<FP16 != 0, _Float16, __bf16>::type E;
   const size_t nchunk = (size_t)G * S * 64;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nchunk;
+  if (plan) ncols = min(ncols, plan->ncols);
+  const size_t nconv = (size_t)((ncols + 15) / 16) * S * 64;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nconv;
        idx += (size_t)gridDim.x * blockDim.x) {
     const int lane = (int)(idx & 63);
     const size_t gs = idx >> 6;
@@ -377,7 +383,9 @@ GramQuery gram_query(int c0, int n, int K) {
 
 void gram_plan_layout(std::vector<GramQuery>& qs, GramPlanHeader& h) {
   long long item = 0, part = 0, entry = 0, v = 0;
+  int ncols = 0;
   for (GramQuery& q : qs) {
+    ncols = std::max(ncols, q.c0 + q.n);
     q.item0 = item;
     q.part = part;
     q.entry0 = entry;
@@ -388,6 +396,7 @@ void gram_plan_layout(std::vector<GramQuery>& qs, GramPlanHeader& h) {
     v += (long long)q.n * q.n;
   }
   h.nq = (int)qs.size();
+  h.ncols = ncols;
   h.nitems = item;
   h.nentries = entry;
   h.part_doubles = part;
@@ -398,16 +407,16 @@ int gram_planes_grid(long long nitems, int cap) {
 }
 
 void launch_split_planes(hipStream_t s, int fp16, const float* d_X, int ldK, int ncols,
-                         const unsigned* d_colmax, void* d_planes) {
+                         const unsigned* d_colmax, void* d_planes, const GramPlanHeader* d_plan) {
   const int G = (ncols + 15) / 16, S = ldK / 32;
   const size_t nchunk = (size_t)G * S * 64;
   const unsigned grid = (unsigned)std::min<size_t>((nchunk + 255) / 256, 16384);
   if (fp16)
     hipLaunchKernelGGL(k_split_planes<1>, dim3(grid), dim3(256), 0, s, d_X, ldK, ncols, G, S,
-                       d_colmax, static_cast<uint4*>(d_planes));
+                       d_colmax, d_plan, static_cast<uint4*>(d_planes));
   else
     hipLaunchKernelGGL(k_split_planes<0>, dim3(grid), dim3(256), 0, s, d_X, ldK, ncols, G, S,
-                       d_colmax, static_cast<uint4*>(d_planes));
+                       d_colmax, d_plan, static_cast<uint4*>(d_planes));
 }
 
 void launch_gram_planes(hipStream_t s, int fp16, const void* d_planes, int ldK, int ncols,

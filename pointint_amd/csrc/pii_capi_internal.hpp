@@ -2,6 +2,7 @@
 // fast-path drivers.
 #pragma once
 
+#include <atomic>
 #include <cstdint>
 #include <exception>
 #include <new>
@@ -14,13 +15,21 @@
 struct pii_context {
   pii::Context c;
 };
+// A k-grid or geometry keeps its device id so that destroying it never
+// touches its context (which the caller may have destroyed first); refs
+// counts the serving queries (queued or running) that read it, and destroy
+// refuses while any does.
 struct pii_kgrid {
   pii::KGridDev k;
   pii_context* ctx = nullptr;
+  int device = 0;
+  std::atomic<int> refs{0};
 };
 struct pii_geometry {
   pii::GeometryDev g;
   pii_context* ctx = nullptr;
+  int device = 0;
+  std::atomic<int> refs{0};
 };
 
 namespace pii {
@@ -50,6 +59,8 @@ int guarded(F&& f) {
 // pre-pass (world vertices + element records) launched on the context stream.
 struct Batch {
   int N = 0;
+  int NC = 0;                     // descriptors laid out (>= N; empty ones past N)
+  size_t stage_bytes = 0;         // bytes of the staging upload
   std::vector<int> order;         // slot -> caller index (largest first)
   std::vector<ObjDesc> h_objs;    // per slot
   std::vector<int> h_vprefix, h_eprefix;
@@ -63,10 +74,15 @@ struct Batch {
   int nseries = 0, Mh = 0, qbeg_min = 0;
   // prof (may be null) records the pre-pass phase with HIP events.  series_kg
   // (fast path only, may be null) enables the moment series on that grid.
-  // cols (may be null: column i) gives object i's output column.
+  // cols (may be null: column i) gives object i's output column.  n_cap > n
+  // lays the upload out for n_cap descriptors (empty ones past n) with 32
+  // DoF doubles per object of room.  enqueue_now = false only prepares the
+  // pinned staging (w.stage, stage_bytes bytes, for w.misc) and the device
+  // pointers: the caller uploads it and launches the pre-pass.
   void build(Workspace& w, hipStream_t s, Context* prof, int n, const GeometryDev* const* geoms,
              const int* kinds, const double* const* dofs, const int* ndofs, bool fast,
-             const KGridDev* series_kg = nullptr, const int* cols = nullptr);
+             const KGridDev* series_kg = nullptr, const int* cols = nullptr, int n_cap = 0,
+             bool enqueue_now = true);
   void build(Context& c, int n, const GeometryDev* const* geoms, const int* kinds,
              const double* const* dofs, const int* ndofs, bool fast,
              const KGridDev* series_kg = nullptr) {

@@ -228,6 +228,17 @@ struct ObjDesc;
 void launch_prepass(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix, int total_v,
                     const int* d_eprefix, int total_e, int N, int fast);
 
+// The same pre-pass from a fixed grid (grid-stride loops; totals read from
+// the device prefix sums d_vprefix[N], d_eprefix[N]): replayable in a graph
+// for any batch laid out with N descriptors.
+void launch_prepass_fixed(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix,
+                          const int* d_eprefix, int N, int fast, int grid);
+// Zeroes n 32-bit words (a kernel: graph-safe, unlike a memset node).
+void launch_clear_u32(hipStream_t s, unsigned* d_p, int n);
+// Writes the device's constant 100 MHz clock to d_out[i] when the stream
+// reaches it (phase timing inside captured graphs).
+void launch_stamp(hipStream_t s, unsigned long long* d_out, int i);
+
 // Plain FP64 form factor (reference formula) over the full grid:
 // J[col*ldJ + q] = A_col(k_q), complex FP64.
 void launch_ff_fp64(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_kfull, int Q,
@@ -335,7 +346,7 @@ struct GramQuery {
 };
 struct GramPlanHeader {
   int nq;
-  int pad_;
+  int ncols;         // columns the queries use (c0 + n of the last)
   long long nitems, nentries, part_doubles;
 };
 int gram_planes_pieces(int fp16);
@@ -349,8 +360,11 @@ GramQuery gram_query(int c0, int n, int K);
 void gram_plan_layout(std::vector<GramQuery>& qs, GramPlanHeader& h);
 // Workgroups of a launch over nitems work items (at most cap; each loops).
 int gram_planes_grid(long long nitems, int cap);
+// Planes laid out for ncols columns; with d_plan (may be null) only the
+// columns its queries use are converted.
 void launch_split_planes(hipStream_t s, int fp16, const float* d_X, int ldK, int ncols,
-                         const unsigned* d_colmax, void* d_planes);
+                         const unsigned* d_colmax, void* d_planes,
+                         const GramPlanHeader* d_plan = nullptr);
 void launch_gram_planes(hipStream_t s, int fp16, const void* d_planes, int ldK, int ncols,
                         const GramPlanHeader* d_plan, int grid_items, long long nentries_cap,
                         double* d_part, const unsigned* d_colmax, double* d_V);

@@ -9,11 +9,8 @@ namespace pii {
 
 namespace {
 
-__global__ void __launch_bounds__(256) k_world_vertices(const ObjDesc* __restrict__ objs,
-                                                        const int* __restrict__ vprefix, int N,
-                                                        int total) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
+__device__ __forceinline__ void world_vertex(const ObjDesc* __restrict__ objs,
+                                             const int* __restrict__ vprefix, int N, int i) {
   const int o = find_owner(vprefix, N, i);
   const ObjDesc& d = objs[o];
   const int v = i - vprefix[o];
@@ -39,13 +36,22 @@ __global__ void __launch_bounds__(256) k_world_vertices(const ObjDesc* __restric
   d.world[3 * v + 2] = w2;
 }
 
+// Both kernels cover the flat index range [0, total); total < 0 takes it from
+// the device prefix sums (prefix[N]), so that a launch of a fixed grid can be
+// replayed for batches of other sizes (grid-stride loops).
+__global__ void __launch_bounds__(256) k_world_vertices(const ObjDesc* __restrict__ objs,
+                                                        const int* __restrict__ vprefix, int N,
+                                                        int total_arg) {
+  const int total = total_arg >= 0 ? total_arg : vprefix[N];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+    world_vertex(objs, vprefix, N, i);
+}
+
 // Element records.  fast != 0 writes the patch-ordered triangle layout and,
 // for objects with a moment series, the moment records.
-__global__ void __launch_bounds__(256) k_records(const ObjDesc* __restrict__ objs,
-                                                 const int* __restrict__ eprefix, int N,
-                                                 int total, int fast) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
+__device__ __forceinline__ void element_record(const ObjDesc* __restrict__ objs,
+                                               const int* __restrict__ eprefix, int N, int i,
+                                               int fast) {
   const int o = find_owner(eprefix, N, i);
   const ObjDesc& d = objs[o];
   const int e = i - eprefix[o];
@@ -112,7 +118,39 @@ __global__ void __launch_bounds__(256) k_records(const ObjDesc* __restrict__ obj
   }
 }
 
+__global__ void __launch_bounds__(256) k_records(const ObjDesc* __restrict__ objs,
+                                                 const int* __restrict__ eprefix, int N,
+                                                 int total_arg, int fast) {
+  const int total = total_arg >= 0 ? total_arg : eprefix[N];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+    element_record(objs, eprefix, N, i, fast);
+}
+
+__global__ void __launch_bounds__(256) k_clear_u32(unsigned* __restrict__ p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+__global__ void k_stamp(unsigned long long* __restrict__ out, int i) {
+  if (threadIdx.x == 0) out[i] = __builtin_amdgcn_s_memrealtime();
+}
+
 }  // namespace
+
+void launch_prepass_fixed(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix,
+                          const int* d_eprefix, int N, int fast, int grid) {
+  hipLaunchKernelGGL(k_world_vertices, dim3(grid), dim3(256), 0, s, d_objs, d_vprefix, N, -1);
+  hipLaunchKernelGGL(k_records, dim3(grid), dim3(256), 0, s, d_objs, d_eprefix, N, -1, fast);
+}
+
+void launch_clear_u32(hipStream_t s, unsigned* d_p, int n) {
+  if (n <= 0) return;
+  const unsigned grid = (unsigned)std::min(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(k_clear_u32, dim3(grid), dim3(256), 0, s, d_p, n);
+}
+
+void launch_stamp(hipStream_t s, unsigned long long* d_out, int i) {
+  hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, d_out, i);
+}
 
 void launch_prepass(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix, int total_v,
                     const int* d_eprefix, int total_e, int N, int fast) {

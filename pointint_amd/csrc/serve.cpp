@@ -1,25 +1,44 @@
 // Serving engine: continuous batching of volume-matrix queries over resident
 // geometry on one GPU.
 //
-// A query is a scene of n resident geometries, each under its own affine
-// (rigid) transform; its answer is the n x n volume matrix.  Queries are
-// admitted against an HBM budget (each holds its memory reservation from
-// admission to completion; a query that does not fit is refused at once).
-// A worker thread keeps up to kSlots batches in flight, each on its own HIP
-// stream with its own workspaces: whenever a slot is free it takes every
-// queued query that fits the slot's object limit, launches one fused
-// form-factor pass over all objects of all those queries, then one Gram per
-// query on its column range, and copies the matrices to pinned host memory.
-// Queries that arrive while batches run join the next free slot, so the work
-// of consecutive batches overlaps on the device.
+// A query is a scene of n resident triangle geometries, each under its own
+// affine (rigid) transform; its answer is the n x n volume matrix.
+//
+// Admission.  A query is refused at once (PII_ERR_BUDGET) when its HBM
+// footprint does not fit the budget left by the queries admitted and not yet
+// finished, or -- with a latency target set -- when the predicted completion
+// (time since its arrival + the outstanding work ahead of it and its own,
+// priced by the measured throughput) would exceed the target.
+//
+// Batching.  A worker thread keeps up to kSlots batches in flight, each on its
+// own HIP stream with its own workspaces.  Whenever a slot is free it takes
+// the queued queries in arrival order, up to max_batch_objects objects and
+// kMaxQueries queries, and runs one fused pass over all their objects:
+// pre-pass, moment series, per-face form factors, then one split-precision
+// Gram launch over every query's column range, and the copy of the matrices
+// to pinned host memory.  Queries that arrive while batches run join the next
+// free slot, so consecutive batches overlap on the device.
+//
+// Graphs.  The launch sequence of a batch is laid out for a capacity (objects
+// rounded up to a bucket, V entries rounded up to a bucket): descriptors past
+// the batch's objects are empty, the grid-stride and persistent kernels read
+// their counts from device memory, and the Gram reads a device plan.  So the
+// sequence of one capacity is captured once into a hipGraph and replayed for
+// every later batch of that capacity, the host only rewriting the pinned
+// staging it uploads.  Replay and direct launch run the same kernels with the
+// same arguments: results are bit-identical, and a query's result does not
+// depend on the batch it is part of (per-object form factors; per-query Gram
+// slices).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -40,17 +59,39 @@ double now_s() {
 struct Request {
   int64_t id = 0;
   int n = 0;
+  std::vector<pii_geometry*> handles;
   std::vector<const GeometryDev*> geoms;
   std::vector<double> dofs;   // n x 12
   size_t footprint = 0;
-  double t_submit = 0, t_launch = 0, t_done = 0;
+  double work = 0;            // triangles: the unit of the latency model
+  double t_arrival = 0, t_submit = 0, t_launch = 0, t_done = 0;
   std::vector<double> V;
   int state = 0;              // 0 queued, 1 running, 2 done
-  int slot = -1;
-  size_t v_off = 0;           // offset of V in the slot's pinned buffer
+  bool failed = false;
+  size_t v_off = 0;           // offset of V in the slot's output
 };
 
 constexpr int kSlots = 2;
+constexpr int kMaxQueries = 64;   // per batch: bounds the column padding between queries
+constexpr int kStamps = 8;
+constexpr int kFixedGrid = 1024;  // workgroups of the grid-stride pre-pass
+constexpr int kGramGrid = 512;    // workgroups of the persistent Gram
+
+// Smallest of lo, 1.5 lo, 2 lo, 3 lo, 4 lo, ... that is >= n.
+long long bucket(long long n, long long lo) {
+  long long b = lo;
+  while (true) {
+    if (b >= n) return b;
+    if (b + b / 2 >= n) return b + b / 2;
+    b *= 2;
+  }
+}
+
+constexpr int kBaked = 12;
+struct Graph {
+  hipGraphExec_t exec = nullptr;
+  std::array<const void*, kBaked> baked{};
+};
 
 struct Slot {
   Workspace w;
@@ -59,21 +100,33 @@ struct Slot {
   bool busy = false;
   std::vector<Request*> reqs;
   Batch batch;
-  double* pinned = nullptr;
-  size_t pinned_n = 0;
-  DevBuf Vdev;
+  HostBuf out;                // pinned: V of every query of the batch, then the stamps
+  DevBuf Vdev, stamps;
+  std::map<std::pair<long long, long long>, Graph> graphs;
+  // the batch in flight
+  long long v_cap = 0;
+  double work = 0, tri_node = 0, face_dir = 0, t_launch = 0;
 };
 
 }  // namespace
 
 struct Server {
   int device = 0;
+  pii_kgrid* kgh = nullptr;
   const KGridDev* kg = nullptr;
   int gram_mode = PII_GRAM_FP16X3;
   bool series = true;                 // moment series (the creating context's setting)
+  std::atomic<bool> use_graphs{true};
   size_t budget = 0;
   size_t reserved = 0;
   int max_batch_objects = 1024;
+  double latency_target_ms = 0.0;     // 0: admission by memory only
+  // latency model: ms of device time per triangle of queued work, from the
+  // marginal completion time of each batch (exponential average)
+  double ms_per_work = 0.0;
+  bool calibrated = false;
+  double last_done = 0.0;
+  double outstanding_work = 0.0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Request*> queue;
@@ -83,117 +136,254 @@ struct Server {
   std::thread worker;
   Slot slots[kSlots];
   // statistics
-  int64_t n_submitted = 0, n_admitted = 0, n_refused = 0, n_completed = 0, n_batches = 0;
-  int64_t n_objects = 0;
-  double busy_since = 0;
+  int64_t n_submitted = 0, n_admitted = 0, n_refused_mem = 0, n_refused_lat = 0;
+  int64_t n_completed = 0, n_batches = 0, n_objects = 0;
+  std::atomic<int64_t> n_replays{0}, n_captures{0};   // counted by the worker without the lock
+  double ms_prepass = 0, ms_series = 0, ms_ff = 0, ms_gram = 0;
+  double units_tri_node = 0, units_face_dir = 0;
   std::string worker_error;
 
+  int ldX() const { return (int)align_up((size_t)kg->Qh, 64); }
+  static int ncol_cap(int n_cap) {
+    return (int)align_up((size_t)n_cap, 16) + 16 * std::min(kMaxQueries, n_cap);
+  }
+
   size_t footprint(int n, const std::vector<const GeometryDev*>& gs) const {
-    const size_t ldX = align_up((size_t)kg->Qh, 64);
+    const size_t ld = (size_t)ldX();
     const size_t ncol = align_up((size_t)n, 16);
-    size_t b = ncol * ldX * sizeof(float2);                   // X columns
+    size_t b = ncol * ld * sizeof(float2);                    // X columns
     if (gram_mode == PII_GRAM_FP64) {
       b += (size_t)n * n * sizeof(double) * (gram_fp64_splits(kg->Qh, n) + 1);
     } else {
       const int fp16 = gram_mode == PII_GRAM_FP16X3 ? 1 : 0;
-      b += gram_plane_bytes(fp16, (int)(2 * ldX), (int)ncol);  // 16-bit planes
+      b += gram_plane_bytes(fp16, (int)(2 * ld), (int)ncol);  // 16-bit planes
       b += (size_t)n * n * sizeof(double) *
-           (gram_query(0, n, (int)(2 * ldX)).nslice + 1);       // partials + V
+           (gram_query(0, n, (int)(2 * ld)).nslice + 1);       // partials + V
     }
     for (const GeometryDev* g : gs) {
-      b += (size_t)g->n * sizeof(double) * kRecFastTri;       // element records
+      b += (size_t)g->n * sizeof(double) * kRecFastTri;        // element records
       if (series) b += (size_t)g->n * sizeof(double) * kRecMoment;  // moment records
-      b += (size_t)g->nv * 3 * sizeof(double);                // world vertices
+      b += (size_t)g->nv * 3 * sizeof(double);                 // world vertices
     }
     return b + 4096;
   }
 
+  // The launch sequence of a batch laid out for capacity (n_cap objects,
+  // ncol columns, v_cap V entries); captured into a graph or run directly.
+  void enqueue(Slot& s, int n_cap, int ncol, long long v_cap, size_t plan_bytes) {
+    hipStream_t st = s.stream;
+    const int fp16 = gram_mode == PII_GRAM_FP16X3 ? 1 : 0;
+    const int ld = ldX();
+    float2* X = s.w.X.as<float2>();
+    unsigned* colmax = s.w.colmax.as<unsigned>();
+    unsigned long long* stamps = s.stamps.as<unsigned long long>();
+    const Batch& b = s.batch;
+    PII_HIP(hipMemcpyAsync(s.w.misc.p, s.w.stage.p, b.stage_bytes, hipMemcpyHostToDevice, st));
+    PII_HIP(hipMemcpyAsync(s.w.gplan.p, s.w.gplan_h.p, plan_bytes, hipMemcpyHostToDevice, st));
+    launch_stamp(st, stamps, 0);
+    launch_clear_u32(st, colmax, ncol);
+    launch_prepass_fixed(st, b.d_objs, b.d_vprefix, b.d_eprefix, n_cap, 1, kFixedGrid);
+    launch_stamp(st, stamps, 1);
+    if (b.Mh > 0)
+      launch_series_tri(st, b.d_objs, n_cap, kg->rep.as<double>(), kg->Qh, b.Mh, 1, nullptr, 0,
+                        X, ld, colmax, 1, nullptr);
+    launch_stamp(st, stamps, 2);
+    launch_ff_fast(st, b.d_objs, n_cap, kg->rep.as<double>(), kg->Qh, 0, 1, nullptr, 0, X, ld,
+                   colmax, 1, nullptr);
+    launch_stamp(st, stamps, 3);
+    launch_split_planes(st, fp16, reinterpret_cast<const float*>(X), 2 * ld, ncol, colmax,
+                        s.w.planes.p, s.w.gplan.as<GramPlanHeader>());
+    launch_gram_planes(st, fp16, s.w.planes.p, 2 * ld, ncol, s.w.gplan.as<GramPlanHeader>(),
+                       kGramGrid, v_cap, s.w.part.as<double>(), colmax, s.Vdev.as<double>());
+    launch_stamp(st, stamps, 4);
+    PII_HIP(hipGetLastError());
+    double* out = s.out.as<double>();
+    PII_HIP(hipMemcpyAsync(out, s.Vdev.p, sizeof(double) * v_cap, hipMemcpyDeviceToHost, st));
+    PII_HIP(hipMemcpyAsync(out + v_cap, s.stamps.p, sizeof(double) * kStamps,
+                           hipMemcpyDeviceToHost, st));
+  }
+
+  // FP64 Gram mode: the plain per-query FP64 Grams, launched directly.
+  void launch_fp64(Slot& s, std::vector<const GeometryDev*>& gs, std::vector<const double*>& dp,
+                   std::vector<int>& kinds, std::vector<int>& nd, std::vector<int>& cols,
+                   std::vector<int>& c0s, int ncol) {
+    const int N = (int)gs.size(), Qh = kg->Qh, ld = ldX();
+    s.batch = Batch();
+    s.batch.build(s.w, s.stream, nullptr, N, gs.data(), kinds.data(), dp.data(), nd.data(), true,
+                  series ? kg : nullptr, cols.data());
+    s.w.X.ensure(sizeof(float2) * (size_t)ld * ncol);
+    s.w.colmax.ensure(sizeof(unsigned) * (size_t)ncol);
+    launch_clear_u32(s.stream, s.w.colmax.as<unsigned>(), ncol);
+    launch_form_factors_fast(s.stream, s.batch, *kg, 1, nullptr, 0, s.w.X.as<float2>(), ld,
+                             s.w.colmax.as<unsigned>(), s.w, nullptr);
+    size_t vtot = 0, maxpart = 1;
+    for (Request* r : s.reqs) {
+      r->v_off = vtot;
+      vtot += (size_t)r->n * r->n;
+      maxpart = std::max(maxpart, (size_t)gram_fp64_splits(Qh, r->n) * r->n * r->n);
+    }
+    s.v_cap = (long long)std::max<size_t>(vtot, 1);
+    s.Vdev.ensure(sizeof(double) * s.v_cap);
+    s.w.part.ensure(sizeof(double) * maxpart);
+    s.out.ensure(sizeof(double) * (s.v_cap + kStamps));
+    for (size_t qi = 0; qi < s.reqs.size(); ++qi) {
+      Request* r = s.reqs[qi];
+      launch_gram_fp64_x(s.stream, s.w.X.as<float2>() + (size_t)c0s[qi] * ld, Qh, r->n, ld,
+                         s.w.part.as<double>(), gram_fp64_splits(Qh, r->n),
+                         s.Vdev.as<double>() + r->v_off);
+      PII_HIP(hipGetLastError());
+    }
+    std::memset(s.out.as<double>() + s.v_cap, 0, sizeof(double) * kStamps);
+    PII_HIP(hipMemcpyAsync(s.out.p, s.Vdev.p, sizeof(double) * vtot, hipMemcpyDeviceToHost,
+                           s.stream));
+  }
+
   void launch(Slot& s) {
-    // Concatenate the queries' objects into one batch.
+    // Concatenate the queries' objects; each query's columns start at a
+    // multiple of 16 (the Gram's column groups).
     std::vector<const GeometryDev*> gs;
     std::vector<const double*> dp;
-    std::vector<int> kinds, nd;
+    std::vector<int> kinds, nd, cols, c0s;
+    int ncol = 0;
+    s.work = 0;
     for (Request* r : s.reqs) {
+      c0s.push_back(ncol);
       for (int i = 0; i < r->n; ++i) {
         gs.push_back(r->geoms[i]);
         dp.push_back(r->dofs.data() + 12 * (size_t)i);
         kinds.push_back(PII_DOF_AFFINE);
         nd.push_back(12);
+        cols.push_back(ncol + i);
       }
+      ncol += (int)align_up((size_t)r->n, 16);
+      s.work += r->work;
     }
     const int N = (int)gs.size();
-    // Each query's columns start at a multiple of 16 (the Gram's column groups).
-    std::vector<int> cols(N);
-    std::vector<int> c0s;
-    int ncol = 0;
-    {
-      int k = 0;
-      for (Request* r : s.reqs) {
-        c0s.push_back(ncol);
-        for (int i = 0; i < r->n; ++i) cols[k++] = ncol + i;
-        ncol += (int)align_up((size_t)r->n, 16);
-      }
-    }
-    s.batch = Batch();
-    s.batch.build(s.w, s.stream, nullptr, N, gs.data(), kinds.data(), dp.data(), nd.data(), true,
-                  series ? kg : nullptr, cols.data());
-    const int Qh = kg->Qh;
-    const int ldX = (int)align_up((size_t)Qh, 64);
-    s.w.X.ensure(sizeof(float2) * (size_t)ldX * ncol);
-    s.w.colmax.ensure(sizeof(unsigned) * (size_t)ncol);
-    unsigned* colmax = s.w.colmax.as<unsigned>();
-    PII_HIP(hipMemsetAsync(colmax, 0, sizeof(unsigned) * ncol, s.stream));
-    launch_form_factors_fast(s.stream, s.batch, *kg, 1, nullptr, 0, s.w.X.as<float2>(), ldX,
-                             colmax, s.w, nullptr);
-    PII_HIP(hipGetLastError());
-    size_t vtot = 0, maxpart = 0;
-    for (Request* r : s.reqs) {
-      r->v_off = vtot;
-      vtot += (size_t)r->n * r->n;
-      if (gram_mode == PII_GRAM_FP64)
-        maxpart = std::max(maxpart, (size_t)gram_fp64_splits(Qh, r->n) * r->n * r->n);
-    }
-    s.Vdev.ensure(sizeof(double) * std::max<size_t>(vtot, 1));
-    if (maxpart) s.w.part.ensure(sizeof(double) * maxpart);
-    if (s.pinned_n < vtot) {
-      if (s.pinned) (void)hipHostFree(s.pinned);
-      s.pinned = nullptr;
-      s.pinned_n = 0;
-      const size_t want = std::max<size_t>(vtot, 4096);
-      PII_HIP(hipHostMalloc((void**)&s.pinned, sizeof(double) * want, hipHostMallocDefault));
-      s.pinned_n = want;
-    }
+    s.t_launch = now_s();
     if (gram_mode == PII_GRAM_FP64) {
-      for (size_t qi = 0; qi < s.reqs.size(); ++qi) {
-        Request* r = s.reqs[qi];
-        const float2* Xr = s.w.X.as<float2>() + (size_t)c0s[qi] * ldX;
-        launch_gram_fp64_x(s.stream, Xr, Qh, r->n, ldX, s.w.part.as<double>(),
-                           gram_fp64_splits(Qh, r->n), s.Vdev.as<double>() + r->v_off);
-        PII_HIP(hipGetLastError());
-      }
+      launch_fp64(s, gs, dp, kinds, nd, cols, c0s, ncol);
     } else {
-      // one launch for the Grams of all queries of the batch
+      const int ld = ldX(), K = 2 * ld;
+      const int fp16 = gram_mode == PII_GRAM_FP16X3 ? 1 : 0;
+      const int n_cap = (int)bucket(N, 16);
+      const int nc = ncol_cap(n_cap);
+      s.batch = Batch();
+      s.batch.build(s.w, s.stream, nullptr, N, gs.data(), kinds.data(), dp.data(), nd.data(),
+                    true, series ? kg : nullptr, cols.data(), n_cap, false);
       std::vector<GramQuery> qs;
       for (size_t qi = 0; qi < s.reqs.size(); ++qi)
-        qs.push_back(gram_query(c0s[qi], s.reqs[qi]->n, 2 * ldX));
-      gram_split_run(s.stream, s.w, gram_mode == PII_GRAM_FP16X3 ? 1 : 0, s.w.X.as<float2>(),
-                     ldX, ncol, colmax, qs, s.Vdev.as<double>());
-      PII_HIP(hipGetLastError());
+        qs.push_back(gram_query(c0s[qi], s.reqs[qi]->n, K));
+      GramPlanHeader h{};
+      gram_plan_layout(qs, h);
+      for (size_t qi = 0; qi < s.reqs.size(); ++qi) s.reqs[qi]->v_off = (size_t)qs[qi].v;
+      s.v_cap = bucket(std::max<long long>(h.nentries, 1), 4096);
+      // buffers whose addresses a captured graph holds
+      s.w.X.ensure(sizeof(float2) * (size_t)ld * nc);
+      s.w.colmax.ensure(sizeof(unsigned) * (size_t)nc);
+      s.w.planes.ensure(gram_plane_bytes(fp16, K, nc));
+      const size_t part_b = sizeof(double) * (size_t)std::max<long long>(h.part_doubles, 1);
+      if (s.w.part.bytes < part_b) s.w.part.ensure(part_b + part_b / 2);
+      const size_t plan_bytes = sizeof(GramPlanHeader) + sizeof(GramQuery) * kMaxQueries;
+      s.w.gplan.ensure(plan_bytes);
+      s.w.gplan_h.ensure(plan_bytes);
+      s.Vdev.ensure(sizeof(double) * s.v_cap);
+      s.stamps.ensure(sizeof(double) * kStamps);
+      s.out.ensure(sizeof(double) * (s.v_cap + kStamps));
+      std::memcpy(s.w.gplan_h.p, &h, sizeof(h));
+      std::memcpy(static_cast<char*>(s.w.gplan_h.p) + sizeof(h), qs.data(),
+                  sizeof(GramQuery) * qs.size());
+      const std::array<const void*, kBaked> baked = {
+          s.w.misc.p, s.w.stage.p, s.w.gplan.p, s.w.gplan_h.p, s.w.X.p,   s.w.colmax.p,
+          s.w.planes.p, s.w.part.p, s.Vdev.p,  s.stamps.p,   s.out.p,   kg->rep.p};
+      if (use_graphs) {
+        const auto key = std::make_pair((long long)n_cap, s.v_cap);
+        auto it = s.graphs.find(key);
+        if (it != s.graphs.end() && it->second.baked != baked) {
+          (void)hipGraphExecDestroy(it->second.exec);
+          s.graphs.erase(it);
+          it = s.graphs.end();
+        }
+        if (it == s.graphs.end()) {
+          hipGraph_t g = nullptr;
+          PII_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+          try {
+            enqueue(s, n_cap, nc, s.v_cap, plan_bytes);
+          } catch (...) {
+            (void)hipStreamEndCapture(s.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          PII_HIP(hipStreamEndCapture(s.stream, &g));
+          Graph gr;
+          gr.baked = baked;
+          const hipError_t e = hipGraphInstantiate(&gr.exec, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          PII_HIP(e);
+          it = s.graphs.emplace(key, gr).first;
+          ++n_captures;
+        }
+        PII_HIP(hipGraphLaunch(it->second.exec, s.stream));
+        ++n_replays;
+      } else {
+        enqueue(s, n_cap, nc, s.v_cap, plan_bytes);
+      }
     }
-    PII_HIP(hipMemcpyAsync(s.pinned, s.Vdev.p, sizeof(double) * vtot, hipMemcpyDeviceToHost,
-                           s.stream));
+    // work of the batch's form-factor kernels
+    s.tri_node = s.face_dir = 0;
+    const int Qh = kg->Qh;
+    for (const ObjDesc& d : s.batch.h_objs) {
+      s.tri_node += (double)d.n * std::max(0, Qh - d.qbeg);
+      if (d.nser > 0) s.face_dir += (double)d.n * s.batch.Mh;
+    }
     PII_HIP(hipEventRecord(s.done, s.stream));
-    s.busy = true;
+  }
+
+  void release(Request* r) {
+    for (pii_geometry* g : r->handles) g->refs.fetch_sub(1);
+    r->handles.clear();
+    reserved -= r->footprint;
+    outstanding_work -= r->work;
   }
 
   void retire(Slot& s) {
     const double t = now_s();
+    const double* out = s.out.as<double>();
     for (Request* r : s.reqs) {
-      r->V.assign(s.pinned + r->v_off, s.pinned + r->v_off + (size_t)r->n * r->n);
+      r->V.assign(out + r->v_off, out + r->v_off + (size_t)r->n * r->n);
       r->t_done = t;
       r->state = 2;
-      reserved -= r->footprint;
+      release(r);
       ++n_completed;
+    }
+    // phase times from the device clock stamps (100 MHz)
+    const unsigned long long* st = reinterpret_cast<const unsigned long long*>(out + s.v_cap);
+    if (gram_mode != PII_GRAM_FP64 && st[4] >= st[0]) {
+      ms_prepass += (double)(st[1] - st[0]) * 1e-5;
+      ms_series += (double)(st[2] - st[1]) * 1e-5;
+      ms_ff += (double)(st[3] - st[2]) * 1e-5;
+      ms_gram += (double)(st[4] - st[3]) * 1e-5;
+      units_tri_node += s.tri_node;
+      units_face_dir += s.face_dir;
+    }
+    // throughput: the time this batch added after the previous completion
+    if (s.work > 0) {
+      const double dt = t - std::max(s.t_launch, last_done);
+      const double sample = dt * 1e3 / s.work;
+      ms_per_work = calibrated ? 0.7 * ms_per_work + 0.3 * sample : sample;
+      calibrated = true;
+    }
+    last_done = t;
+    s.reqs.clear();
+    s.busy = false;
+  }
+
+  void fail(Slot& s, const std::string& msg) {
+    worker_error = msg;
+    for (Request* r : s.reqs) {
+      r->state = 2;
+      r->failed = true;
+      r->t_done = now_s();
+      release(r);
     }
     s.reqs.clear();
     s.busy = false;
@@ -215,38 +405,45 @@ struct Server {
           cv_done.notify_all();
         }
       }
-      if (stop && queue.empty() && !slots[0].busy && !slots[1].busy) break;
+      bool any_busy = false;
+      for (Slot& s : slots) any_busy = any_busy || s.busy;
+      if (stop && queue.empty() && !any_busy) break;
       for (Slot& s : slots) {
         if (s.busy || queue.empty()) continue;
         int nobj = 0;
-        while (!queue.empty()) {
+        const double t = now_s();
+        while (!queue.empty() && (int)s.reqs.size() < kMaxQueries) {
           Request* r = queue.front();
           if (!s.reqs.empty() && nobj + r->n > max_batch_objects) break;
           queue.pop_front();
           s.reqs.push_back(r);
           r->state = 1;
-          r->t_launch = now_s();
+          r->t_launch = t;
           nobj += r->n;
         }
         ++n_batches;
         n_objects += nobj;
+        s.busy = true;
+        // host work of the batch (descriptors, plan, launches) without the lock
+        lk.unlock();
+        std::string err;
         try {
           launch(s);
         } catch (const Error& e) {
-          worker_error = e.msg;
-          for (Request* r : s.reqs) {
-            r->state = 2;
-            r->t_done = now_s();
-            reserved -= r->footprint;
-          }
-          s.reqs.clear();
-          s.busy = false;
+          err = e.msg.empty() ? "launch failed" : e.msg;
+        } catch (const std::exception& e) {
+          err = e.what();
+        }
+        lk.lock();
+        if (!err.empty()) {
+          fail(s, err);
           cv_done.notify_all();
         }
         progressed = true;
       }
       if (!progressed) {
-        const bool any_busy = slots[0].busy || slots[1].busy;
+        any_busy = false;
+        for (Slot& s : slots) any_busy = any_busy || s.busy;
         if (any_busy) {
           // poll the in-flight batches without holding the lock
           lk.unlock();
@@ -270,6 +467,13 @@ using namespace pii;
 
 extern "C" {
 
+int pii_clock_seconds(double* t) {
+  return guarded([&] {
+    if (!t) throw Error(PII_ERR_INVALID, "pii_clock_seconds: NULL output");
+    *t = now_s();
+  });
+}
+
 int pii_server_create(pii_context* ctx, const pii_kgrid* kg, int gram_mode, double hbm_budget,
                       int max_batch_objects, pii_server** out) {
   return guarded([&] {
@@ -281,6 +485,7 @@ int pii_server_create(pii_context* ctx, const pii_kgrid* kg, int gram_mode, doub
     auto srv = std::make_unique<pii_server>();
     Server& S = srv->s;
     S.device = ctx->c.device;
+    S.kgh = const_cast<pii_kgrid*>(kg);
     S.kg = &kg->k;
     S.series = ctx->c.series;
     S.gram_mode = gram_mode;
@@ -290,6 +495,7 @@ int pii_server_create(pii_context* ctx, const pii_kgrid* kg, int gram_mode, doub
       PII_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
       PII_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     }
+    S.kgh->refs.fetch_add(1);
     S.worker = std::thread([p = &S] { p->run(); });
     *out = srv.release();
   });
@@ -308,16 +514,41 @@ int pii_server_destroy(pii_server* srv) {
     (void)hipSetDevice(S.device);
     for (auto& sl : S.slots) {
       if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-      if (sl.pinned) (void)hipHostFree(sl.pinned);
+      for (auto& kv : sl.graphs) (void)hipGraphExecDestroy(kv.second.exec);
+      sl.graphs.clear();
       if (sl.done) (void)hipEventDestroy(sl.done);
       if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
+    // queries queued or running were all retired by the worker; release any
+    // reference a failed launch left
+    for (auto& kv : S.reqs) {
+      for (pii_geometry* g : kv.second->handles) g->refs.fetch_sub(1);
+      kv.second->handles.clear();
+    }
+    if (S.kgh) S.kgh->refs.fetch_sub(1);
     delete srv;
   });
 }
 
-int pii_server_submit(pii_server* srv, int n, const pii_geometry* const* geoms,
-                      const double* affine_dofs, int64_t* id) {
+int pii_server_set_latency_target(pii_server* srv, double target_ms) {
+  return guarded([&] {
+    if (!srv || !(target_ms >= 0.0))
+      throw Error(PII_ERR_INVALID, "pii_server_set_latency_target: bad arguments");
+    std::lock_guard<std::mutex> lk(srv->s.mu);
+    srv->s.latency_target_ms = target_ms;
+  });
+}
+
+int pii_server_set_graphs(pii_server* srv, int enable) {
+  return guarded([&] {
+    if (!srv) throw Error(PII_ERR_INVALID, "NULL server");
+    std::lock_guard<std::mutex> lk(srv->s.mu);
+    srv->s.use_graphs = enable != 0;
+  });
+}
+
+int pii_server_submit_at(pii_server* srv, int n, const pii_geometry* const* geoms,
+                         const double* affine_dofs, double t_arrival, int64_t* id) {
   return guarded([&] {
     if (!srv || !geoms || !affine_dofs || !id || n <= 0)
       throw Error(PII_ERR_INVALID, "pii_server_submit: bad arguments");
@@ -329,22 +560,42 @@ int pii_server_submit(pii_server* srv, int n, const pii_geometry* const* geoms,
       if (!geoms[i]) throw Error(PII_ERR_INVALID, "pii_server_submit: NULL geometry");
       if (geoms[i]->g.type != PII_GEOM_TRIANGLE)
         throw Error(PII_ERR_UNSUPPORTED, "pii_server_submit: triangle geometry only");
+      if (geoms[i]->device != S.device)
+        throw Error(PII_ERR_INVALID, "pii_server_submit: geometry on another device");
       r->geoms[i] = &geoms[i]->g;
+      r->work += (double)geoms[i]->g.n;
     }
     r->dofs.assign(affine_dofs, affine_dofs + 12 * (size_t)n);
     r->footprint = S.footprint(n, r->geoms);
     r->t_submit = now_s();
+    r->t_arrival = t_arrival > 0.0 ? std::min(t_arrival, r->t_submit) : r->t_submit;
     std::lock_guard<std::mutex> lk(S.mu);
     ++S.n_submitted;
     if (!S.worker_error.empty()) throw Error(PII_ERR_HIP, "serving worker failed: " + S.worker_error);
     if (n > S.max_batch_objects || S.reserved + r->footprint > S.budget) {
-      ++S.n_refused;
+      ++S.n_refused_mem;
       throw Error(PII_ERR_BUDGET, "query refused: HBM budget (" + std::to_string(S.reserved) +
                                       " + " + std::to_string(r->footprint) + " > " +
                                       std::to_string(S.budget) + " bytes)");
     }
+    if (S.latency_target_ms > 0.0 && S.calibrated) {
+      const double waited = (r->t_submit - r->t_arrival) * 1e3;
+      const double predicted = waited + (S.outstanding_work + r->work) * S.ms_per_work;
+      if (predicted > S.latency_target_ms) {
+        ++S.n_refused_lat;
+        throw Error(PII_ERR_BUDGET, "query refused: predicted latency " +
+                                        std::to_string(predicted) + " ms > target " +
+                                        std::to_string(S.latency_target_ms) + " ms");
+      }
+    }
     ++S.n_admitted;
     S.reserved += r->footprint;
+    S.outstanding_work += r->work;
+    r->handles.resize(n);
+    for (int i = 0; i < n; ++i) {
+      r->handles[i] = const_cast<pii_geometry*>(geoms[i]);
+      r->handles[i]->refs.fetch_add(1);
+    }
     r->id = S.next_id++;
     *id = r->id;
     S.queue.push_back(r.get());
@@ -353,7 +604,13 @@ int pii_server_submit(pii_server* srv, int n, const pii_geometry* const* geoms,
   });
 }
 
-int pii_server_wait(pii_server* srv, int64_t id, double timeout_s, double* V, double* latency_ms) {
+int pii_server_submit(pii_server* srv, int n, const pii_geometry* const* geoms,
+                      const double* affine_dofs, int64_t* id) {
+  return pii_server_submit_at(srv, n, geoms, affine_dofs, 0.0, id);
+}
+
+int pii_server_wait_times(pii_server* srv, int64_t id, double timeout_s, double* V,
+                          double* times) {
   return guarded([&] {
     if (!srv) throw Error(PII_ERR_INVALID, "NULL server");
     Server& S = srv->s;
@@ -367,14 +624,27 @@ int pii_server_wait(pii_server* srv, int64_t id, double timeout_s, double* V, do
       if (S.cv_done.wait_until(lk, deadline) == std::cv_status::timeout && r->state != 2)
         throw Error(PII_ERR_BUSY, "query " + std::to_string(id) + " not finished");
     }
-    if (r->V.size() != (size_t)r->n * r->n)
-      throw Error(PII_ERR_HIP, "query " + std::to_string(id) + " failed: " + S.worker_error);
+    if (r->failed || r->V.size() != (size_t)r->n * r->n) {
+      const std::string msg = S.worker_error;
+      S.reqs.erase(it);
+      throw Error(PII_ERR_HIP, "query " + std::to_string(id) + " failed: " + msg);
+    }
     if (V) std::memcpy(V, r->V.data(), sizeof(double) * r->V.size());
-    if (latency_ms) {
-      latency_ms[0] = (r->t_done - r->t_submit) * 1e3;
+    if (times) {
+      times[0] = r->t_arrival;
+      times[1] = r->t_submit;
+      times[2] = r->t_launch;
+      times[3] = r->t_done;
     }
     S.reqs.erase(it);
   });
+}
+
+int pii_server_wait(pii_server* srv, int64_t id, double timeout_s, double* V, double* latency_ms) {
+  double t[4];
+  const int rc = pii_server_wait_times(srv, id, timeout_s, V, t);
+  if (rc == PII_OK && latency_ms) latency_ms[0] = (t[3] - t[0]) * 1e3;
+  return rc;
 }
 
 int pii_server_stats(pii_server* srv, double* out, int n) {
@@ -382,10 +652,16 @@ int pii_server_stats(pii_server* srv, double* out, int n) {
     if (!srv || !out) throw Error(PII_ERR_INVALID, "pii_server_stats: bad arguments");
     Server& S = srv->s;
     std::lock_guard<std::mutex> lk(S.mu);
-    const double v[8] = {(double)S.n_submitted, (double)S.n_admitted, (double)S.n_refused,
-                         (double)S.n_completed, (double)S.n_batches, (double)S.n_objects,
-                         (double)S.reserved, (double)S.budget};
-    for (int i = 0; i < n; ++i) out[i] = i < 8 ? v[i] : 0.0;
+    const double v[] = {(double)S.n_submitted, (double)S.n_admitted,
+                        (double)(S.n_refused_mem + S.n_refused_lat), (double)S.n_completed,
+                        (double)S.n_batches, (double)S.n_objects, (double)S.reserved,
+                        (double)S.budget, (double)S.n_refused_mem, (double)S.n_refused_lat,
+                        (double)S.n_replays.load(), (double)S.n_captures.load(), S.ms_prepass,
+                        S.ms_series,
+                        S.ms_ff, S.ms_gram, S.units_tri_node, S.units_face_dir,
+                        S.calibrated ? S.ms_per_work : 0.0};
+    const int nv = (int)(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n; ++i) out[i] = i < nv ? v[i] : 0.0;
   });
 }
 

@@ -1,10 +1,14 @@
 """GPU serving engine: each query's volume matrix is the same alone and in a
-mix of concurrent queries, matches the FP64 oracle, and the HBM budget
-refuses what does not fit."""
+mix of concurrent queries, the same from a replayed hipGraph as from direct
+launches, matches the FP64 oracle, and admission refuses what does not fit
+the HBM budget or the latency target."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
 import pointint_amd as P
+from pointint_amd import _native as N
 from pointint_amd import serving as S
 from pointint_amd import workloads as W
 
@@ -14,6 +18,12 @@ pytestmark = pytest.mark.gpu
 def scaled(V, Vr):
     d = np.sqrt(np.abs(np.outer(np.diag(Vr), np.diag(Vr))))
     return float(np.max(np.abs(V - Vr) / d))
+
+
+def pair_rel(V, V64, floor=1e-3):
+    d = np.sqrt(np.abs(np.outer(np.diag(V64), np.diag(V64))))
+    m = np.triu(V64 >= floor * d, 1)
+    return float((np.abs(V - V64)[m] / np.abs(V64[m])).max()), int(m.sum())
 
 
 @pytest.fixture(scope="module")
@@ -44,12 +54,43 @@ def test_alone_equals_in_mix_and_oracle(oracle, pool):
             srv.wait(q, k)
         st = srv.stats()
         assert st["completed"] == 26 and st["refused"] == 0 and st["reserved_bytes"] == 0
+        assert st["graph_launches"] == st["batches"] >= 2
     assert np.array_equal(V_alone, V_mix)
     ogeoms = [oracle.geom_mesh(pool[i].V, pool[i].F) for i in idx]
     Jo = np.stack([oracle.compute_A_grid(g, kg.dirs, kg.kmag, d, oracle.DOF_AFFINE)
                    for g, d in zip(ogeoms, dofs)], axis=1)
     Vo = oracle.gram(Jo, kg.w)
     assert scaled(V_alone, Vo) < 1e-5
+
+
+@pytest.mark.parametrize("gram", ["bf16x3", "fp16x3"])
+def test_graph_replay_equals_direct_launch(pool, gram):
+    """The same queries through a server replaying captured graphs and one
+    launching the kernels directly: bit-identical matrices; the graph server
+    captures one graph per batch capacity and replays it."""
+    kg = P.kgrid_from_rule(29, 16)
+    geoms = [P.make_triangle_mesh(m.V, m.F) for m in pool]
+    rng = np.random.default_rng(11)
+    scenes = []
+    for _ in range(12):
+        k = int(rng.integers(3, 14))
+        scenes.append((rng.integers(0, len(pool), size=k), S.rigid_dofs(rng, 1.0, k)))
+    out = {}
+    for graphs in (True, False):
+        res = []
+        with S.Server(kg, gram=gram, hbm_budget=4e9, graphs=graphs) as srv:
+            # one query at a time: every batch has the same capacity class
+            for idx, d in scenes:
+                q = srv.submit([geoms[i] for i in idx], d)
+                res.append(srv.wait(q, len(idx))[0])
+            st = srv.stats()
+        out[graphs] = (res, st)
+    for a, b in zip(out[True][0], out[False][0]):
+        assert np.array_equal(a, b)
+    st_g, st_d = out[True][1], out[False][1]
+    assert st_g["graph_launches"] == st_g["batches"] == 12
+    assert 1 <= st_g["graphs_captured"] < st_g["graph_launches"]
+    assert st_d["graph_launches"] == 0
 
 
 def test_budget_refusal(pool):
@@ -61,14 +102,105 @@ def test_budget_refusal(pool):
         big = srv.submit([geoms[i % 12] for i in range(200)], S.rigid_dofs(rng, 1.0, 200))
         assert big is None
         st = srv.stats()
-        assert st["refused"] == 1
+        assert st["refused"] == 1 and st["refused_memory"] == 1
+
+
+def test_latency_target_refusal(pool):
+    """Once the engine has measured its throughput, a target far below one
+    query's service time refuses new queries (and a query that has already
+    waited past the target is refused at submission)."""
+    kg = P.kgrid_from_rule(29, 16)
+    geoms = [P.make_triangle_mesh(m.V, m.F) for m in pool]
+    rng = np.random.default_rng(5)
+    with S.Server(kg, hbm_budget=4e9, latency_target_ms=1e-6) as srv:
+        q = srv.submit(geoms[:4], S.rigid_dofs(rng, 1.0, 4))   # not calibrated: admitted
+        assert q is not None
+        srv.wait(q, 4)
+        assert srv.stats()["ms_per_triangle"] > 0
+        assert srv.submit(geoms[:4], S.rigid_dofs(rng, 1.0, 4)) is None
+        st = srv.stats()
+        assert st["refused_latency"] == 1 and st["refused_memory"] == 0
+    with S.Server(kg, hbm_budget=4e9, latency_target_ms=50.0) as srv:
+        q = srv.submit(geoms[:4], S.rigid_dofs(rng, 1.0, 4))
+        srv.wait(q, 4)
+        # arrived one second ago: already past the target
+        assert srv.submit(geoms[:4], S.rigid_dofs(rng, 1.0, 4), t_arrival=S.clock() - 1.0) is None
+
+
+def test_geometry_in_use_cannot_be_destroyed(pool):
+    kg = P.kgrid_from_rule(29, 16)
+    geoms = [P.make_triangle_mesh(m.V, m.F) for m in pool[:3]]
+    rng = np.random.default_rng(6)
+    lib = N.load()
+    with S.Server(kg, hbm_budget=4e9) as srv:
+        q = srv.submit(geoms, S.rigid_dofs(rng, 1.0, 3))
+        assert lib.pii_geometry_destroy(geoms[0].handle) == N.PII_ERR_BUSY
+        assert lib.pii_kgrid_destroy(kg.handle) == N.PII_ERR_BUSY
+        srv.wait(q, 3)
+    # released: the Python objects free them normally afterwards
 
 
 def test_poisson_load_reports_latency(pool):
     kg = P.kgrid_from_rule(29, 16)
     geoms = [P.make_triangle_mesh(m.V, m.F) for m in pool]
     with S.Server(kg, hbm_budget=2e9) as srv:
-        rep = S.poisson_load(srv, geoms, rate=200.0, duration=1.0, kmin=4, kmax=32, box=2.0)
+        rep = S.poisson_load(srv, geoms, rate=200.0, duration=1.0, kmin=4, kmax=32, box=2.0,
+                             keep=3)
     s = rep.summary()
     assert s["completed"] == s["admitted"] > 0
     assert s["p99_ms"] >= s["p50_ms"] > 0
+    assert len(rep.scenes) == 3 and rep.scenes[0].V.shape[0] == len(rep.scenes[0].idx)
+
+
+@pytest.fixture(scope="module")
+def c5_pool():
+    """64 meshes generated as the serving workload's pool (triangle counts
+    log-uniform in [80, 20480], star-shaped, radius U[0.2, 0.5])."""
+    return W.serving_pool(n_mesh=64, seed=5)
+
+
+def test_serving_scale_scene(oracle, c5_pool):
+    """The serving workload's grid (Lebedev 1202 x L=64, Q = 76,928) and its
+    largest scene (K = 512 objects of the pool under random rigid transforms,
+    centres in [0, 4]^3): the served V equals the same scene served in a mix
+    of other scenes bit for bit, its overlapping pairs match the plain FP64 GPU
+    path within 1e-5 relative, and the form factors of three of its objects
+    match the FP64 oracle on a strided node subset (including the outermost
+    shells) within 1e-6 of each shell's largest |A|."""
+    kg = P.kgrid_from_rule(59, 64)
+    geoms = [P.make_triangle_mesh(m.V, m.F) for m in c5_pool]
+    rng = np.random.default_rng(12)
+    K = 512
+    idx = rng.integers(0, len(geoms), size=K)
+    dofs = S.rigid_dofs(rng, 4.0, K)
+    sg = [geoms[i] for i in idx]
+    print(f"scene: {K} objects, {sum(g.num_elements() for g in sg)} triangles")
+    with S.Server(kg, gram="bf16x3", hbm_budget=64e9) as srv:
+        V_alone, _ = srv.wait(srv.submit(sg, dofs), K)
+        others = []
+        for j in range(6):
+            k = int(W.log_uniform_int(rng, 8, 512))
+            oi = rng.integers(0, len(geoms), size=k)
+            others.append((srv.submit([geoms[i] for i in oi], S.rigid_dofs(rng, 4.0, k)), k))
+            if j == 2:
+                qmix = srv.submit(sg, dofs)
+        V_mix, _ = srv.wait(qmix, K)
+        for q, k in others:
+            srv.wait(q, k)
+    assert np.array_equal(V_alone, V_mix)
+    V64 = P.compute_volume_matrix(sg, kg, list(dofs), ["affine"] * K, ff="fp64", gram="fp64")
+    e, npairs = pair_rel(V_alone, V64)
+    print(f"per-pair relative error vs FP64 GPU path: {e:.3e} over {npairs} overlapping pairs")
+    assert npairs > 100 and e < 1e-5
+    M = 1202
+    sub = np.concatenate([np.arange(s * M, (s + 1) * M, 7) for s in range(64)])
+    for j in (0, 1, 2):
+        m = c5_pool[idx[j]]
+        A = P.compute_form_factor(sg[j], kg, dofs[j], "affine", precision="fast")[sub]
+        Ao = oracle.compute_A_grid(oracle.geom_mesh(m.V, m.F), kg.dirs[sub], kg.kmag[sub],
+                                   dofs[j], oracle.DOF_AFFINE)
+        nsub = len(sub) // 64
+        err = (np.abs(A - Ao).reshape(64, nsub).max(axis=1) /
+               np.abs(Ao).reshape(64, nsub).max(axis=1)).max()
+        print(f"object {j}: per node / shell max {err:.2e}")
+        assert err < 1e-6
