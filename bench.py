@@ -1,16 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the k-space intersection-volume path on one MI355X.
 
-Workload (one "step"): the all-pairs volume matrix of 256 icospheres at
-subdivision level 3 (1,280 triangles each; radius U[0.2,0.5], axis scales
-U[0.7,1.3], uniform rotation, centres U[0,2]^3, seed 2) on the Lebedev 1202 x
-64 Gauss-Legendre k-grid (Q = 76,928): form factors of all objects, then the
-Gram V = Re(J^H D J)/(8 pi^3), inputs resident in HBM.
+Default workload (serving): a Poisson stream of 2,000 queries/s against a
+device-resident pool of 4,096 star-shaped meshes (triangle counts
+log-uniform in [80, 20480]); each query asks for the all-pairs volume matrix
+of a scene of K (log-uniform in [8, 512]) pool objects under fresh random
+rigid transforms, on the Lebedev 1202 x L=64 k-grid (Q = 76,928).  One
+"step" is one segment (--segment-s seconds) of arrivals; the warm-up
+segments' queries are served but not counted.  Queries are admitted by HBM
+budget and a latency target and served by continuous batching replayed from
+hipGraphs.
 
-Prints one JSON line: pairs/s (N(N-1)/2 distinct pairs per step) as `value`,
-plus prim*k evals/s, the plain FP64 GPU path measured on the same inputs, the
-relative error of V against FP64, the form-factor kernel's roofline figures
-and the FP64 CPU oracle timed on a bounded sample on this host.
+Prints one JSON line: completed pairs/s in the timed window as `value`,
+p50/p99 latency from the scheduled arrival, the plain FP64 GPU path and its
+per-pair error on a sample of the served scenes, the form-factor kernel's
+roofline figures, the FP64 CPU oracle on the same sample, and (extras) the
+fixed 256-icosphere and 1,024-mixed-object matrices.  Other workloads:
+--workload icospheres | mixed | diff | align.
 """
 from __future__ import annotations
 
@@ -52,7 +58,7 @@ PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 (= non-packed FP32) peak
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB device copy
 # (fetch x2.00, write x1.00); profiles/ff_fast_counters_icospheres256_L64x1202.md
 # holds the counts and commands.  Counters cannot be read inside this run.
-MEASURED_TRAFFIC = {("icospheres", "k_ff_fast (per-element)"): 60.77e6}
+MEASURED_TRAFFIC = {("icospheres_L3_N256", "k_ff_fast"): 60.77e6}
 PEAK_MFMA_F16_TFLOPS = 2500.0
 
 
@@ -308,8 +314,139 @@ def run_align(args):
     print(json.dumps(out))
 
 
+def measure_matrix(wl, gram, steps, warmup, ff="fast", fp64_ref=True, attribution=False):
+    """One all-pairs volume matrix per step on a fixed workload: timing, phase
+    and kernel figures, the plain FP64 GPU path on the same inputs, the
+    relative error of every Gram mode against it and, with attribution, the
+    split of the speed-up into hemisphere symmetry, moment series and
+    precision."""
+    ctx = P.Context.default()
+    kg = P.kgrid_from_rule(wl.lebedev_order, wl.n_radial)
+    geoms = make_geoms(wl)
+    N = len(geoms)
+    pairs = N * (N - 1) // 2
+    primk = wl.prim_count() * kg.Q
+
+    def step(f, g):
+        return P.compute_volume_matrix(geoms, kg, ff=f, gram=g)
+
+    for _ in range(warmup):
+        step(ff, gram)
+    ctx.set_profiling(True)
+    ff_ms, gram_ms, pre_ms, ser_ms, face_ms = [], [], [], [], []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        V = step(ff, gram)
+        t = ctx.timings()
+        ff_ms.append(t["form_factor"]); gram_ms.append(t["gram"]); pre_ms.append(t["prepass"])
+        ser_ms.append(t["ff_series"]); face_ms.append(t["ff_per_face"])
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.set_profiling(False)
+    work = ctx.work()
+    ms_step = (t1 - t0) * 1e3 / steps
+    print(f"[bench] {wl.name} {ff}/{gram}: {ms_step:.2f} ms/step, FF {np.mean(ff_ms):.2f} ms, "
+          f"Gram {np.mean(gram_ms):.3f} ms", file=sys.stderr, flush=True)
+    out = {"workload": wl.name, "objects": N, "elements": wl.prim_count(), "Q": kg.Q,
+           "ff": ff, "gram": gram, "steps": steps, "warmup": warmup, "ms_per_step": ms_step,
+           "pairs_per_s": pairs / (ms_step * 1e-3),
+           "prim_k_per_s_nominal": primk / (ms_step * 1e-3),
+           "ff_ms": float(np.mean(ff_ms)), "gram_ms": float(np.mean(gram_ms)),
+           "prepass_ms": float(np.mean(pre_ms))}
+    kernels = {}
+    if ff == "fast":
+        kernels["k_ff_fast"] = {
+            "ms": float(np.mean(face_ms)), "units": work["tri_node"] + work["surfel_node"],
+            "unit": "element x node",
+            "flop": work["tri_node"] * FLOP_PER_TRI_K + work["surfel_node"] * FLOP_PER_DISK_K}
+        if work["series_face_dir"] + work["series_surfel_dir"] > 0:
+            kernels["k_series_tri / k_series_surfel"] = {
+                "ms": float(np.mean(ser_ms)),
+                "units": work["series_face_dir"] + work["series_surfel_dir"],
+                "unit": "element x direction (+ object x node evaluations)",
+                "flop": (work["series_face_dir"] * FLOP_PER_FACE_DIR
+                         + work["series_surfel_dir"] * FLOP_PER_SURFEL_DIR
+                         + work["series_node"] * FLOP_PER_SERIES_NODE)}
+    for k in kernels.values():
+        k["tflops"] = k["flop"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
+        k["units_per_s"] = k["units"] / (k["ms"] * 1e-3) if k["ms"] > 0 else 0.0
+    out["kernels"] = kernels
+    if kernels:
+        dom = max(kernels, key=lambda n: kernels[n]["ms"])
+        out["roofline"] = {"bound": "valu", "kernel": dom, "achieved": kernels[dom]["tflops"],
+                           "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                           "frac": kernels[dom]["tflops"] / PEAK_VALU_TFLOPS,
+                           "traffic": MEASURED_TRAFFIC.get((wl.name, dom))}
+    # the Gram against its roofs: algorithmic flops 4 Q per pair over the
+    # N (N + 1) / 2 unique pairs (the split's extra products not counted),
+    # bytes X read once (8 B per representative node and object) + V written
+    gram_flop = 2.0 * kg.Q * N * (N + 1)
+    gram_bytes = 8.0 * kg.Qh * N + 8.0 * N * N
+    g_s = float(np.mean(gram_ms)) * 1e-3
+    out["gram_roofline"] = {"mfma_tflops": gram_flop / g_s / 1e12,
+                            "mfma_frac": gram_flop / g_s / 1e12 / PEAK_MFMA_F16_TFLOPS,
+                            "hbm_gbps": gram_bytes / g_s / 1e9,
+                            "hbm_frac": gram_bytes / g_s / 1e9 / 8000.0,
+                            "algorithmic_flop": gram_flop, "algorithmic_bytes": gram_bytes}
+    if fp64_ref:
+        ctx.set_profiling(True)
+        ctx.synchronize()
+        r0 = time.perf_counter()
+        V64 = step("fp64", "fp64")
+        r1 = time.perf_counter()
+        t64 = ctx.timings()
+        ctx.set_profiling(False)
+        d = np.sqrt(np.abs(np.outer(np.diag(V64), np.diag(V64))))
+        ovl = np.triu(V64 > 1e-3 * d, 1)
+
+        def errs(Vx):
+            return {"max_rel_err_overlapping_pairs":
+                    float((np.abs(Vx - V64)[ovl] / np.abs(V64[ovl])).max()) if ovl.any() else 0.0,
+                    "max_err_scaled": float((np.abs(Vx - V64) / d).max())}
+
+        # every Gram mode on the same fast form factors: the fp64 Gram shows
+        # the form factors' own share of the error
+        by_mode = {g: errs(step("fast", g)) for g in ("fp64", "bf16x3", "fp16x3")}
+        out.update({"fp64_ms_per_step": (r1 - r0) * 1e3, "fp64_ff_ms": t64["form_factor"],
+                    "fp64_gram_ms": t64["gram"],
+                    "speedup_vs_fp64_gpu": (r1 - r0) * 1e3 / ms_step,
+                    "phase_speedup_vs_fp64": {"form_factor": t64["form_factor"] / out["ff_ms"],
+                                              "gram": t64["gram"] / out["gram_ms"]},
+                    "overlapping_pairs": int(ovl.sum()),
+                    **errs(V), "errors_by_gram_mode": by_mode})
+        print(f"[bench] {wl.name} fp64 reference: {out['fp64_ms_per_step']:.2f} ms/step, "
+              f"per-pair err {out['max_rel_err_overlapping_pairs']:.2e}", file=sys.stderr,
+              flush=True)
+        if attribution:
+            # per-face kernel on every representative node (series off)
+            ctx.set_series(False)
+            step(ff, gram)
+            ctx.set_profiling(True)
+            step(ff, gram)
+            tns = ctx.timings()
+            ctx.set_profiling(False)
+            ctx.set_series(True)
+            sym = kg.Q / kg.Qh
+            prec = t64["form_factor"] / sym / tns["form_factor"]
+            ser = tns["form_factor"] / out["ff_ms"]
+            out["form_factor_speedup_attribution"] = {
+                "hemisphere_symmetry": sym, "precision_fp32_per_face_vs_fp64": prec,
+                "moment_series": ser, "product": sym * prec * ser,
+                "measured": t64["form_factor"] / out["ff_ms"],
+                "ff_ms_without_series": tns["form_factor"]}
+    return out
+
+
+def scene_geoms_dofs(geoms, sc):
+    return [geoms[i] for i in sc.idx], list(sc.dofs)
+
+
 def run_serving(args):
-    """Poisson query stream against a resident pool of star-shaped meshes."""
+    """The serving workload: a Poisson stream of scene queries against a
+    device-resident pool of star-shaped meshes; one step = one segment of the
+    stream (--segment-s seconds of arrivals); the warm-up segments' queries
+    are served but not counted."""
     from pointint_amd import serving as S
     pool = W.serving_pool(n_mesh=args.pool, seed=5)
     print(f"[bench] pool of {len(pool)} meshes, {sum(m.n for m in pool)} triangles",
@@ -317,32 +454,195 @@ def run_serving(args):
     ctx = P.Context.default()
     kg = P.kgrid_from_rule(59, 64)
     geoms = [P.make_triangle_mesh(m.V, m.F) for m in pool]
+    seg = args.segment_s
+    duration = (args.warmup + args.steps) * seg
     with S.Server(kg, gram=args.gram, hbm_budget=args.budget_gb * 1e9,
-                  max_batch_objects=args.max_batch) as srv:
-        # warm the engine (workspace growth) outside the measured stream
-        rng = np.random.default_rng(1)
-        q = srv.submit(geoms[:16], S.rigid_dofs(rng, 4.0, 16))
-        srv.wait(q, 16)
-        rep = S.poisson_load(srv, geoms, rate=args.rate, duration=args.duration, kmin=8,
-                             kmax=512, box=4.0, seed=11)
+                  max_batch_objects=args.max_batch, latency_target_ms=args.latency_target_ms,
+                  graphs=not args.no_graphs) as srv:
+        rep = S.poisson_load(srv, geoms, rate=args.rate, duration=duration, kmin=8, kmax=512,
+                             box=4.0, seed=11, keep=args.sample_scenes)
         st = srv.stats()
-    s = rep.summary()
+    # the timed window: arrivals in the last `steps` segments
+    ws, we = args.warmup * seg, duration
+    rec = [r for r in rep.records if ws <= r[0] < we]
+    done = [r for r in rep.records if not np.isnan(r[1])]
+    in_window = [r for r in done if ws <= r[1] < we]
+    pairs_w = sum(r[2] * (r[2] - 1) // 2 for r in in_window)
+    tris_w = sum(r[3] for r in in_window)
+    lat = np.array([(r[1] - r[0]) * 1e3 for r in rec if not np.isnan(r[1])])
+    value = pairs_w / (we - ws)
+    n_adm = sum(1 for r in rec if not np.isnan(r[1]))
+    print(f"[bench] serving: {value:.3e} pairs/s, p50 {np.percentile(lat, 50):.1f} ms, "
+          f"p99 {np.percentile(lat, 99):.1f} ms, {n_adm}/{len(rec)} admitted in the window",
+          file=sys.stderr, flush=True)
+    # form-factor kernels of the served batches (device-clock stamps around
+    # each launch on the batch's stream; two batches in flight share the GPU)
+    ff_flop = st["tri_node"] * FLOP_PER_TRI_K
+    ff_tflops = ff_flop / (st["ff_ms"] * 1e-3) / 1e12 if st["ff_ms"] > 0 else 0.0
+    roofline = {"bound": "valu", "kernel": "k_ff_fast", "achieved": ff_tflops,
+                "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s", "frac": ff_tflops / PEAK_VALU_TFLOPS,
+                "traffic": MEASURED_TRAFFIC.get(("serving", "k_ff_fast")),
+                "kernel_ms_per_batch": st["ff_ms"] / max(st["batches"], 1),
+                "units": st["tri_node"], "unit_of_work": "triangle x node",
+                "batches": st["batches"],
+                "phase_ms_total": {"prepass": st["prepass_ms"], "series": st["series_ms"],
+                                   "per_face": st["ff_ms"], "gram": st["gram_ms"]}}
+    # the plain FP64 GPU path on a sample of the served scenes: speed and error
+    vs64 = None
+    scenes = rep.scenes
+    if scenes and not args.no_fp64_ref:
+        t64, tfast, pairs64, worst, npairs = 0.0, 0.0, 0, 0.0, 0
+        for sc in scenes:
+            g, d = scene_geoms_dofs(geoms, sc)
+            k = len(g)
+            ctx.synchronize()
+            r0 = time.perf_counter()
+            V64 = P.compute_volume_matrix(g, kg, d, ["affine"] * k, ff="fp64", gram="fp64")
+            t64 += time.perf_counter() - r0
+            # the same scene alone through the one-call fast path (warm)
+            P.compute_volume_matrix(g, kg, d, ["affine"] * k, ff="fast", gram=args.gram)
+            r0 = time.perf_counter()
+            P.compute_volume_matrix(g, kg, d, ["affine"] * k, ff="fast", gram=args.gram)
+            tfast += time.perf_counter() - r0
+            pairs64 += k * (k - 1) // 2
+            dd = np.sqrt(np.abs(np.outer(np.diag(V64), np.diag(V64))))
+            ovl = np.triu(V64 > 1e-3 * dd, 1)
+            npairs += int(ovl.sum())
+            if ovl.any():
+                worst = max(worst, float((np.abs(sc.V - V64)[ovl] / np.abs(V64[ovl])).max()))
+        vs64 = {"scenes": len(scenes), "scene_objects": [len(sc.idx) for sc in scenes],
+                "fp64_pairs_per_s": pairs64 / t64, "fp64_s": t64,
+                "speedup": value / (pairs64 / t64),
+                "one_call_fast_pairs_per_s": pairs64 / tfast,
+                "one_call_fast_speedup": t64 / tfast,
+                "max_rel_err_overlapping_pairs": worst, "overlapping_pairs": npairs}
+        print(f"[bench] FP64 GPU path on {len(scenes)} served scenes: "
+              f"{pairs64 / t64:.3e} pairs/s; x{vs64['speedup']:.1f}; per-pair err {worst:.2e}",
+              file=sys.stderr, flush=True)
+    cpu = None
+    if scenes and not args.no_cpu:
+        cpu = cpu_baseline_scenes(pool, kg, scenes, args.cpu_seconds)
+    extras = {}
+    if not args.no_extras:
+        extras["icospheres_256_L3"] = measure_matrix(
+            W.config_icospheres(n_obj=256, level=3, seed=2), "bf16x3", 10, 3, attribution=True)
+        extras["mixed_1024"] = measure_matrix(W.config_mixed(), "fp16x3", 3, 1)
+    elem_q = np.mean([pool[i].n for sc in scenes for i in sc.idx]) if scenes else float("nan")
     out = {
-        "metric": "serving: completed pairs/s and p50/p99 query latency under a Poisson stream",
-        "value": s["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": s["completed"],
-        "warmup": 1, "ms_per_step": 1e3 / s["completed_qps"] if s["completed_qps"] else None,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-        "data": "synthetic (seeded star-shaped meshes, random rigid transforms)",
-        "config": {"workload": f"pool {len(pool)} meshes (T log-uniform [80, 20480]); "
-                               f"K log-uniform [8, 512]; Lebedev 1202 x L=64; Poisson "
-                               f"{args.rate:g} q/s for {args.duration:g} s",
+        "metric": "overlap pairs/s & prim*k evals/s vs plain FP64 at 1e-5 rel err; p50/p99 latency",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": seg * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded star-shaped meshes, random rigid transforms, Poisson arrivals)",
+        "config": {"workload": f"serving: pool of {len(pool)} star meshes (T log-uniform "
+                               f"[80, 20480]), scenes of K log-uniform [8, 512] objects under "
+                               f"random rigid transforms (centres U[0,4]^3), Poisson "
+                               f"{args.rate:g} queries/s, Lebedev 1202 x L=64 (Q=76,928)",
                    "gram": args.gram, "hbm_budget_gb": args.budget_gb,
-                   "max_batch_objects": args.max_batch},
-        "latency_ms": {"p50": s["p50_ms"], "p99": s["p99_ms"], "mean": s["mean_ms"]},
-        "queries": {k: s[k] for k in ("submitted", "admitted", "refused", "completed")},
-        "completed_qps": s["completed_qps"], "batches": st["batches"],
+                   "latency_target_ms": args.latency_target_ms,
+                   "max_batch_objects": args.max_batch, "graphs": not args.no_graphs,
+                   "step": f"{seg:g} s of arrivals", "parallelism": "single GPU"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean()), "max": float(lat.max()),
+                       "from": "scheduled arrival"},
+        "queries": {"submitted": st["submitted"], "admitted": st["admitted"],
+                    "refused_memory": st["refused_memory"],
+                    "refused_latency_target": st["refused_latency"],
+                    "completed": st["completed"], "window_arrivals": len(rec),
+                    "window_admitted": n_adm, "window_completed": len(in_window)},
+        "completed_qps": len(in_window) / (we - ws),
+        "generator_lag_p99_ms": rep.summary()["submit_lag_p99_ms"],
+        # triangles x full-grid nodes of the completed queries (the plain
+        # path's evaluations), per second of the window: a nominal rate
+        "prim_k_per_s_nominal": tris_w * kg.Q / (we - ws),
+        # evaluations the fast path made (whole run): per-face triangle x
+        # node and series triangle x direction
+        "evaluated_per_s": {"tri_node": st["tri_node"] / rep.wall_s,
+                            "series_face_dir": st["face_dir"] / rep.wall_s},
+        "mean_triangles_per_object": float(elem_q),
+        "batches": st["batches"], "graph_launches": st["graph_launches"],
+        "graphs_captured": st["graphs_captured"],
         "objects_per_batch": st["objects"] / max(st["batches"], 1),
+        "roofline": roofline,
+        "vs_fp64_gpu": vs64,
+        "cpu_baseline": cpu,
+        "extras": extras,
     }
+    print(json.dumps(out))
+
+
+def cpu_baseline_scenes(pool, kg, scenes, target_seconds):
+    """FP64 CPU oracle (the restated reference CPU path) on the served scenes:
+    every scene on a strided k-node subsample sized so that the whole sample
+    takes about target_seconds, extrapolated linearly in Q."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    og = {}
+
+    def run(stride):
+        t = 0.0
+        for sc in scenes:
+            gs = []
+            for i in sc.idx:
+                if i not in og:
+                    og[i] = O.geom_mesh(pool[i].V, pool[i].F)
+                gs.append(og[i])
+            idx = np.arange(0, kg.Q, stride)
+            t0 = time.perf_counter()
+            O.volume_matrix(gs, kg.dirs[idx], kg.kmag[idx], kg.w[idx], list(sc.dofs),
+                            [O.DOF_AFFINE] * len(gs))
+            t += (time.perf_counter() - t0) * kg.Q / len(idx)
+        return t
+
+    cal_stride = max(1, kg.Q // 32)
+    t0 = time.perf_counter()
+    run(cal_stride)
+    t_cal = time.perf_counter() - t0
+    stride = max(1, int(cal_stride * t_cal / max(target_seconds, 1.0)))
+    t0 = time.perf_counter()
+    t_full = run(stride)
+    t_s = time.perf_counter() - t0
+    pairs = sum(len(sc.idx) * (len(sc.idx) - 1) // 2 for sc in scenes)
+    print(f"[bench] cpu oracle: {t_full:.1f} s for the {len(scenes)} scenes, extrapolated",
+          file=sys.stderr, flush=True)
+    return {"value": pairs / t_full, "unit": "pairs/s", "cores": O.threads(), "kind": "port",
+            "sample": f"FP64 oracle on the {len(scenes)} sampled served scenes "
+                      f"({sum(len(sc.idx) for sc in scenes)} objects), every {stride}th of "
+                      f"{kg.Q} k-nodes, {t_s:.1f} s, extrapolated x{stride}; host {cpu_model()}"}
+
+
+def run_matrix(args):
+    """One fixed workload (icospheres: 256 icospheres L3 on Lebedev 1202 x
+    L=64; mixed: 1,024 mixed objects on Lebedev 2702 x L=96), one JSON line."""
+    if args.workload == "mixed":
+        wl = W.config_mixed()
+        if "--gram" not in sys.argv:
+            args.gram = "fp16x3"
+    else:
+        wl = W.config_icospheres(n_obj=args.n_obj, level=3, seed=2)
+    m = measure_matrix(wl, args.gram, args.steps, args.warmup, ff=args.ff,
+                       fp64_ref=not args.no_fp64_ref, attribution=args.workload == "icospheres")
+    cpu = None
+    if not args.no_cpu:
+        kg = P.kgrid_from_rule(wl.lebedev_order, wl.n_radial)
+        N = len(wl.objects)
+        t_full, nq, t_s, cores = cpu_baseline(wl, kg, args.cpu_seconds)
+        cpu = {"value": N * (N - 1) // 2 / t_full, "unit": "pairs/s", "cores": cores,
+               "kind": "port",
+               "sample": f"FP64 oracle on {nq} of {kg.Q} k-nodes (every {round(kg.Q / nq)}th), "
+                         f"all {N} objects, {t_s:.1f} s, extrapolated x{kg.Q / nq:.1f}; "
+                         f"host {cpu_model()}"}
+    out = {"metric": "overlap pairs/s & prim*k evals/s vs plain FP64 at 1e-5 rel err; "
+                     "p50/p99 latency",
+           "value": m["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64" if args.ff == "fast" else "f64", "data": "synthetic (seeded)",
+           "config": {"workload": wl.name, "objects": m["objects"], "elements": m["elements"],
+                      "Q": m["Q"], "ff": args.ff, "gram": args.gram,
+                      "parallelism": "single GPU"},
+           "roofline": m.get("roofline"), "cpu_baseline": cpu}
+    out.update({k: v for k, v in m.items() if k not in ("roofline",)})
     print(json.dumps(out))
 
 
@@ -355,210 +655,39 @@ def main():
     ap.add_argument("--gram", default="bf16x3", choices=["fp64", "bf16x3", "fp16x3"])
     ap.add_argument("--no-fp64-ref", action="store_true", help="skip the plain FP64 GPU run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=30.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="serving: skip the fixed-workload measurements")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--n-obj", type=int, default=256)
-    ap.add_argument("--workload", default="icospheres",
-                    choices=["icospheres", "mixed", "diff", "serving", "align"])
+    ap.add_argument("--workload", default="serving",
+                    choices=["serving", "icospheres", "mixed", "diff", "align"])
     ap.add_argument("--fd-vertices", type=int, default=200)
     ap.add_argument("--rate", type=float, default=2000.0)
-    ap.add_argument("--duration", type=float, default=10.0)
+    ap.add_argument("--segment-s", type=float, default=0.5,
+                    help="serving: seconds of arrivals per step")
     ap.add_argument("--pool", type=int, default=4096)
-    ap.add_argument("--budget-gb", type=float, default=2.0)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--latency-target-ms", type=float, default=100.0)
     ap.add_argument("--max-batch", type=int, default=1024)
+    ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--sample-scenes", type=int, default=24)
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB copied device-to-device before the run (counter calibration)")
     args = ap.parse_args()
-    if args.copy_calibration > 0:
-        copy_calibration(int(args.copy_calibration * (1 << 30)))
-
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path "
               "would shard k-nodes across GPUs with one all-reduce of V, which this build "
               "does not run.", file=sys.stderr)
         sys.exit(2)
-
+    if args.copy_calibration > 0:
+        copy_calibration(int(args.copy_calibration * (1 << 30)))
     if args.workload == "diff":
         return run_diff(args)
-    if args.workload == "serving":
-        return run_serving(args)
     if args.workload == "align":
         return run_align(args)
-    if args.workload == "mixed":
-        wl = W.config_mixed()
-        if args.gram == "bf16x3" and "--gram" not in sys.argv:
-            args.gram = "fp16x3"
-    else:
-        wl = W.config_icospheres(n_obj=args.n_obj, level=3, seed=2)
-    print(f"[bench] workload {wl.name}: {len(wl.objects)} objects, "
-          f"{wl.prim_count()} elements", file=sys.stderr, flush=True)
-    ctx = P.Context.default()
-    kg = P.kgrid_from_rule(wl.lebedev_order, wl.n_radial)
-    geoms = make_geoms(wl)
-    N = len(geoms)
-    pairs = N * (N - 1) // 2
-    primk = wl.prim_count() * kg.Q
-
-    def step(ff, gram):
-        return P.compute_volume_matrix(geoms, kg, ff=ff, gram=gram)
-
-    for _ in range(args.warmup):
-        step(args.ff, args.gram)
-    ctx.set_profiling(True)
-    ff_ms, gram_ms, pre_ms, ser_ms, face_ms = [], [], [], [], []
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        V = step(args.ff, args.gram)
-        t = ctx.timings()
-        ff_ms.append(t["form_factor"]); gram_ms.append(t["gram"]); pre_ms.append(t["prepass"])
-        ser_ms.append(t["ff_series"]); face_ms.append(t["ff_per_face"])
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    ctx.set_profiling(False)
-    work = ctx.work()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    print(f"[bench] {args.ff}/{args.gram}: {ms_step:.2f} ms/step, FF {np.mean(ff_ms):.2f} ms, "
-          f"Gram {np.mean(gram_ms):.3f} ms", file=sys.stderr, flush=True)
-
-    ref = {}
-    if not args.no_fp64_ref:
-        ctx.set_profiling(True)
-        ctx.synchronize()
-        r0 = time.perf_counter()
-        V64 = step("fp64", "fp64")
-        r1 = time.perf_counter()
-        t64 = ctx.timings()
-        ctx.set_profiling(False)
-        d = np.sqrt(np.abs(np.outer(np.diag(V64), np.diag(V64))))
-        err = np.abs(V - V64) / d
-        ovl = V64 > 1e-3 * d
-
-        def pair_err(Vx):
-            return float((np.abs(Vx - V64)[ovl] / np.abs(V64[ovl])).max())
-
-        # every Gram mode on the same fast form factors: the fp64 Gram shows
-        # the form factors' own share of the error
-        by_mode = {}
-        for g in ("fp64", "bf16x3", "fp16x3"):
-            Vg = step("fast", g)
-            by_mode[g] = {"max_rel_err_overlapping_pairs": pair_err(Vg),
-                          "max_err_scaled": float((np.abs(Vg - V64) / d).max())}
-        ref = {
-            "fp64_ms_per_step": (r1 - r0) * 1e3,
-            "fp64_ff_ms": t64["form_factor"],
-            "fp64_gram_ms": t64["gram"],
-            "speedup_vs_fp64_gpu": (r1 - r0) * 1e3 / ms_step,
-            "max_err_vs_fp64_scaled": float(err.max()),
-            "max_rel_err_overlapping_pairs": float((err[ovl] * d[ovl] / np.abs(V64[ovl])).max()),
-            "overlapping_pairs": int(np.triu(ovl, 1).sum()),
-            "errors_by_gram_mode": by_mode,
-        }
-        print(f"[bench] fp64 reference: {ref['fp64_ms_per_step']:.2f} ms/step, "
-              f"err {ref['max_err_vs_fp64_scaled']:.2e}", file=sys.stderr, flush=True)
-
-    # Roofline of the dominant form-factor kernel, from its HIP-event time and
-    # the work counters of the last call (units evaluated by that launch).
-    kernels = {}
-    if args.ff == "fast":
-        kernels["k_ff_fast (per-element)"] = {
-            "ms": float(np.mean(face_ms)), "units": work["tri_node"] + work["surfel_node"],
-            "unit": "element x node",
-            "flop": work["tri_node"] * FLOP_PER_TRI_K + work["surfel_node"] * FLOP_PER_DISK_K}
-        if work["series_face_dir"] + work["series_surfel_dir"] > 0:
-            kernels["k_series_tri / k_series_surfel (moment series)"] = {
-                "ms": float(np.mean(ser_ms)),
-                "units": work["series_face_dir"] + work["series_surfel_dir"],
-                "unit": "element x direction (+ object x node evaluations)",
-                "flop": (work["series_face_dir"] * FLOP_PER_FACE_DIR
-                         + work["series_surfel_dir"] * FLOP_PER_SURFEL_DIR
-                         + work["series_node"] * FLOP_PER_SERIES_NODE)}
-    else:
-        kernels["k_ff_fp64"] = {"ms": float(np.mean(ff_ms)), "units": wl.prim_count() * kg.Q,
-                                "unit": "element x node",
-                                "flop": wl.prim_count() * kg.Q * FLOP_PER_TRI_K}
-    for k in kernels.values():
-        k["tflops"] = k["flop"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
-    dom = max(kernels, key=lambda n: kernels[n]["ms"])
-    ach = kernels[dom]["tflops"]
-    traffic = MEASURED_TRAFFIC.get((args.workload, dom)) if args.ff == "fast" else None
-    roofline = {"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_VALU_TFLOPS, "traffic": traffic,
-                "kernel": dom, "kernel_ms": kernels[dom]["ms"],
-                "units_per_launch": kernels[dom]["units"], "form_factor_kernels": kernels}
-    if args.ff == "fast" and args.workload == "icospheres":
-        # algorithmic bytes of the per-element launch: face records (64 B),
-        # world vertices (24 B), representative grid (64 B per node) in; X
-        # (8 B per object x node it evaluates) out
-        n_direct = work["tri_node"] / (wl.prim_count() / N)
-        roofline["algorithmic_bytes"] = (64.0 * wl.prim_count()
-                                         + 24.0 * sum(o.V.shape[0] for o in wl.objects)
-                                         + 64.0 * kg.Qh + 8.0 * n_direct)
-
-    # The Gram against both of its roofs: algorithmic flops 4 Q per pair over
-    # the N (N + 1) / 2 unique pairs (the split's x3 products not counted),
-    # bytes X read once (8 B per representative node and object) + V written.
-    gram_flop = 2.0 * kg.Q * N * (N + 1)
-    gram_bytes = 8.0 * kg.Qh * N + 8.0 * N * N
-    g_s = float(np.mean(gram_ms)) * 1e-3
-    gram_roof = {"mfma_tflops": gram_flop / g_s / 1e12,
-                 "mfma_frac": gram_flop / g_s / 1e12 / PEAK_MFMA_F16_TFLOPS,
-                 "hbm_gbps": gram_bytes / g_s / 1e9, "hbm_frac": gram_bytes / g_s / 1e9 / 8000.0,
-                 "algorithmic_flop": gram_flop, "algorithmic_bytes": gram_bytes}
-    # sustained issue rates of this device from probe kernels, beside the
-    # published peaks (the chip's clock under load sets the difference)
-    measured_peaks = {"fp64_fma_tflops": ctx.issue_rate("fp64_fma") / 1e12,
-                      "fp32_fma_tflops": ctx.issue_rate("fp32_fma") / 1e12,
-                      "fp32_sin_per_s": ctx.issue_rate("sin")}
-
-    cpu = None
-    if not args.no_cpu:
-        t_full, nq, t_s, cores = cpu_baseline(wl, kg, args.cpu_seconds)
-        cpu = {"value": pairs / t_full, "unit": "pairs/s", "cores": cores, "kind": "port",
-               "sample": f"FP64 oracle on {nq} of {kg.Q} k-nodes (every {round(kg.Q / nq)}th), all "
-                         f"{N} objects, {t_s:.1f} s, extrapolated x{kg.Q / nq:.1f}; "
-                         f"host {cpu_model()}",
-               "ms_per_step_extrapolated": t_full * 1e3}
-        print(f"[bench] cpu oracle: {t_full:.1f} s/step extrapolated on {cores} threads",
-              file=sys.stderr, flush=True)
-
-    out = {
-        "metric": "overlap pairs/s & prim*k evals/s vs plain FP64 at 1e-5 rel err; p50/p99 latency",
-        "value": pairs / (ms_step * 1e-3),
-        "unit": "pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64" if args.ff == "fast" else "f64",
-        "data": "synthetic (seeded)",
-        "config": {"workload": ("256 icospheres L3 (1,280 tris), Lebedev 1202 x L=64 "
-                                "(Q=76,928), fast form factors (FP64 moment series for "
-                                "|k|R <= 15, per-face FP32/FP64 above), BF16x3 Gram")
-                               if args.workload == "icospheres" else
-                               ("1,024 mixed objects (512 star meshes T in [80, 20480], "
-                                "512 disk clouds in [1k, 50k]), Lebedev 2702 x L=96 "
-                                f"(Q=259,392), {args.gram} Gram"),
-                   "objects": N, "triangles": wl.prim_count(), "Q": kg.Q,
-                   "ff": args.ff, "gram": args.gram, "parallelism": "single GPU"},
-        "prim_k_per_s": primk / (ms_step * 1e-3),
-        "pairs_per_step": pairs,
-        "ff_ms": float(np.mean(ff_ms)),
-        "gram_ms": float(np.mean(gram_ms)),
-        "prepass_ms": float(np.mean(pre_ms)),
-        "roofline": roofline,
-        "gram_roofline": gram_roof,
-        "measured_peaks": measured_peaks,
-        "cpu_baseline": cpu,
-    }
-    out.update(ref)
-    if ref:
-        out["phase_speedup_vs_fp64"] = {
-            "form_factor": ref["fp64_ff_ms"] / out["ff_ms"],
-            "gram": ref["fp64_gram_ms"] / out["gram_ms"]}
-    print(json.dumps(out))
+    if args.workload == "serving":
+        return run_serving(args)
+    return run_matrix(args)
 
 
 if __name__ == "__main__":

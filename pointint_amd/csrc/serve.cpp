@@ -20,12 +20,12 @@
 // free slot, so consecutive batches overlap on the device.
 //
 // Graphs.  The launch sequence of a batch is laid out for a capacity (objects
-// rounded up to a bucket, V entries rounded up to a bucket): descriptors past
+// rounded up to a bucket, V room for any batch of that many): descriptors past
 // the batch's objects are empty, the grid-stride and persistent kernels read
 // their counts from device memory, and the Gram reads a device plan.  So the
 // sequence of one capacity is captured once into a hipGraph and replayed for
 // every later batch of that capacity, the host only rewriting the pinned
-// staging it uploads.  Replay and direct launch run the same kernels with the
+// staging it uploads; the copy of the results (exact size) follows it.  Replay and direct launch run the same kernels with the
 // same arguments: results are bit-identical, and a query's result does not
 // depend on the batch it is part of (per-object form factors; per-query Gram
 // slices).
@@ -102,7 +102,7 @@ struct Slot {
   Batch batch;
   HostBuf out;                // pinned: V of every query of the batch, then the stamps
   DevBuf Vdev, stamps;
-  std::map<std::pair<long long, long long>, Graph> graphs;
+  std::map<long long, Graph> graphs;     // by object capacity
   // the batch in flight
   long long v_cap = 0;
   double work = 0, tri_node = 0, face_dir = 0, t_launch = 0;
@@ -197,10 +197,15 @@ struct Server {
                        kGramGrid, v_cap, s.w.part.as<double>(), colmax, s.Vdev.as<double>());
     launch_stamp(st, stamps, 4);
     PII_HIP(hipGetLastError());
+  }
+
+  // The batch's results to pinned memory (exact sizes, outside the graph).
+  void copy_out(Slot& s, long long nentries) {
     double* out = s.out.as<double>();
-    PII_HIP(hipMemcpyAsync(out, s.Vdev.p, sizeof(double) * v_cap, hipMemcpyDeviceToHost, st));
-    PII_HIP(hipMemcpyAsync(out + v_cap, s.stamps.p, sizeof(double) * kStamps,
-                           hipMemcpyDeviceToHost, st));
+    PII_HIP(hipMemcpyAsync(out, s.Vdev.p, sizeof(double) * nentries, hipMemcpyDeviceToHost,
+                           s.stream));
+    PII_HIP(hipMemcpyAsync(out + s.v_cap, s.stamps.p, sizeof(double) * kStamps,
+                           hipMemcpyDeviceToHost, s.stream));
   }
 
   // FP64 Gram mode: the plain per-query FP64 Grams, launched directly.
@@ -276,7 +281,9 @@ struct Server {
       GramPlanHeader h{};
       gram_plan_layout(qs, h);
       for (size_t qi = 0; qi < s.reqs.size(); ++qi) s.reqs[qi]->v_off = (size_t)qs[qi].v;
-      s.v_cap = bucket(std::max<long long>(h.nentries, 1), 4096);
+      // V room for any batch of n_cap objects (queries of at most
+      // max_batch_objects each): the graph of a capacity depends on n_cap only
+      s.v_cap = (long long)n_cap * std::min(n_cap, max_batch_objects);
       // buffers whose addresses a captured graph holds
       s.w.X.ensure(sizeof(float2) * (size_t)ld * nc);
       s.w.colmax.ensure(sizeof(unsigned) * (size_t)nc);
@@ -296,7 +303,7 @@ struct Server {
           s.w.misc.p, s.w.stage.p, s.w.gplan.p, s.w.gplan_h.p, s.w.X.p,   s.w.colmax.p,
           s.w.planes.p, s.w.part.p, s.Vdev.p,  s.stamps.p,   s.out.p,   kg->rep.p};
       if (use_graphs) {
-        const auto key = std::make_pair((long long)n_cap, s.v_cap);
+        const long long key = n_cap;
         auto it = s.graphs.find(key);
         if (it != s.graphs.end() && it->second.baked != baked) {
           (void)hipGraphExecDestroy(it->second.exec);
@@ -327,6 +334,7 @@ struct Server {
       } else {
         enqueue(s, n_cap, nc, s.v_cap, plan_bytes);
       }
+      copy_out(s, h.nentries);
     }
     // work of the batch's form-factor kernels
     s.tri_node = s.face_dir = 0;

@@ -149,6 +149,9 @@ class LoadReport:
     queue_ms: List[float] = field(default_factory=list)
     submit_lag_ms: List[float] = field(default_factory=list)
     scenes: List[Scene] = field(default_factory=list)
+    # per submitted query: (arrival, completion or nan when refused, K,
+    # triangles); seconds from the start of the stream
+    records: List[tuple] = field(default_factory=list)
 
     def summary(self) -> dict:
         lat = np.array(self.latencies_ms) if self.latencies_ms else np.array([float("nan")])
@@ -181,9 +184,10 @@ def poisson_load(server: Server, pool: Sequence[core.Geometry], rate: float, dur
     transforms.  Arrival times are scheduled in advance on the engine's clock
     and passed with each submission, so latencies (completion - scheduled
     arrival) include any lag of this generator.  Submission runs on this
-    thread, collection on another.  The first `keep` completed queries keep
-    their scene and result (LoadReport.scenes)."""
+    thread, collection on another.  `keep` completed queries, a uniform
+    sample of all of them, keep their scene and result (LoadReport.scenes)."""
     rng = np.random.default_rng(seed)
+    pick = np.random.default_rng(seed + 1)
     n_arr = int(rate * duration * 1.2) + 16
     gaps = rng.exponential(1.0 / rate, size=n_arr)
     arrivals = np.cumsum(gaps)
@@ -205,14 +209,22 @@ def poisson_load(server: Server, pool: Sequence[core.Geometry], rate: float, dur
                     continue
                 time.sleep(0.0005)
                 continue
-            qid, k, idx, dofs = item
+            qid, k, idx, dofs, ta, tris = item
             V, t = server.wait_times(qid, k)
             rep.latencies_ms.append((t[3] - t[0]) * 1e3)
             rep.queue_ms.append((t[2] - t[0]) * 1e3)
+            rep.records.append((ta, t[3] - t0, k, tris))
             rep.completed += 1
             rep.pairs_completed += k * (k - 1) // 2
-            if len(rep.scenes) < keep:
-                rep.scenes.append(Scene(idx, dofs, V, (t[3] - t[0]) * 1e3, (t[2] - t[0]) * 1e3))
+            if keep > 0:
+                # reservoir sample of the completed queries
+                sc = Scene(idx, dofs, V, (t[3] - t[0]) * 1e3, (t[2] - t[0]) * 1e3)
+                if len(rep.scenes) < keep:
+                    rep.scenes.append(sc)
+                else:
+                    j = int(pick.integers(rep.completed))
+                    if j < keep:
+                        rep.scenes[j] = sc
 
     th = threading.Thread(target=collector, daemon=True)
     th.start()
@@ -223,6 +235,7 @@ def poisson_load(server: Server, pool: Sequence[core.Geometry], rate: float, dur
         idx = rng.integers(0, len(pool), size=k)
         dofs = rigid_dofs(rng, box, k)
         geoms = [pool[i] for i in idx]
+        tris = sum(g.num_elements() for g in geoms)
         t_sched = t0 + ta
         delay = t_sched - clock()
         if delay > 0:
@@ -232,10 +245,11 @@ def poisson_load(server: Server, pool: Sequence[core.Geometry], rate: float, dur
         rep.submit_lag_ms.append(max(0.0, clock() - t_sched) * 1e3)
         if qid is None:
             rep.refused += 1
+            rep.records.append((ta, float("nan"), k, tris))
             continue
         rep.admitted += 1
         with lock:
-            pending.append((qid, k, idx, dofs))
+            pending.append((qid, k, idx, dofs, ta, tris))
     done_flag.set()
     th.join()
     rep.wall_s = time.perf_counter() - w0
