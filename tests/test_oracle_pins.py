@@ -119,6 +119,69 @@ def test_cube_transform_is_exact(oracle):
         assert abs(oracle.compute_A(g, k) - ex) < 1e-9 * abs(ex) + 1e-12 * np.linalg.norm(k)
 
 
+def test_unit_cube_mesh(oracle):
+    """The oracle's unit cube (create_unit_cube_mesh) is the workloads' one, a
+    closed outward-oriented mesh of volume 1 (divergence theorem)."""
+    V, F = oracle.unit_cube()
+    Vw, Fw = W.unit_cube()
+    assert np.array_equal(V, Vw) and np.array_equal(F, Fw)
+    a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    assert abs(np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0 - 1.0) < 1e-15
+    assert np.allclose(np.cross(b - a, c - a).sum(axis=0), 0.0)
+
+
+def test_exact_sphere_transform(oracle):
+    """F_ball(k) = 4 pi (sin kR - kR cos kR) / k^3 against 40-digit mpmath on
+    both sides of the k < 1e-4 series window, its k -> 0 limit 4 pi R^3 / 3,
+    and Parseval: sum_q w_q k^2 F^2 / (8 pi^3) = volume on the Lebedev 2702 x
+    L=96 grid within the reference's 5 % (cpp/test/test_sphere_pointcloud.cpp:100-137)."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    for R in (1.0, 0.37, 2.5):
+        for k in (1e-7, 3e-5, 9.9e-5, 1.01e-4, 1e-3, 0.3, 1.0, 7.0, 55.0):
+            kk, RR = mp.mpf(k), mp.mpf(R)
+            ex = 4 * mp.pi * (mp.sin(kk * RR) - kk * RR * mp.cos(kk * RR)) / kk ** 3
+            got = oracle.exact_sphere_ff(k, R)
+            # the closed form in FP64: rounding of kR and of the two terms
+            # (about eps (kR + 1)^2 / k^3) and, just above the window, their
+            # cancellation (relative eps (kR)^-2); the series below it: (kR)^6 / 15120
+            if k >= 1e-4:
+                tol = 1e-15 * 4 * math.pi * (k * R + 1) ** 2 / k ** 3 + \
+                    4e-16 / (k * R) ** 2 * abs(float(ex))
+            else:
+                tol = 1e-14 * abs(float(ex))
+            assert abs(got - float(ex)) <= tol, (R, k)
+        assert abs(oracle.exact_sphere_ff(0.0, R) - 4 * math.pi * R ** 3 / 3) < 1e-14 * R ** 3
+    d, w = oracle.load_lebedev(os.path.join(LEB, "lebedev_089.txt"))
+    dirs, kmag, wq = oracle.build_kgrid(d, w, 96)
+    F = np.array([oracle.exact_sphere_ff(k) for k in kmag])
+    vol = (wq * F ** 2 * kmag ** 2).sum() / (8 * math.pi ** 3)
+    assert abs(vol / (4 * math.pi / 3) - 1.0) < 0.05
+
+
+def test_sphere_pointcloud_converges_to_ball(oracle):
+    """A Fibonacci disk cloud on the unit sphere (create_sphere_pointcloud):
+    points and normals on the sphere, disk areas summing to 4 pi, and its
+    A(k) -> i|k| F_ball(k) as the cloud refines (error falling as 1/n), at the
+    reference test's k-points (cpp/test/test_sphere_pointcloud.cpp:35-42)."""
+    ks = [((1, 0, 0), 0.1), ((1, 0, 0), 1.0), ((0, 1, 0), 1.0), ((0, 0, 1), 2.0),
+          ((0.707, 0.707, 0), 1.414), ((0.3, -0.5, 0.8), 3.0)]
+    errs = []
+    for n in (1000, 4000):
+        P, N, r = oracle.sphere_pointcloud(n)
+        assert np.allclose(np.linalg.norm(P, axis=1), 1.0, atol=1e-15) and np.array_equal(P, N)
+        assert abs((math.pi * r ** 2).sum() - 4 * math.pi) < 1e-12
+        g = oracle.geom_disks(P, N, r)
+        e = 0.0
+        for kd, km in ks:
+            kd = np.array(kd, float) / np.linalg.norm(kd)
+            ex = 1j * km * oracle.exact_sphere_ff(km)
+            e = max(e, abs(oracle.compute_A(g, kd * km) - ex) / abs(ex))
+        errs.append(e)
+    assert errs[0] < 3e-3
+    assert 3.0 < errs[0] / errs[1] < 5.0
+
+
 def test_affine_cube_is_mapped_box(oracle):
     """A(k) of A*cube + t equals i|k| |det A| e^{ik.t} F_cube(A^T k)."""
     V, F = W.unit_cube()
