@@ -65,12 +65,23 @@ enum { PII_GRAM_FP64 = 0, PII_GRAM_BF16X3 = 1, PII_GRAM_FP16X3 = 2 };
 /* ComputationFlags: cpp/include/computation_flags.hpp:12-17 */
 enum { PII_FLAG_VOLUME = 1, PII_FLAG_GRADIENT = 2, PII_FLAG_HESSIAN = 4 };
 
+/* On any error the matrix and form-factor entries leave zeros in the
+ * outputs they were given (whose sizes the arguments define), as the
+ * reference drivers return zero results
+ * (cpp/src/compute_intersection_volume_multi_object.cu:250-261). */
+
 typedef struct pii_context pii_context;
 typedef struct pii_kgrid pii_kgrid;
 typedef struct pii_geometry pii_geometry;
 
 const char* pii_last_error(void);
 const char* pii_version(void);
+
+/* Testing hook, no counterpart in the reference: after `after` more
+ * successful device or pinned-host allocations, the next one fails as an
+ * out-of-memory would (PII_ERR_NOMEM); -1 turns it off.  Lets the error
+ * paths be tested without exhausting the device. */
+int pii_debug_fail_allocation(int after);
 
 /* ---- host quadrature utilities ------------------------------------------ */
 /* Gauss-Legendre nodes/weights on [a,b] (ascending).
@@ -159,7 +170,12 @@ int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const
 /* The same A(k_q) written into caller-owned DEVICE memory d_out[2*Q] (device
  * memory of the context's device, else PII_ERR_INVALID), synchronously: the
  * contract of the CudaComputeAkFunc plug-ins, which write into a device
- * double2* (cpp/include/dof/cuda/dof_cuda_interface.hpp:24-30). */
+ * double2* (cpp/include/dof/cuda/dof_cuda_interface.hpp:24-30).
+ * Ordering: the writes follow all work queued on the caller's stream before
+ * the call (the null stream, as the reference's plug-ins use, unless
+ * pii_context_set_caller_stream names another), and are complete when the
+ * call returns, so work queued on any stream afterwards sees them. */
+int pii_context_set_caller_stream(pii_context* ctx, void* hip_stream);
 int pii_form_factor_device(pii_context* ctx, const pii_geometry* g, int dof_kind,
                            const double* dofs, int ndofs, const pii_kgrid* kg, int precision,
                            double* d_out);
@@ -179,8 +195,9 @@ int pii_form_factor_jacobian(pii_context* ctx, const pii_geometry* g, int dof_ki
                              const double* dofs, int ndofs, const pii_kgrid* kg, double* out);
 
 /* The same Jacobian written into caller-owned DEVICE memory d_out[2*Q*ndofs]
- * (row-major Q x ndofs complex), synchronously: the CudaComputeAkGradientFunc
- * plug-in contract (cpp/include/dof/cuda/dof_cuda_interface.hpp:34-40). */
+ * (row-major Q x ndofs complex), synchronously, ordered as
+ * pii_form_factor_device: the CudaComputeAkGradientFunc plug-in contract
+ * (cpp/include/dof/cuda/dof_cuda_interface.hpp:34-40). */
 int pii_form_factor_jacobian_device(pii_context* ctx, const pii_geometry* g, int dof_kind,
                                     const double* dofs, int ndofs, const pii_kgrid* kg,
                                     double* d_out);

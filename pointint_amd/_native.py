@@ -39,6 +39,7 @@ _S = C.c_char_p
 SIGNATURES = {
     "pii_last_error": ([], _S),
     "pii_version": ([], _S),
+    "pii_debug_fail_allocation": ([_I], _I),
     "pii_gauss_legendre": ([_I, _D, _D, _P, _P], _I),
     "pii_lebedev_load": ([_S, _P, _P, _I, _P], _I),
     "pii_kgrid_build": ([_P, _P, _I, _I, _P, _P, _P], _I),
@@ -49,6 +50,7 @@ SIGNATURES = {
     "pii_context_timings": ([_P, _P, _I], _I),
     "pii_context_set_series": ([_P, _I], _I),
     "pii_context_work": ([_P, _P, _I], _I),
+    "pii_context_set_caller_stream": ([_P, _P], _I),
     "pii_volume_matrix_hessian": ([_P, _I, _P, _P, _P, _P, _P, _I, _P, _P], _I),
     "pii_volume_matrix_affine_derivatives": ([_P, _I, _P, _P, _P, _I, _P, _P, _P], _I),
     "pii_volume_matrix_derivatives": ([_P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P], _I),

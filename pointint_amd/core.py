@@ -311,6 +311,10 @@ def compute_form_factor_into(out, geom: Geometry, kgrid: KGrid, dofs=None, dof_k
             and out.element_size() == 8 and out.numel() == 2 * kgrid.Q):
         raise ValueError("out must be a contiguous float64 CUDA tensor of 2 Q elements")
     kinds, dl = _dof_args([geom], None if dofs is None else [dofs], [dof_kind])
+    # the library's writes follow what torch has queued on its current stream
+    import torch
+    N.check(N.load().pii_context_set_caller_stream(
+        geom.ctx.handle, C.c_void_p(torch.cuda.current_stream(out.device).cuda_stream)))
     N.check(N.load().pii_form_factor_device(geom.ctx.handle, geom.handle, kinds[0],
                                             dl[0].ctypes.data, dl[0].shape[0], kgrid.handle,
                                             _FF[precision], out.data_ptr()))

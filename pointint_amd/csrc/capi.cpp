@@ -28,11 +28,26 @@ void DevBuf::release() {
   }
 }
 
+// Allocation fault injection for the error-path tests
+// (pii_debug_fail_allocation): after `g_fail_after` more successful
+// allocations the next one fails as an out-of-memory would.  -1: off.
+static std::atomic<int> g_fail_after{-1};
+static bool injected_allocation_failure() {
+  int v = g_fail_after.load();
+  while (v >= 0) {
+    if (g_fail_after.compare_exchange_weak(v, v - 1)) return v == 0;
+  }
+  return false;
+}
+
 void DevBuf::ensure(size_t n) {
   if (n <= bytes && p) return;
   release();
   if (n == 0) return;
   const size_t want = std::max<size_t>(n, 256);
+  if (injected_allocation_failure())
+    throw Error(PII_ERR_NOMEM, "hipMalloc of " + std::to_string(want) +
+                                   " bytes failed (injected for testing)");
   if (hipMalloc(&p, want) != hipSuccess) {
     p = nullptr;
     (void)hipGetLastError();
@@ -54,6 +69,9 @@ void HostBuf::ensure(size_t n) {
   release();
   if (n == 0) return;
   const size_t want = std::max<size_t>(n, 4096);
+  if (injected_allocation_failure())
+    throw Error(PII_ERR_NOMEM, "hipHostMalloc of " + std::to_string(want) +
+                                   " bytes failed (injected for testing)");
   if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
     p = nullptr;
     (void)hipGetLastError();
@@ -483,6 +501,33 @@ void volume_matrix_fp64(Context& c, const Batch& b, const KGridDev& kg, double* 
 
 }  // namespace pii
 
+namespace pii {
+
+// A failed call leaves zero results in the caller's outputs, as the
+// reference drivers return zero matrices on an error
+// (cpp/src/compute_intersection_volume_multi_object.cu:250-261).
+template <class F, class Z>
+static int guarded_zero(F&& f, Z&& zero) {
+  const int rc = guarded(std::forward<F>(f));
+  if (rc != PII_OK) zero();
+  return rc;
+}
+static void zero_fill(double* p, long long n) {
+  if (p && n > 0) std::fill(p, p + n, 0.0);
+}
+// Sum of the DoF counts, or -1 when the arrays are unusable.
+static long long dof_total(int n, const int* ndofs) {
+  if (n <= 0 || !ndofs) return -1;
+  long long d = 0;
+  for (int i = 0; i < n; ++i) {
+    if (ndofs[i] < 0) return -1;
+    d += ndofs[i];
+  }
+  return d;
+}
+
+}  // namespace pii
+
 using namespace pii;
 
 extern "C" {
@@ -548,6 +593,7 @@ int pii_context_destroy(pii_context* ctx) {
     for (auto& pe : ctx->c.ev)
       for (auto& e : pe)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->c.caller_ev) (void)hipEventDestroy(ctx->c.caller_ev);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
   });
@@ -898,13 +944,39 @@ static void form_factor_to(pii_context* ctx, const pii_geometry* g, int dof_kind
   }
 }
 
+int pii_debug_fail_allocation(int after) {
+  return guarded([&] {
+    require(after >= -1, "pii_debug_fail_allocation: after must be >= -1");
+    g_fail_after.store(after);
+  });
+}
+
 int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const double* dofs,
                     int ndofs, const pii_kgrid* kg, int precision, double* out) {
-  return guarded([&] {
+  return guarded_zero([&] {
     require(ctx && g && kg && dofs && out, "pii_form_factor: NULL argument");
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     PII_HIP(hipSetDevice(ctx->c.device));
     form_factor_to(ctx, g, dof_kind, dofs, ndofs, kg, precision, out, hipMemcpyDeviceToHost);
+  }, [&] { if (kg) zero_fill(out, 2LL * kg->k.Q); });
+}
+
+// The context stream waits for the work the caller has queued on its stream
+// (the null stream unless pii_context_set_caller_stream named another), so
+// that a fill or read of the caller's buffer queued before the call is
+// ordered before the library's writes into it.
+static void order_after_caller(pii_context* ctx) {
+  if (!ctx->c.caller_ev)
+    PII_HIP(hipEventCreateWithFlags(&ctx->c.caller_ev, hipEventDisableTiming));
+  PII_HIP(hipEventRecord(ctx->c.caller_ev, ctx->c.caller_stream));
+  PII_HIP(hipStreamWaitEvent(ctx->c.stream, ctx->c.caller_ev, 0));
+}
+
+int pii_context_set_caller_stream(pii_context* ctx, void* stream) {
+  return guarded([&] {
+    require(ctx != nullptr, "NULL context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    ctx->c.caller_stream = static_cast<hipStream_t>(stream);
   });
 }
 
@@ -916,6 +988,7 @@ int pii_form_factor_device(pii_context* ctx, const pii_geometry* g, int dof_kind
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     PII_HIP(hipSetDevice(ctx->c.device));
     require_device_pointer(d_out, ctx->c.device, "pii_form_factor_device");
+    order_after_caller(ctx);
     form_factor_to(ctx, g, dof_kind, dofs, ndofs, kg, precision, d_out,
                    hipMemcpyDeviceToDevice);
   });
@@ -954,6 +1027,7 @@ int pii_form_factor_jacobian_device(pii_context* ctx, const pii_geometry* g, int
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     PII_HIP(hipSetDevice(ctx->c.device));
     require_device_pointer(d_out, ctx->c.device, "pii_form_factor_jacobian_device");
+    order_after_caller(ctx);
     jacobian_to(ctx, g, dof_kind, dofs, ndofs, kg, d_out, hipMemcpyDeviceToDevice);
   });
 }
@@ -961,7 +1035,7 @@ int pii_form_factor_jacobian_device(pii_context* ctx, const pii_geometry* g, int
 int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
                       const int* dof_kinds, const double* const* dofs, const int* ndofs,
                       const pii_kgrid* kg, int ff_precision, int gram_mode, double* V) {
-  return guarded([&] {
+  return guarded_zero([&] {
     require(ctx && kg && V, "pii_volume_matrix: NULL argument");
     require(n >= 0, "pii_volume_matrix: negative object count");
     if (n == 0) return;
@@ -981,14 +1055,14 @@ int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
     } else {
       throw Error(PII_ERR_INVALID, "unsupported precision / Gram mode combination");
     }
-  });
+  }, [&] { if (n > 0) zero_fill(V, (long long)n * n); });
 }
 
 int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* geoms,
                           const int* dof_kinds, const double* const* dofs, const int* ndofs,
                           const pii_kgrid* kg, const double* dL_dV, double* const* grads,
                           double* V) {
-  return guarded([&] {
+  return guarded_zero([&] {
     require(ctx && kg && dL_dV && grads, "pii_volume_matrix_vjp: NULL argument");
     require(n > 0 && geoms && dof_kinds && dofs && ndofs, "pii_volume_matrix_vjp: bad arrays");
     for (int i = 0; i < n; ++i)
@@ -1000,13 +1074,17 @@ int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* ge
     Batch b;
     b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
     volume_matrix_vjp(ctx->c, b, gs.data(), kg->k, dL_dV, grads, V);
+  }, [&] {
+    if (n > 0) zero_fill(V, (long long)n * n);
+    if (grads && dof_total(n, ndofs) >= 0)
+      for (int i = 0; i < n; ++i) zero_fill(grads[i], ndofs[i]);
   });
 }
 
 int pii_volume_matrix_hessian(pii_context* ctx, int n, const pii_geometry* const* geoms,
                               const int* dof_kinds, const double* const* dofs, const int* ndofs,
                               const pii_kgrid* kg, int gram_mode, double* H, double* V) {
-  return guarded([&] {
+  return guarded_zero([&] {
     require(ctx && kg && H, "pii_volume_matrix_hessian: NULL argument");
     require(n > 0 && geoms && dof_kinds && dofs && ndofs,
             "pii_volume_matrix_hessian: bad arrays");
@@ -1018,6 +1096,10 @@ int pii_volume_matrix_hessian(pii_context* ctx, int n, const pii_geometry* const
     Batch b;
     b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
     volume_matrix_hessian(ctx->c, b, kg->k, gram_mode, ndofs, H, V);
+  }, [&] {
+    if (n > 0) zero_fill(V, (long long)n * n);
+    const long long D = dof_total(n, ndofs);
+    if (D > 0 && D <= kMaxDerivativeColumns) zero_fill(H, D * D);
   });
 }
 
@@ -1025,7 +1107,7 @@ int pii_volume_matrix_derivatives(pii_context* ctx, int n, const pii_geometry* c
                                   const int* dof_kinds, const double* const* dofs,
                                   const int* ndofs, const pii_kgrid* kg, int gram_mode, double* V,
                                   double* grad, double* H) {
-  return guarded([&] {
+  return guarded_zero([&] {
     require(ctx && kg, "pii_volume_matrix_derivatives: NULL argument");
     require(V || grad || H, "pii_volume_matrix_derivatives: no output requested");
     require(n > 0 && geoms && dof_kinds && dofs && ndofs,
@@ -1039,13 +1121,18 @@ int pii_volume_matrix_derivatives(pii_context* ctx, int n, const pii_geometry* c
     Batch b;
     b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
     volume_matrix_derivatives(ctx->c, b, kg->k, gram_mode, ndofs, V, grad, H);
+  }, [&] {
+    if (n > 0) zero_fill(V, (long long)n * n);
+    const long long D = dof_total(n, ndofs);
+    if (D >= 0) zero_fill(grad, (long long)n * D);
+    if (D > 0 && D <= kMaxDerivativeColumns) zero_fill(H, D * D);
   });
 }
 
 int pii_volume_matrix_affine_derivatives(pii_context* ctx, int n, const pii_geometry* const* geoms,
                                          const double* const* dofs, const pii_kgrid* kg,
                                          int gram_mode, double* V, double* grad, double* H) {
-  return guarded([&] {
+  return guarded_zero([&] {
     require(ctx && kg, "pii_volume_matrix_affine_derivatives: NULL argument");
     require(V || grad || H, "pii_volume_matrix_affine_derivatives: no output requested");
     require(n > 0 && geoms && dofs, "pii_volume_matrix_affine_derivatives: bad arrays");
@@ -1059,6 +1146,12 @@ int pii_volume_matrix_affine_derivatives(pii_context* ctx, int n, const pii_geom
     Batch b;
     b.build(ctx->c, n, gs.data(), kinds.data(), dofs, nd.data(), true, ctx->c.series_grid(kg->k));
     volume_matrix_affine_all(ctx->c, b, kg->k, gram_mode, V, grad, H);
+  }, [&] {
+    if (n > 0) {
+      zero_fill(V, (long long)n * n);
+      zero_fill(grad, 12LL * n * n);
+      if (12LL * n <= kMaxDerivativeColumns) zero_fill(H, 144LL * n * n);
+    }
   });
 }
 

@@ -202,7 +202,12 @@ static void column_gram(Context& c, const KGridDev& kg, int gram_mode, int D, in
 static int affine_gram(Context& c, const Batch& b, const KGridDev& kg, int gram_mode,
                        bool with_A) {
   const int N = b.N, Qh = kg.Qh;
-  const int na = with_A ? N : 0, D = na + 12 * N;
+  const int na = with_A ? N : 0;
+  if ((long long)na + 12LL * N > kMaxDerivativeColumns)
+    throw Error(PII_ERR_INVALID, "affine derivative Gram: " + std::to_string(na + 12LL * N) +
+                                     " columns exceed the limit of " +
+                                     std::to_string(kMaxDerivativeColumns));
+  const int D = na + 12 * N;
   for (const ObjDesc& d : b.h_objs)
     if (d.dof_kind != PII_DOF_AFFINE)
       throw Error(PII_ERR_UNSUPPORTED, "affine derivatives support AffineDoF objects only");

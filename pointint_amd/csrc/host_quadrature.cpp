@@ -128,26 +128,25 @@ bool kgrid_pairs(int Q, const double* dirs, const double* kmag, const double* w,
     // lexicographic lookup of the negated direction.
     std::vector<int> idx(end - beg);
     for (int i = beg; i < end; ++i) idx[i - beg] = i;
-    auto key_less = [&](const double* a, const double* b) {
-      for (int c = 0; c < 3; ++c) {
-        if (a[c] < b[c] - 1e-12) return true;
-        if (a[c] > b[c] + 1e-12) return false;
-      }
-      return false;
-    };
-    std::sort(idx.begin(), idx.end(),
-              [&](int a, int b) { return key_less(dirs + 3 * (size_t)a, dirs + 3 * (size_t)b); });
+    // Exact lexicographic order (a strict weak ordering, as std::sort
+    // requires); the 1e-12 tolerance applies only to the antipode match.
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) {
+      const double* p = dirs + 3 * (size_t)a;
+      const double* q = dirs + 3 * (size_t)b;
+      if (p[0] != q[0]) return p[0] < q[0];
+      if (p[1] != q[1]) return p[1] < q[1];
+      return p[2] < q[2];
+    });
     for (int i = beg; i < end && ok; ++i) {
       if (anti[i] >= 0) continue;
       const double neg[3] = {-dirs[3 * (size_t)i], -dirs[3 * (size_t)i + 1],
                              -dirs[3 * (size_t)i + 2]};
-      auto it = std::lower_bound(idx.begin(), idx.end(), -1, [&](int a, int) {
-        return key_less(dirs + 3 * (size_t)a, neg);
-      });
+      // candidates: x within 1e-12 of -x_i, a contiguous run of the order
+      auto it = std::lower_bound(idx.begin(), idx.end(), neg[0] - 1e-12,
+                                 [&](int a, double x) { return dirs[3 * (size_t)a] < x; });
       int found = -1;
-      for (; it != idx.end(); ++it) {
+      for (; it != idx.end() && dirs[3 * (size_t)*it] <= neg[0] + 1e-12; ++it) {
         const double* d = dirs + 3 * (size_t)*it;
-        if (key_less(neg, d)) break;
         double s = std::abs(d[0] - neg[0]) + std::abs(d[1] - neg[1]) + std::abs(d[2] - neg[2]);
         if (s < 1e-12 && *it != i && anti[*it] < 0) { found = *it; break; }
       }

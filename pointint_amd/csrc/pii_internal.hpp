@@ -187,6 +187,10 @@ struct Workspace {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // The caller's stream (nullptr: the null stream) that the device-output
+  // entries order their writes after, and an event to do so.
+  hipStream_t caller_stream = nullptr;
+  hipEvent_t caller_ev = nullptr;
   std::mutex mu;                     // serialises calls on one context
   Workspace w;
   // Moment series for low-|k| shells of triangle objects on the fast path.

@@ -6,7 +6,12 @@
 // the whole cross Hessian under vertex DoFs, and A(k) with dA(k)/dtheta of
 // the second cube at one k under an affine map.
 //
-//   usage: example_cube_pair <lebedev_029.txt> [fp64|fast]
+// Mode "mismatch" runs no device code: the matrix driver called with fewer
+// DoF parameterisations / DoF vectors than geometries prints its message and
+// returns the zero matrix, as the reference's driver does
+// (cpp/src/compute_intersection_volume_multi_object.cu:255-261).
+//
+//   usage: example_cube_pair <lebedev_029.txt> [fp64|fast|mismatch]
 #include <cstdio>
 #include <cstring>
 
@@ -31,6 +36,18 @@ int main(int argc, char** argv) {
   for (int i = 0; i < V2.rows(); ++i)
     for (int c = 0; c < 3; ++c) V2(i, c) += shift[c];
   std::vector<Geometry> geoms = {make_triangle_mesh(V, F), make_triangle_mesh(V2, F)};
+  if (argc > 2 && std::strcmp(argv[2], "mismatch") == 0) {
+    auto a = std::make_shared<AffineDoF>();
+    const VectorXd id = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+    auto r1 = compute_intersection_volume_matrix_cuda(geoms, {a}, {id, id}, kg);
+    auto r2 = compute_intersection_volume_matrix_cuda(geoms, {a, a}, {id}, kg,
+                                                      ComputationFlags::GRADIENT);
+    for (const auto* r : {&r1, &r2})
+      std::printf("mismatch %d %d %.17g %.17g %.17g %.17g %d\n", (int)r->volume_matrix.rows(),
+                  (int)r->volume_matrix.cols(), r->volume_matrix(0, 0), r->volume_matrix(0, 1),
+                  r->volume_matrix(1, 0), r->volume_matrix(1, 1), (int)r->grad_matrix.size());
+    return 0;
+  }
   auto res = compute_intersection_volume_matrix_cuda(geoms, kg);
   if (res.num_objects() != 2) return 1;
   std::printf("V %.15f %.15f %.15f %.15f\n", res.volume_matrix(0, 0), res.volume_matrix(0, 1),

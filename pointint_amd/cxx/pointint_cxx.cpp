@@ -367,7 +367,10 @@ IntersectionVolumeMatrixResult volume_matrix_resident(
   if (enable_profiling) pii_context_set_profiling(ctx, 1);
   bool all_affine = true;
   for (int i = 0; i < n; ++i) all_affine = all_affine && kinds[i] == PII_DOF_AFFINE;
-  const bool one_pass = all_affine && (needs_gradient(flags) || needs_hessian(flags));
+  // one Gram over the A and 12 n derivative columns while they fit its
+  // column limit (16,384); beyond it, one VJP per pair as for other DoFs
+  const bool one_pass = all_affine && (needs_gradient(flags) || needs_hessian(flags)) &&
+                        13LL * n <= 16384;
   auto print_profile = [&] {
     if (!enable_profiling) return;
     double ms[4];

@@ -98,3 +98,19 @@ def test_cxx_example_cube_pair(oracle, mode, tol):
     g64 = np.asarray(oracle.compute_A_gradient(g1, kk, th, oracle.DOF_AFFINE)).ravel()
     assert abs(v["ak"] - a64) <= 1e-12 * abs(a64)
     assert np.max(np.abs(v["gradk"] - g64)) <= 1e-11 * np.max(np.abs(g64))
+
+
+def test_matrix_driver_size_mismatch_returns_zero():
+    """CPU: fewer DoF parameterisations or DoF vectors than geometries: the
+    driver reports the mismatch and returns the n x n zero matrix without a
+    device call (cpp/src/compute_intersection_volume_multi_object.cu:255-261)."""
+    r = subprocess.run([os.path.join(PKG, "example_cube_pair"), LEB29, "mismatch"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stderr.count("Mismatch between number of geometries (2)") == 2
+    rows = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("mismatch")]
+    assert len(rows) == 2
+    for row in rows:
+        assert row[1:3] == ["2", "2"]
+        assert [float(x) for x in row[3:7]] == [0.0] * 4
+        assert row[7] == "0"   # no gradients

@@ -70,3 +70,26 @@ def test_python_form_factor_into_tensor():
     assert np.array_equal(got[0::2] + 1j * got[1::2], ref)
     with pytest.raises(ValueError):
         P.compute_form_factor_into(torch.empty(2 * kg.Q, dtype=torch.float32, device="cuda"), g, kg)
+
+
+def test_device_output_is_ordered_after_the_callers_stream():
+    """A fill of the output queued on torch's current stream behind long work
+    must land before the library's write (the writes wait for the caller's
+    stream), and the result must be complete when the call returns."""
+    import torch
+    V, F = W.icosphere(2)
+    g = P.make_triangle_mesh(V * 0.4, F)
+    kg = P.kgrid_from_rule(9, 8)
+    ref = P.compute_form_factor(g, kg, precision="fast")
+    s = torch.cuda.Stream()
+    dev = torch.zeros(2 * kg.Q, dtype=torch.float64, device="cuda")
+    with torch.cuda.stream(s):
+        a = torch.randn(2048, 2048, device="cuda")
+        for _ in range(8):
+            a = a @ a / 2048.0          # keeps the stream busy before the fill
+        dev.fill_(float("nan"))
+        P.compute_form_factor_into(dev, g, kg, precision="fast")
+        out = dev.clone()
+    s.synchronize()
+    got = out.cpu().numpy()
+    assert np.array_equal(got[0::2] + 1j * got[1::2], ref)
