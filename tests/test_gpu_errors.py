@@ -105,13 +105,13 @@ def test_affine_derivatives_column_limit():
 
 
 def test_caller_grid_with_near_equal_coordinates(oracle):
-    """Directions perturbed by up to 3e-13 (ties within the antipode tolerance):
+    """Directions perturbed by up to 1e-13 (ties within the antipode tolerance):
     the grid is still recognised as centrally symmetric, and the fast form
     factors on it match the oracle's."""
     d, w = oracle.load_lebedev(P.lebedev_path(9))
     dirs, kmag, wq = oracle.build_kgrid(d, w, 6)
     rng = np.random.default_rng(3)
-    dirs = dirs + rng.uniform(-3e-13, 3e-13, size=dirs.shape)
+    dirs = dirs + rng.uniform(-1e-13, 1e-13, size=dirs.shape)
     kg = P.KGrid(dirs, kmag, wq)
     assert kg.symmetric and 2 * kg.Qh == kg.Q
     Vi, Fi = W.icosphere(2)
