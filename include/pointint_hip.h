@@ -176,6 +176,20 @@ int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const
  * pii_context_set_caller_stream names another), and are complete when the
  * call returns, so work queued on any stream afterwards sees them. */
 int pii_context_set_caller_stream(pii_context* ctx, void* hip_stream);
+
+/* pii_form_factor_device without allocations.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, result and ordering as pii_form_factor_device; every
+ * intermediate buffer comes from a fixed 64 MiB device array per GPU (held
+ * until the process ends) and a fixed 1 MiB host staging array per thread,
+ * and no hipMalloc / hipFree / hipHostMalloc is called.  A call that needs
+ * more (records of about 160 bytes per triangle, the element-split partial
+ * sums, 32 bytes per k-node) fails with PII_ERR_INVALID ("... exhausted").
+ * Calls on one GPU share its array and are serialised by a lock. */
+int pii_form_factor_device_noalloc(pii_context* ctx, const pii_geometry* g, int dof_kind,
+                                   const double* dofs, int ndofs, const pii_kgrid* kg,
+                                   int precision, double* d_out);
 int pii_form_factor_device(pii_context* ctx, const pii_geometry* g, int dof_kind,
                            const double* dofs, int ndofs, const pii_kgrid* kg, int precision,
                            double* d_out);

@@ -66,6 +66,7 @@ SIGNATURES = {
     "pii_geometry_info": ([_P, _P, _P, _P], _I),
     "pii_form_factor": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),
     "pii_form_factor_device": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),
+    "pii_form_factor_device_noalloc": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),
     "pii_form_factor_jacobian": ([_P, _P, _I, _P, _I, _P, _P], _I),
     "pii_form_factor_jacobian_device": ([_P, _P, _I, _P, _I, _P, _P], _I),
     "pii_volume_matrix": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _P], _I),

@@ -20,7 +20,21 @@
 
 namespace pii {
 
+void* Arena::take(size_t n) {
+  const size_t at = align_up(used, 256);
+  if (at > cap || n > cap - at)
+    throw Error(PII_ERR_INVALID, std::string(what) + " exhausted: the call needs more than its " +
+                                     std::to_string(cap) + " bytes");
+  used = at + n;
+  return base + at;
+}
+
 void DevBuf::release() {
+  if (arena) {       // carved from a region: nothing to free
+    p = nullptr;
+    bytes = 0;
+    return;
+  }
   if (p) {
     (void)hipFree(p);
     p = nullptr;
@@ -44,6 +58,11 @@ void DevBuf::ensure(size_t n) {
   if (n <= bytes && p) return;
   release();
   if (n == 0) return;
+  if (arena) {
+    p = arena->take(n);
+    bytes = n;
+    return;
+  }
   const size_t want = std::max<size_t>(n, 256);
   if (injected_allocation_failure())
     throw Error(PII_ERR_NOMEM, "hipMalloc of " + std::to_string(want) +
@@ -57,6 +76,11 @@ void DevBuf::ensure(size_t n) {
 }
 
 void HostBuf::release() {
+  if (arena) {
+    p = nullptr;
+    bytes = 0;
+    return;
+  }
   if (p) {
     (void)hipHostFree(p);
     p = nullptr;
@@ -68,6 +92,11 @@ void HostBuf::ensure(size_t n) {
   if (n <= bytes && p) return;
   release();
   if (n == 0) return;
+  if (arena) {
+    p = arena->take(n);
+    bytes = n;
+    return;
+  }
   const size_t want = std::max<size_t>(n, 4096);
   if (injected_allocation_failure())
     throw Error(PII_ERR_NOMEM, "hipHostMalloc of " + std::to_string(want) +
@@ -581,6 +610,11 @@ int pii_context_create(int device, pii_context** out) {
       delete ctx;
       throw Error(PII_ERR_HIP, "hipStreamCreate failed");
     }
+    if (hipEventCreateWithFlags(&ctx->c.caller_ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipStreamDestroy(ctx->c.stream);
+      delete ctx;
+      throw Error(PII_ERR_HIP, "hipEventCreate failed");
+    }
     *out = ctx;
   });
 }
@@ -913,7 +947,11 @@ int pii_geometry_info(const pii_geometry* g, int* type, int* num_elements, int* 
 
 // Caller memory for the device-output entries: device memory of the
 // context's device (the reference's plug-ins write into a device double2*).
-static void require_device_pointer(const void* p, int device, const char* who) {
+}  // extern "C"
+
+namespace pii {
+
+void require_device_pointer(const void* p, int device, const char* who) {
   hipPointerAttribute_t a{};
   const hipError_t e = hipPointerGetAttributes(&a, p);
   if (e != hipSuccess) (void)hipGetLastError();
@@ -921,28 +959,39 @@ static void require_device_pointer(const void* p, int device, const char* who) {
     throw Error(PII_ERR_INVALID, std::string(who) + ": out is not device memory of the context's device");
 }
 
-static void form_factor_to(pii_context* ctx, const pii_geometry* g, int dof_kind,
-                           const double* dofs, int ndofs, const pii_kgrid* kg, int precision,
-                           double* out, hipMemcpyKind kind) {
+}  // namespace pii
+
+extern "C" {
+
+}  // extern "C"
+
+namespace pii {
+
+void form_factor_to(Context& c, const pii_geometry* g, int dof_kind, const double* dofs,
+                    int ndofs, const pii_kgrid* kg, int precision, double* out,
+                    hipMemcpyKind kind) {
   const GeometryDev* gp = &g->g;
   Batch b;
   if (precision == PII_FF_FP64) {
-    b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
+    b.build(c, 1, &gp, &dof_kind, &dofs, &ndofs, false);
     const int Q = kg->k.Q;
     const int ldJ = (int)align_up((size_t)Q, 64);
-    ctx->c.w.J.ensure(sizeof(double2) * (size_t)ldJ);
-    launch_ff_fp64(ctx->c.stream, b.d_objs, 1, kg->k.full.as<double>(), Q,
-                   ctx->c.w.J.as<double2>(), ldJ);
+    c.w.J.ensure(sizeof(double2) * (size_t)ldJ);
+    launch_ff_fp64(c.stream, b.d_objs, 1, kg->k.full.as<double>(), Q, c.w.J.as<double2>(), ldJ);
     PII_HIP(hipGetLastError());
-    PII_HIP(hipMemcpyAsync(out, ctx->c.w.J.p, sizeof(double2) * Q, kind, ctx->c.stream));
-    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+    PII_HIP(hipMemcpyAsync(out, c.w.J.p, sizeof(double2) * Q, kind, c.stream));
+    PII_HIP(hipStreamSynchronize(c.stream));
   } else if (precision == PII_FF_FAST) {
-    b.build(ctx->c, 1, &gp, &dof_kind, &dofs, &ndofs, true, ctx->c.series_grid(kg->k));
-    form_factor_fast_full(ctx->c, b, kg->k, out, kind);
+    b.build(c, 1, &gp, &dof_kind, &dofs, &ndofs, true, c.series_grid(kg->k));
+    form_factor_fast_full(c, b, kg->k, out, kind);
   } else {
     throw Error(PII_ERR_INVALID, "unknown form-factor precision");
   }
 }
+
+}  // namespace pii
+
+extern "C" {
 
 int pii_debug_fail_allocation(int after) {
   return guarded([&] {
@@ -957,7 +1006,7 @@ int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const
     require(ctx && g && kg && dofs && out, "pii_form_factor: NULL argument");
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     PII_HIP(hipSetDevice(ctx->c.device));
-    form_factor_to(ctx, g, dof_kind, dofs, ndofs, kg, precision, out, hipMemcpyDeviceToHost);
+    form_factor_to(ctx->c, g, dof_kind, dofs, ndofs, kg, precision, out, hipMemcpyDeviceToHost);
   }, [&] { if (kg) zero_fill(out, 2LL * kg->k.Q); });
 }
 
@@ -965,12 +1014,18 @@ int pii_form_factor(pii_context* ctx, const pii_geometry* g, int dof_kind, const
 // (the null stream unless pii_context_set_caller_stream named another), so
 // that a fill or read of the caller's buffer queued before the call is
 // ordered before the library's writes into it.
-static void order_after_caller(pii_context* ctx) {
-  if (!ctx->c.caller_ev)
-    PII_HIP(hipEventCreateWithFlags(&ctx->c.caller_ev, hipEventDisableTiming));
+}  // extern "C"
+
+namespace pii {
+
+void order_after_caller(pii_context* ctx) {
   PII_HIP(hipEventRecord(ctx->c.caller_ev, ctx->c.caller_stream));
   PII_HIP(hipStreamWaitEvent(ctx->c.stream, ctx->c.caller_ev, 0));
 }
+
+}  // namespace pii
+
+extern "C" {
 
 int pii_context_set_caller_stream(pii_context* ctx, void* stream) {
   return guarded([&] {
@@ -989,7 +1044,7 @@ int pii_form_factor_device(pii_context* ctx, const pii_geometry* g, int dof_kind
     PII_HIP(hipSetDevice(ctx->c.device));
     require_device_pointer(d_out, ctx->c.device, "pii_form_factor_device");
     order_after_caller(ctx);
-    form_factor_to(ctx, g, dof_kind, dofs, ndofs, kg, precision, d_out,
+    form_factor_to(ctx->c, g, dof_kind, dofs, ndofs, kg, precision, d_out,
                    hipMemcpyDeviceToDevice);
   });
 }

@@ -90,6 +90,16 @@ struct Batch {
   }
 };
 
+// Device-output entries: the caller's pointer must be device memory of the
+// context's device; the context stream waits for the caller's stream.
+void require_device_pointer(const void* p, int device, const char* who);
+void order_after_caller(pii_context* ctx);
+// A(k_q) of one object over the full grid of kg into out (host or device
+// memory by kind), on c's stream and workspace; synchronous.
+void form_factor_to(Context& c, const pii_geometry* g, int dof_kind, const double* dofs,
+                    int ndofs, const pii_kgrid* kg, int precision, double* out,
+                    hipMemcpyKind kind);
+
 // Fast form factors of a batch: the moment series for the leading shells of
 // triangle objects, the per-face kernel for the remaining nodes.  prof (may be
 // null) records the two kernels' phases and the work counters.

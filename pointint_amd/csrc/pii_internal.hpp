@@ -42,17 +42,37 @@ struct Error {
 };
 
 // ---------------------------------------------------------------------------
-// Device buffer owned by the library (hipMalloc / hipFree).
+// A fixed region (device or host memory the library does not allocate) that
+// buffers bound to it carve their space from, in order, without freeing:
+// the workspace of the allocation-free entry point.
+// ---------------------------------------------------------------------------
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  const char* what = "scratch";
+  // 256-byte aligned space of n bytes; throws PII_ERR_INVALID when the
+  // region is exhausted.
+  void* take(size_t n);
+};
+
+// ---------------------------------------------------------------------------
+// Device buffer owned by the library (hipMalloc / hipFree), or carved from an
+// Arena when bound to one (never allocated or freed then).
 // ---------------------------------------------------------------------------
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  Arena* arena = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), arena(o.arena) {
+    o.p = nullptr; o.bytes = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release(); p = o.p; bytes = o.bytes; arena = o.arena; o.p = nullptr; o.bytes = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
@@ -140,6 +160,7 @@ struct GeometryDev {
 struct HostBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  Arena* arena = nullptr;   // carve from a host region instead of hipHostMalloc
   HostBuf() = default;
   HostBuf(const HostBuf&) = delete;
   HostBuf& operator=(const HostBuf&) = delete;
@@ -182,6 +203,15 @@ struct Workspace {
   HostBuf gplan_h;           // pinned: the Gram plan of a call
   HostBuf stage;             // pinned: descriptors, prefix sums and DoFs of a call
   HostBuf hout;              // pinned: per-vertex gradients coming back
+  // Every buffer takes its space from dev (device) / host instead of
+  // allocating (the allocation-free entry point).
+  void bind(Arena* dev, Arena* host) {
+    for (DevBuf* b : {&world, &faces, &J, &X, &part, &V, &misc, &colmax, &B, &C, &gw, &mrec,
+                      &gam, &dth, &Xt, &cmax2, &H, &dsrec, &dspart, &dsval, &split, &planes,
+                      &gplan})
+      b->arena = dev;
+    for (HostBuf* b : {&gplan_h, &stage, &hout}) b->arena = host;
+  }
 };
 
 struct Context {
