@@ -2,18 +2,25 @@
 (demo/bindings.cpp:16-103), served by the MI355X library.
 
 ``import pointint_amd.pointint_cpp as pi`` gives the same names, argument
-names and return shapes as the reference module for the device path:
+names and return shapes as the reference module:
 
   Geometry, KGrid, make_triangle_mesh, make_point_cloud, create_unit_cube_mesh,
   create_sphere_pointcloud, load_lebedev, build_kgrid,
-  compute_intersection_volume_cuda, compute_volume_cpu,
-  box_box_intersection_volume, sphere_sphere_intersection_volume,
-  translate_points
+  compute_intersection_volume_cuda, compute_intersection_volume_cpu,
+  compute_volume_cpu, box_box_intersection_volume,
+  sphere_sphere_intersection_volume, translate_points
 
-plus the matrix path the reference sketches for Python
-(docs/DESIGN_PROPOSAL.md:767-803): compute_intersection_volume_matrix_cuda,
-make_gaussian_splat.  ``compute_intersection_volume_cpu`` is the reference's
-TBB path; this package ships no CPU implementation of the path, so it raises.
+so that demo/exp.py's call sequence runs unchanged, plus the API the
+reference sketches for Python (docs/DESIGN_PROPOSAL.md:767-793): the matrix
+path compute_intersection_volume_matrix_cuda, the DoF parameterisations
+AffineDoF / TriangleMeshDoF and compute_intersection_volume_gradient_cuda,
+and make_gaussian_splat.
+
+There is no CPU implementation of the path in this package:
+``compute_intersection_volume_cpu`` (the reference's FP64 TBB path) returns
+the same FP64 value from the plain FP64 path on the device, and
+``compute_intersection_volume_cuda`` the fast path (FP32 transcendentals,
+FP64 phases and moment series, FP64 Gram of the two columns).
 """
 from __future__ import annotations
 
@@ -61,13 +68,72 @@ def compute_intersection_volume_cuda(geom1, geom2, kgrid, block_size=256):
 
 def compute_intersection_volume_matrix_cuda(geoms, kgrid, block_size=256, dofs=None,
                                             dof_kinds=None, gram="fp16x3"):
+    """The n x n volume matrix (docs/DESIGN_PROPOSAL.md:782-784); dofs /
+    dof_kinds optional (identity AffineDoF)."""
     return core.compute_volume_matrix(list(geoms), kgrid, dofs, dof_kinds, ff="fast", gram=gram)
 
 
 def compute_intersection_volume_cpu(geom1, geom2, kgrid):
-    raise NotImplementedError(
-        "compute_intersection_volume_cpu is the reference's CPU/TBB path; pointint_amd serves "
-        "the device path only (use compute_intersection_volume_cuda)")
+    """Pairwise V of two geometries (identity DoFs) in FP64, the value of the
+    reference's CPU path (cpp/src/compute_intersection_volume_cpu.cpp:21-208):
+    computed by the plain FP64 path on the device (FP64 form factors over the
+    full grid, FP64 Gram)."""
+    V = core.compute_volume_matrix([geom1, geom2], kgrid, ff="fp64", gram="fp64")
+    return float(V[0, 1])
+
+
+class AffineDoF:
+    """12 DoFs [t (3), A row-major (9)]: x' = A x + t
+    (cpp/include/dof/affine_dof.hpp:15-22)."""
+    kind = "affine"
+
+    def num_dofs(self, geom=None) -> int:
+        return 12
+
+    @staticmethod
+    def identity() -> np.ndarray:
+        return np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], dtype=np.float64)
+
+
+class TriangleMeshDoF:
+    """3 DoFs per vertex: the vertex positions themselves
+    (cpp/include/dof/triangle_mesh_dof.hpp:12-17)."""
+    kind = "trimesh"
+
+    def __init__(self, num_vertices: int = 0):
+        self.num_vertices = num_vertices
+
+    def num_dofs(self, geom=None) -> int:
+        return 3 * (geom.num_dof_vertices if geom is not None else self.num_vertices)
+
+
+class IntersectionVolumeResult:
+    """volume and dV/dtheta of each geometry
+    (cpp/include/compute_intersection_volume.hpp:18-34)."""
+
+    def __init__(self, volume, grad_geom1, grad_geom2):
+        self.volume = volume
+        self.grad_geom1 = grad_geom1
+        self.grad_geom2 = grad_geom2
+
+    def __repr__(self):
+        return (f"IntersectionVolumeResult(volume={self.volume}, "
+                f"grad_geom1={self.grad_geom1.shape}, grad_geom2={self.grad_geom2.shape})")
+
+
+def compute_intersection_volume_gradient_cuda(geom1, geom2, dof1, dof2, dofs1, dofs2, kgrid,
+                                              block_size=256, gram="fp64"):
+    """V_12 of the two geometries under their DoFs and its gradient with
+    respect to each one's DoFs (docs/DESIGN_PROPOSAL.md:786-793; the GRADIENT
+    flag of compute_intersection_volume_cuda, cpp/src/compute_intersection_volume.cu:60-101).
+    One pass: form factors and their DoF derivatives, one Gram."""
+    kinds = [dof1.kind, dof2.kind]
+    dofs = [np.asarray(dofs1, dtype=np.float64).ravel(),
+            np.asarray(dofs2, dtype=np.float64).ravel()]
+    V, G = core.volume_matrix_derivatives([geom1, geom2], kgrid, dofs, gram=gram, hessian=False,
+                                          dof_kinds=kinds)
+    # G[i] is (2, D_i): row j = dV_ij / dtheta_i
+    return IntersectionVolumeResult(float(V[0, 1]), np.array(G[0][1]), np.array(G[1][0]))
 
 
 def compute_volume_cpu(geom):

@@ -399,3 +399,23 @@ def test_vjp_equals_fd_of_pair_volume(oracle):
         xp[d] += h; xm[d] -= h
         fd = (oracle.compute_volume(g0, xp, oracle.DOF_MESH) - oracle.compute_volume(g0, xm, oracle.DOF_MESH)) / (2 * h)
         assert abs(gv[d] - fd) < 1e-8
+
+
+def test_reference_recorded_two_sphere_volumes(oracle):
+    """The reference's own outputs (demo/exp_results.csv, copied to
+    tests/golden/exp_two_spheres.npz by make_exp_vectors.py): V_01 of two
+    1000-disk Fibonacci spheres at distances d = 0 .. 2 on Lebedev 302 x L=32,
+    from its C++ CPU path and its C++ CUDA path.  The oracle reproduces the CPU
+    values to summation-order rounding, and the CUDA values within the
+    reference GPU's own J1 approximation (1.3e-7 absolute in J1)."""
+    z = np.load(os.path.join(GOLD, "exp_two_spheres.npz"))
+    dirs, kmag, w = grid(oracle, 29, 32)
+    P, N, R = oracle.sphere_pointcloud(1000)
+    g1 = oracle.geom_disks(P, N, R)
+    ds = np.arange(0, 2.1, 0.1)            # the values demo/exp.py uses
+    assert np.allclose(ds, z["distance"], atol=1e-12)
+    for i in range(0, 21, 4):
+        g2 = oracle.geom_disks(P + np.array([ds[i], 0.0, 0.0]), N, R)
+        V, _ = oracle.volume_matrix([g1, g2], dirs, kmag, w)
+        assert abs(V[0, 1] - z["v_cpu"][i]) < 1e-13, (ds[i], V[0, 1], z["v_cpu"][i])
+        assert abs(V[0, 1] - z["v_cuda"][i]) < 3e-8
