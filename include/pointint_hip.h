@@ -156,6 +156,19 @@ int pii_geometry_create_gaussians(pii_context* ctx, const double* el, int n,
                                   pii_geometry** out);
 int pii_geometry_destroy(pii_geometry* g);
 int pii_geometry_info(const pii_geometry* g, int* type, int* num_elements, int* num_dof_vertices);
+/* Geometry preprocessing for rigid poses (no counterpart in the reference,
+ * whose series-free path re-reads every face at every node): expands the
+ * low-|k| moment series of each triangle mesh of geoms[0..n) with at least
+ * 192 faces in spherical harmonics (degree <= 57, 17,544 coefficient pairs,
+ * 274 KiB of device memory per mesh), in the mesh's own frame.  Afterwards
+ * the fast form factor, volume matrix and serving entries evaluate that
+ * series from the harmonics (a fixed cost per direction instead of one pass
+ * over the faces) whenever the mesh's affine DoFs are a similarity
+ * x' = s R x + t (R a rotation); other DoFs, the FP64 and derivative entries
+ * are unaffected.  Other geometries are left as they are.  Synchronous;
+ * PII_ERR_BUSY while a serving query uses one of the geometries; not to be
+ * called concurrently with other calls that use them. */
+int pii_geometry_cache_moments(pii_context* ctx, int n, pii_geometry* const* geoms);
 
 /* ---- form factors ---------------------------------------------------------- */
 /* A(k_q) for every node of kg, written to out[2*Q] (interleaved re, im, host

@@ -12,6 +12,7 @@ from .core import (  # noqa: F401
     KGrid,
     build_kgrid,
     build_kgrid_arrays,
+    cache_moments,
     compute_form_factor,
     compute_form_factor_into,
     compute_form_factor_jacobian,

@@ -64,6 +64,7 @@ SIGNATURES = {
     "pii_geometry_create_gaussians": ([_P, _P, _I, _P], _I),
     "pii_geometry_destroy": ([_P], _I),
     "pii_geometry_info": ([_P, _P, _P, _P], _I),
+    "pii_geometry_cache_moments": ([_P, _I, _P], _I),
     "pii_form_factor": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),
     "pii_form_factor_device": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),
     "pii_form_factor_device_noalloc": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),

@@ -206,6 +206,20 @@ class Geometry:
             self._h = None
 
 
+def cache_moments(geoms: Sequence[Geometry], ctx: Optional[Context] = None) -> None:
+    """Expands the low-|k| moment series of each triangle mesh (>= 192 faces)
+    in spherical harmonics, once: afterwards the fast forward entries evaluate
+    that series at a fixed cost per direction for any pose x' = s R x + t
+    (include/pointint_hip.h, pii_geometry_cache_moments).  Other geometries
+    are left as they are."""
+    geoms = list(geoms)
+    if not geoms:
+        return
+    ctx = ctx or geoms[0].ctx
+    arr = (C.c_void_p * len(geoms))(*[g._hv for g in geoms])
+    N.check(N.load().pii_geometry_cache_moments(ctx.handle, len(geoms), arr))
+
+
 def make_triangle_mesh(V, F, ctx: Optional[Context] = None) -> Geometry:
     ctx = ctx or Context.default()
     V = N.f64(V).reshape(-1, 3)

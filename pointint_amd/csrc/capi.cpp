@@ -280,6 +280,26 @@ static void world_bounds(const GeometryDev& g, int kind, const double* dofs, dou
   R = std::sqrt(r2) * (1.0 + 1e-12);
 }
 
+// A = s R with R a rotation (rows of A orthogonal, of equal length s, to
+// 1e-12 relative; det A > 0): the transforms under which a geometry's
+// cached harmonic coefficients give its moment series.
+static bool similarity_scale(const double* A, double& s) {
+  double B[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      B[i][j] = A[3 * i] * A[3 * j] + A[3 * i + 1] * A[3 * j + 1] + A[3 * i + 2] * A[3 * j + 2];
+  const double s2 = (B[0][0] + B[1][1] + B[2][2]) / 3.0;
+  if (!(s2 > 0.0) || !std::isfinite(s2)) return false;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (std::abs(B[i][j] - (i == j ? s2 : 0.0)) > 1e-12 * s2) return false;
+  const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                     A[2] * (A[3] * A[7] - A[4] * A[6]);
+  if (!(det > 0.0)) return false;
+  s = std::sqrt(s2);
+  return true;
+}
+
 // Smallest series order of kSeriesOrders whose tail bound at X = |k| R,
 // X^(P+1) / (P+1)! / (1 - X / (P+2)), is below 3.6e-10 (of the area).
 // One pass over m = 1..P+1 serves every candidate order (this runs per object
@@ -408,13 +428,21 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   std::vector<double> kser(N, 0.0);
   std::vector<std::array<double, 3>> ctr(N, {0.0, 0.0, 0.0});
   std::vector<size_t> moff(N, 0);
+  std::vector<double> shs(N, 0.0);
   size_t total_m = 0;
+  nsh = 0;
   for (int slot = 0; slot < N; ++slot) {
     const int i = order[slot];
     nser[slot] = nser_i[i];
     nord[slot] = nord_i[i];
     kser[slot] = kser_i[i];
     ctr[slot] = ctr_i[i];
+    if (use_sh && nser[slot] > 0 && geoms[i]->type == PII_GEOM_TRIANGLE && geoms[i]->shc.p &&
+        kinds[i] == PII_DOF_AFFINE && similarity_scale(dofs[i] + 3, shs[slot])) {
+      ++nsh;
+      continue;  // no moment records
+    }
+    shs[slot] = 0.0;
     if (nser[slot] > 0 && geoms[i]->type == PII_GEOM_TRIANGLE) {
       moff[slot] = total_m;
       total_m += align_up((size_t)geoms[i]->n * kRecMoment, 32);
@@ -464,8 +492,12 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
       for (int k = 0; k < 9; ++k) fro2 += A[k] * A[k];
       d.dser = std::abs(det) >= 1e-6 * std::pow(fro2 / 3.0, 1.5) ? 1 : 0;
     }
-    d.mrec = (nser[slot] && g.type == PII_GEOM_TRIANGLE) ? w.mrec.as<double>() + moff[slot]
-                                                          : nullptr;
+    if (shs[slot] > 0.0) {
+      d.shc = g.shc.as<double>();
+      d.shs = shs[slot];
+    } else if (nser[slot] && g.type == PII_GEOM_TRIANGLE) {
+      d.mrec = w.mrec.as<double>() + moff[slot];
+    }
   }
   qbeg_min = 0;
   if (N > 0) {
@@ -936,6 +968,29 @@ int pii_geometry_destroy(pii_geometry* g) {
   });
 }
 
+int pii_geometry_cache_moments(pii_context* ctx, int n, pii_geometry* const* geoms) {
+  return guarded([&] {
+    require(ctx != nullptr, "pii_geometry_cache_moments: NULL context");
+    require(n >= 0, "pii_geometry_cache_moments: negative geometry count");
+    if (n == 0) return;
+    require(geoms != nullptr, "pii_geometry_cache_moments: NULL array");
+    std::vector<GeometryDev*> gs(n);
+    for (int i = 0; i < n; ++i) {
+      require(geoms[i] != nullptr, "pii_geometry_cache_moments: NULL geometry");
+      if (geoms[i]->device != ctx->c.device)
+        throw Error(PII_ERR_INVALID,
+                    "pii_geometry_cache_moments: geometry on another device than the context");
+      if (geoms[i]->refs.load() > 0)
+        throw Error(PII_ERR_BUSY, "pii_geometry_cache_moments: the geometry is in use by " +
+                                      std::to_string(geoms[i]->refs.load()) + " serving queries");
+      gs[i] = &geoms[i]->g;
+    }
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    cache_moments(ctx->c, n, gs.data());
+  });
+}
+
 int pii_geometry_info(const pii_geometry* g, int* type, int* num_elements, int* num_dof_vertices) {
   return guarded([&] {
     require(g != nullptr, "NULL geometry");
@@ -982,6 +1037,7 @@ void form_factor_to(Context& c, const pii_geometry* g, int dof_kind, const doubl
     PII_HIP(hipMemcpyAsync(out, c.w.J.p, sizeof(double2) * Q, kind, c.stream));
     PII_HIP(hipStreamSynchronize(c.stream));
   } else if (precision == PII_FF_FAST) {
+    b.use_sh = true;
     b.build(c, 1, &gp, &dof_kind, &dofs, &ndofs, true, c.series_grid(kg->k));
     form_factor_fast_full(c, b, kg->k, out, kind);
   } else {
@@ -1105,6 +1161,7 @@ int pii_volume_matrix(pii_context* ctx, int n, const pii_geometry* const* geoms,
       b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, false);
       volume_matrix_fp64(ctx->c, b, kg->k, V);
     } else if (ff_precision == PII_FF_FAST) {
+      b.use_sh = true;
       b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
       volume_matrix_fast(ctx->c, b, kg->k, gram_mode, V);
     } else {

@@ -38,6 +38,11 @@ struct ObjDesc {
   double* world;              // nv*3 world vertices (written by the pre-pass)
   double* rec;                // element records (written by the pre-pass)
   double* mrec;               // n*12 moment records {v0-c, v1-c, v2-c, S} (nser > 0)
+  // Spherical-harmonic form of the triangle series (sh_common.hpp): the
+  // geometry's cached coefficients (null: the direct series over mrec) and
+  // the scale s of the object's similarity transform A = s R.
+  const double* shc;
+  double shs;
 };
 
 

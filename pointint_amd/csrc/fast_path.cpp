@@ -43,11 +43,13 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
                               unsigned* d_colmax, Workspace& w, Context* prof) {
   const double* krep = kg.rep.as<double>();
   const int N = b.N;
-  bool tri_series = false, surfel_series = false;
+  bool tri_series = false, surfel_series = false, sh_series = false;
   int ser_tri_faces = 0, ser_surf_elems = 0, max_patches = 0, max_surf = 0;
   for (const ObjDesc& d : b.h_objs) {
     const bool tri = d.type == PII_GEOM_TRIANGLE;
-    if (d.nser > 0) {
+    if (d.nser > 0 && d.shc) {
+      sh_series = true;
+    } else if (d.nser > 0) {
       (tri ? tri_series : surfel_series) = true;
       (tri ? ser_tri_faces : ser_surf_elems) = std::max(tri ? ser_tri_faces : ser_surf_elems, d.n);
     }
@@ -56,7 +58,7 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
     if (!prof) continue;
     prof->work[tri ? 0 : 3] += (double)d.n * (kg.Qh - d.qbeg);
     if (d.nser > 0) {
-      prof->work[tri ? 1 : 4] += (double)d.n * b.Mh;
+      if (!d.shc) prof->work[tri ? 1 : 4] += (double)d.n * b.Mh;
       prof->work[2] += (double)d.qbeg;
     }
   }
@@ -79,8 +81,11 @@ void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
                                 ff_split_doubles(N, std::max(span, 0), s_ff)});
   if (need) w.split.ensure(sizeof(double) * need);
   double* part = need ? w.split.as<double>() : nullptr;
-  if (tri_series || surfel_series) {
+  if (tri_series || surfel_series || sh_series) {
     if (prof) prof->phase_begin(4);
+    if (sh_series)
+      launch_series_sh(s, b.d_objs, N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
+                       d_colmax);
     if (tri_series)
       launch_series_tri(s, b.d_objs, N, krep, kg.Qh, b.Mh, out_mode, d_A, ldA, d_X, ldX,
                         d_colmax, s_tri, part);

@@ -30,6 +30,7 @@
 #include "device_common.hpp"
 #include "pii_internal.hpp"
 #include "series_common.hpp"
+#include "sh_common.hpp"
 
 namespace pii {
 
@@ -213,7 +214,8 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_tri(
   __shared__ double red[P + 1][kLanes];
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
-  if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE) return;  // uniform over the workgroup
+  // uniform over the workgroup; objects with cached harmonics take k_series_sh
+  if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE || d.shc) return;
 #define PII_TRI_BODY(PP) \
   series_tri_body<PP>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout, ldX, colmax, part)
   PII_SERIES_SWITCH(PII_TRI_BODY)
@@ -350,7 +352,7 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
   __shared__ double red[P + 1][kLanes];
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
-  if (d.nser <= 0 || d.type != type) return;
+  if (d.nser <= 0 || d.type != type || d.shc) return;
   if (type == PII_GEOM_TRIANGLE) {
 #define PII_FIN_BODY(PP)                                                                     \
   series_finish_body<PP, kKindTri>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, \
@@ -364,6 +366,92 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
     series_finish_body<P, kKindGauss>(d, bo, &red[0][0], krep, Qh, Mh, out_mode, Aout, ldA, Xout,
                                       ldX, colmax, part, nsplit);
   }
+}
+
+// Spherical-harmonic form of the triangle series (objects with cached
+// coefficients d.shc; sh_common.hpp): the rows of A = s R are s times
+// orthonormal, p = A^T d / s = (p_x, p_y, x) is a unit vector, and
+//   H_n(d) = s^(n+2) sum_{m, l} Q_l^m(x) (a_nlm Re z^m + b_nlm Im z^m),
+//   z = p_x + i p_y,
+// (corners scale by s, area vectors by s^2).  Degree l of order m: the
+// recurrence step and the pairs (a, b) of every n of that degree, read as
+// scalar loads at offsets fixed at compile time from the block base of m.
+__device__ constexpr sh::Recurrence kShRec{};
+__device__ constexpr sh::Layout kShLayout{};
+constexpr sh::Layout kShLay{};
+
+template <int l>
+__device__ __forceinline__ void sh_degree(double (&H)[P + 1], const PII_CONST_AS double* cm,
+                                          int m, double x, double zr, double zi, double& qa,
+                                          double& qb) {
+  if (l >= m) {
+    const double q =
+        l == m ? kShRec.seed[m] : kShRec.a[l][m] * fma(x, qa, -kShRec.b[l][m] * qb);
+    qb = qa;
+    qa = q;
+    const double u = q * zr, v = q * zi;
+    constexpr int c0 = kShLay.C[l];
+#pragma unroll
+    for (int k = 0; k < sh::count(l); ++k) {
+      const int n = sh::n0(l) + 2 * k;
+      H[n] = fma(cm[2 * (c0 + k)], u, fma(cm[2 * (c0 + k) + 1], v, H[n]));
+    }
+  }
+  if constexpr (l < sh::kL) sh_degree<l + 1>(H, cm, m, x, zr, zi, qa, qb);
+}
+
+// One workgroup per (object, 64 directions); wave w sums the orders
+// m = w, w + 4, ..; then the usual reduction and evaluation at order P.
+__global__ void __launch_bounds__(kLanes * kWaves) k_series_sh(
+    const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
+    int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
+    unsigned* __restrict__ colmax) {
+  __shared__ double red[P + 1][kLanes];
+  const BlockObj bo = xcd_block_map();
+  const ObjDesc& d = objs[bo.obj];
+  if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE || !d.shc) return;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j = bo.blk * kLanes + lane;
+  const int jc = j < Mh ? j : Mh - 1;
+  const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
+               dz = krep[5 * (size_t)Qh + jc];
+  // p = A^T d / s (A row-major in dofs[3:12])
+  const auto* a = as_const(d.dofs);
+  const double sc = d.shs, inv = 1.0 / sc;
+  const double px = (a[3] * dx + a[6] * dy + a[9] * dz) * inv;
+  const double py = (a[4] * dx + a[7] * dy + a[10] * dz) * inv;
+  const double x = (a[5] * dx + a[8] * dy + a[11] * dz) * inv;
+  double H[P + 1];
+#pragma unroll
+  for (int n = 0; n <= P; ++n) H[n] = 0.0;
+  // z^wave, and z^4 for the stride of m
+  double zr = 1.0, zi = 0.0;
+  for (int t = 0; t < wave; ++t) {
+    const double r = zr * px - zi * py;
+    zi = zr * py + zi * px;
+    zr = r;
+  }
+  const double z2r = px * px - py * py, z2i = 2.0 * px * py;
+  const double z4r = z2r * z2r - z2i * z2i, z4i = 2.0 * z2r * z2i;
+  const auto* cf = as_const(d.shc);
+  for (int m = wave; m <= sh::kL; m += kWaves) {
+    // pair (m, l, n) at mstart[m] + C[l] - C[m] + (n - n0(l)) / 2
+    const auto* cm = cf + 2 * (kShLayout.mstart[m] - kShLayout.C[m]);
+    double qa = 0.0, qb = 0.0;
+    sh_degree<0>(H, cm, m, x, zr, zi, qa, qb);
+    const double r = zr * z4r - zi * z4i;
+    zi = zr * z4i + zi * z4r;
+    zr = r;
+  }
+  double f = sc * sc;
+#pragma unroll
+  for (int n = 0; n <= P; ++n) {
+    H[n] *= f;
+    f *= sc;
+  }
+  reduce_and_evaluate<P, kKindTri>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh,
+                                   out_mode, Aout, ldA, Xout, ldX, colmax);
 }
 
 }  // namespace
@@ -402,6 +490,21 @@ void launch_series_surfel(hipStream_t s, const ObjDesc* d_objs, int N, const dou
     hipLaunchKernelGGL(k_series_finish, grid2, dim3(kLanes * kWaves), 0, s, d_objs,
                        (int)PII_GEOM_GAUSSIAN, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX,
                        d_colmax, part, nsplit);
+}
+
+void launch_series_sh(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
+                      int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                      unsigned* d_colmax) {
+  if (N <= 0 || Mh <= 0) return;
+  hipLaunchKernelGGL(k_series_sh, dim3((Mh + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves), 0,
+                     s, d_objs, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+}
+
+void launch_series_tri_raw(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_dirs,
+                           int Qd, int Md, double* d_H) {
+  if (N <= 0 || Md <= 0) return;
+  hipLaunchKernelGGL(k_series_tri, dim3((Md + kLanes - 1) / kLanes, N, 1), dim3(kLanes * kWaves),
+                     0, s, d_objs, d_dirs, Qd, Md, 0, nullptr, 0, nullptr, 0, nullptr, d_H);
 }
 
 void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,

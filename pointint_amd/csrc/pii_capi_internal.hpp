@@ -72,6 +72,11 @@ struct Batch {
   // series, directions per shell, and the smallest first node of the
   // per-face kernel over all objects.
   int nseries = 0, Mh = 0, qbeg_min = 0;
+  // Set before build: triangle objects of geometries with cached
+  // spherical-harmonic coefficients, under similarity transforms, take the
+  // harmonic form of the series (nsh of them).
+  bool use_sh = false;
+  int nsh = 0;
   // prof (may be null) records the pre-pass phase with HIP events.  series_kg
   // (fast path only, may be null) enables the moment series on that grid.
   // cols (may be null: column i) gives object i's output column.  n_cap > n

@@ -151,6 +151,9 @@ struct GeometryDev {
   // ---- disks / Gaussians: n x 8 FP64 (c.xyz, n.xyz, rho|sigma, area|w) ----
   std::vector<double> h_el;
   DevBuf el;
+  // ---- spherical-harmonic coefficients of the moment series (triangles,
+  // reference frame, about ref_c; empty until cached) ----
+  DevBuf shc;
 };
 
 // Pinned host staging (hipHostMalloc), grow-only: the upload of a call's DoFs
@@ -307,6 +310,23 @@ size_t ff_split_doubles(int N, int span, int nsplit);
 void launch_series_tri(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
                        int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                        unsigned* d_colmax, int nsplit = 1, double* d_part = nullptr);
+// The series of triangle objects with cached spherical-harmonic coefficients
+// (ObjDesc::shc), which launch_series_tri skips.
+void launch_series_sh(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep, int Qh,
+                      int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
+                      unsigned* d_colmax);
+// Raw sums H_n (n <= kSeriesOrder; descriptors of order kSeriesOrder) of the
+// triangle objects at Md directions (rows 3..5 of d_dirs, row stride Qd):
+// d_H[((obj * nblk + blk) * (kSeriesOrder + 1) + n) * 64 + lane] for direction
+// blk * 64 + lane, nblk = ceil(Md / 64).
+void launch_series_tri_raw(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_dirs,
+                           int Qd, int Md, double* d_H);
+// Spherical-harmonic coefficients of the moment series of triangle
+// geometries (reference frame, about ref_c) into each one's shc buffer; the
+// geometries with fewer than kShMinFaces faces are left without (the direct
+// series is cheaper for them).  Synchronous on c's stream.
+constexpr int kShMinFaces = 192;
+void cache_moments(Context& c, int n, GeometryDev* const* gs);
 // The same for disk and Gaussian objects.
 void launch_series_surfel(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_krep,
                           int Qh, int Mh, int out_mode, double2* d_A, int ldA, float2* d_X,

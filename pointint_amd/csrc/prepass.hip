@@ -91,8 +91,9 @@ __device__ __forceinline__ void element_record(const ObjDesc* __restrict__ objs,
       int code = 0;
       if (diam > 0.0) code = min(127, max(-64, (int)ceil(log2(diam * (1.0 + 1e-12))))) + 64;
       reinterpret_cast<uint32_t*>(rf)[3] = (d.patch_floc[e] & 0xffffffu) | ((uint32_t)code << 24);
-      if (d.nser > 0) {
+      if (d.nser > 0 && d.mrec) {
         // corners relative to the series centre, and the area vector
+        // (objects with cached harmonics have none)
         double* m = d.mrec + (size_t)kRecMoment * e;
         const double c0 = d.ctr[0], c1 = d.ctr[1], c2 = d.ctr[2];
         m[0] = ax - c0; m[1] = ay - c1; m[2] = az - c2;

@@ -184,9 +184,12 @@ struct Server {
     launch_clear_u32(st, colmax, ncol);
     launch_prepass_fixed(st, b.d_objs, b.d_vprefix, b.d_eprefix, n_cap, 1, kFixedGrid);
     launch_stamp(st, stamps, 1);
-    if (b.Mh > 0)
+    if (b.Mh > 0) {
+      launch_series_sh(st, b.d_objs, n_cap, kg->rep.as<double>(), kg->Qh, b.Mh, 1, nullptr, 0, X,
+                       ld, colmax);
       launch_series_tri(st, b.d_objs, n_cap, kg->rep.as<double>(), kg->Qh, b.Mh, 1, nullptr, 0,
                         X, ld, colmax, 1, nullptr);
+    }
     launch_stamp(st, stamps, 2);
     launch_ff_fast(st, b.d_objs, n_cap, kg->rep.as<double>(), kg->Qh, 0, 1, nullptr, 0, X, ld,
                    colmax, 1, nullptr);
@@ -214,6 +217,7 @@ struct Server {
                    std::vector<int>& c0s, int ncol) {
     const int N = (int)gs.size(), Qh = kg->Qh, ld = ldX();
     s.batch = Batch();
+    s.batch.use_sh = true;
     s.batch.build(s.w, s.stream, nullptr, N, gs.data(), kinds.data(), dp.data(), nd.data(), true,
                   series ? kg : nullptr, cols.data());
     s.w.X.ensure(sizeof(float2) * (size_t)ld * ncol);
@@ -273,6 +277,7 @@ struct Server {
       const int n_cap = (int)bucket(N, 16);
       const int nc = ncol_cap(n_cap);
       s.batch = Batch();
+      s.batch.use_sh = true;
       s.batch.build(s.w, s.stream, nullptr, N, gs.data(), kinds.data(), dp.data(), nd.data(),
                     true, series ? kg : nullptr, cols.data(), n_cap, false);
       std::vector<GramQuery> qs;
