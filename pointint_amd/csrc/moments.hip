@@ -373,85 +373,170 @@ __global__ void __launch_bounds__(kLanes * kWaves) k_series_finish(
 // orthonormal, p = A^T d / s = (p_x, p_y, x) is a unit vector, and
 //   H_n(d) = s^(n+2) sum_{m, l} Q_l^m(x) (a_nlm Re z^m + b_nlm Im z^m),
 //   z = p_x + i p_y,
-// (corners scale by s, area vectors by s^2).  Degree l of order m: the
-// recurrence step and the pairs (a, b) of every n of that degree, read as
-// scalar loads at offsets fixed at compile time from the block base of m.
+// (corners scale by s, area vectors by s^2).
+//
+// The coefficients are staged through LDS and the orders n split by parity:
+// a workgroup takes 128 directions; waves 0 and 1 sum the even n (harmonics
+// of odd degree l), waves 2 and 3 the odd n (even l), each for its own 64
+// directions over every order m.  The pairs of order m + 1 are loaded from
+// global memory while the waves sum order m from LDS (double-buffered, one
+// barrier per order).  The even sums give E(kappa) = sum_t c_2t H_2t kappa^2t
+// and the odd ones O(kappa) = sum_t c_2t+1 H_2t+1 kappa^2t+1 at every series
+// shell; the two halves meet in LDS, eight shells at a time, for the phase
+// and the output.
 __device__ constexpr sh::Recurrence kShRec{};
 __device__ constexpr sh::Layout kShLayout{};
 constexpr sh::Layout kShLay{};
+constexpr int kShPairs0 = kShLay.C[sh::kL + 1];   // pairs of the block m = 0
+constexpr int kShThreads = kLanes * kWaves;
+constexpr int kShDirs = kShThreads / 2;           // directions per workgroup
+constexpr int kShStage = (kShPairs0 + kShThreads - 1) / kShThreads;
+constexpr int kShH = P / 2 + 1;                   // sums per parity (29 even, 28 odd)
+constexpr int kShChunk = 8;                       // shells per evaluation round
 
+// Degree l of order m for the sums of parity par (n = 2 i + par in H[i]; the
+// degrees of the other parity only advance the recurrence).
 template <int l>
-__device__ __forceinline__ void sh_degree(double (&H)[P + 1], const PII_CONST_AS double* cm,
-                                          int m, double x, double zr, double zi, double& qa,
-                                          double& qb) {
+__device__ __forceinline__ void sh_degree_par(double (&H)[kShH], const double2* cb, int cm,
+                                              int m, int par, double x, double zr, double zi,
+                                              double& qa, double& qb) {
   if (l >= m) {
     const double q =
         l == m ? kShRec.seed[m] : kShRec.a[l][m] * fma(x, qa, -kShRec.b[l][m] * qb);
     qb = qa;
     qa = q;
-    const double u = q * zr, v = q * zi;
-    constexpr int c0 = kShLay.C[l];
+    if (((l + 1) & 1) == par) {
+      const double u = q * zr, v = q * zi;
+      constexpr int c0 = kShLay.C[l];
 #pragma unroll
-    for (int k = 0; k < sh::count(l); ++k) {
-      const int n = sh::n0(l) + 2 * k;
-      H[n] = fma(cm[2 * (c0 + k)], u, fma(cm[2 * (c0 + k) + 1], v, H[n]));
+      for (int k = 0; k < sh::count(l); ++k) {
+        const int n = sh::n0(l) + 2 * k;
+        const double2 c = cb[c0 + k - cm];
+        H[n >> 1] = fma(c.x, u, fma(c.y, v, H[n >> 1]));
+      }
     }
   }
-  if constexpr (l < sh::kL) sh_degree<l + 1>(H, cm, m, x, zr, zi, qa, qb);
+  if constexpr (l < sh::kL) sh_degree_par<l + 1>(H, cb, cm, m, par, x, zr, zi, qa, qb);
 }
 
-// One workgroup per (object, 64 directions); wave w sums the orders
-// m = w, w + 4, ..; then the usual reduction and evaluation at order P.
-__global__ void __launch_bounds__(kLanes * kWaves) k_series_sh(
+__global__ void __launch_bounds__(kShThreads) k_series_sh(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
     int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
     unsigned* __restrict__ colmax) {
-  __shared__ double red[P + 1][kLanes];
+  __shared__ double2 cbuf[2][kShPairs0];
+  __shared__ double eo[kShChunk][2][kShDirs];
+  __shared__ double dcs[kShDirs];
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
   if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE || !d.shc) return;
-  const int lane = threadIdx.x & (kLanes - 1);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int j = bo.blk * kLanes + lane;
+  const int tid = threadIdx.x;
+  const int par = __builtin_amdgcn_readfirstlane(tid >> 7);  // 0: even n, 1: odd n
+  const int dl = tid & (kShDirs - 1);
+  const int j = bo.blk * kShDirs + dl;
   const int jc = j < Mh ? j : Mh - 1;
   const double dx = krep[3 * (size_t)Qh + jc], dy = krep[4 * (size_t)Qh + jc],
                dz = krep[5 * (size_t)Qh + jc];
-  // p = A^T d / s (A row-major in dofs[3:12])
+  if (par == 0) dcs[dl] = dx * d.ctr[0] + dy * d.ctr[1] + dz * d.ctr[2];
   const auto* a = as_const(d.dofs);
   const double sc = d.shs, inv = 1.0 / sc;
   const double px = (a[3] * dx + a[6] * dy + a[9] * dz) * inv;
   const double py = (a[4] * dx + a[7] * dy + a[10] * dz) * inv;
   const double x = (a[5] * dx + a[8] * dy + a[11] * dz) * inv;
-  double H[P + 1];
+  double H[kShH];
 #pragma unroll
-  for (int n = 0; n <= P; ++n) H[n] = 0.0;
-  // z^wave, and z^4 for the stride of m
-  double zr = 1.0, zi = 0.0;
-  for (int t = 0; t < wave; ++t) {
-    const double r = zr * px - zi * py;
-    zi = zr * py + zi * px;
-    zr = r;
-  }
-  const double z2r = px * px - py * py, z2i = 2.0 * px * py;
-  const double z4r = z2r * z2r - z2i * z2i, z4i = 2.0 * z2r * z2i;
-  const auto* cf = as_const(d.shc);
-  for (int m = wave; m <= sh::kL; m += kWaves) {
-    // pair (m, l, n) at mstart[m] + C[l] - C[m] + (n - n0(l)) / 2
-    const auto* cm = cf + 2 * (kShLayout.mstart[m] - kShLayout.C[m]);
+  for (int i = 0; i < kShH; ++i) H[i] = 0.0;
+  const double2* src = reinterpret_cast<const double2*>(d.shc);
+  // staging of one order's pairs: every thread up to kShStage of them
+  double2 r0 = {}, r1 = {}, r2 = {}, r3 = {};
+  static_assert(kShStage == 4, "staging registers");
+  auto fetch = [&](int m) {
+    const int last = kShLayout.C[sh::kL + 1] - kShLayout.C[m] - 1;
+    const double2* g = src + kShLayout.mstart[m];
+    r0 = g[min(tid, last)];
+    r1 = g[min(tid + kShThreads, last)];
+    r2 = g[min(tid + 2 * kShThreads, last)];
+    r3 = g[min(tid + 3 * kShThreads, last)];
+  };
+  auto put = [&](int b, int m) {
+    const int np = kShLayout.C[sh::kL + 1] - kShLayout.C[m];
+    if (tid < np) cbuf[b][tid] = r0;
+    if (tid + kShThreads < np) cbuf[b][tid + kShThreads] = r1;
+    if (tid + 2 * kShThreads < np) cbuf[b][tid + 2 * kShThreads] = r2;
+    if (tid + 3 * kShThreads < np) cbuf[b][tid + 3 * kShThreads] = r3;
+  };
+  fetch(0);
+  put(0, 0);
+  __syncthreads();
+  double zr = 1.0, zi = 0.0;  // z^m
+  for (int m = 0; m <= sh::kL; ++m) {
+    const int cur = m & 1;
+    if (m < sh::kL) fetch(m + 1);
     double qa = 0.0, qb = 0.0;
-    sh_degree<0>(H, cm, m, x, zr, zi, qa, qb);
-    const double r = zr * z4r - zi * z4i;
-    zi = zr * z4i + zi * z4r;
-    zr = r;
+    sh_degree_par<0>(H, &cbuf[cur][0], kShLayout.C[m], m, par, x, zr, zi, qa, qb);
+    if (m < sh::kL) put(cur ^ 1, m + 1);
+    __syncthreads();
+    const double t = zr * px - zi * py;
+    zi = zr * py + zi * px;
+    zr = t;
   }
-  double f = sc * sc;
+  // series coefficients s^(n+2) c_n H_n, n = 2 i + par
+  {
+    double f = sc * sc;
+    if (par) f *= sc;
+    const double s2 = sc * sc;
 #pragma unroll
-  for (int n = 0; n <= P; ++n) {
-    H[n] *= f;
-    f *= sc;
+    for (int i = 0; i < kShH; ++i) {
+      H[i] *= f * series_coef<kKindTri>(2 * i + (i < kShH - 1 ? par : 0));
+      f *= s2;
+    }
   }
-  reduce_and_evaluate<P, kKindTri>(H, red, wave, lane, j, jc, dx, dy, dz, d, krep, Qh, Mh,
-                                   out_mode, Aout, ldA, Xout, ldX, colmax);
+  const int nser = d.nser;
+  float mx = 0.0f;
+  for (int s0 = 0; s0 < nser; s0 += kShChunk) {
+    // this wave's half (E, or O) at up to kShChunk shells
+    for (int t = 0; t < kShChunk && s0 + t < nser; ++t) {
+      const double kap = krep[6 * (size_t)Qh + (size_t)(s0 + t) * Mh + jc];
+      const double k2 = kap * kap;
+      double v;
+      if (par == 0) {
+        v = H[kShH - 1];
+#pragma unroll
+        for (int i = kShH - 2; i >= 0; --i) v = fma(v, k2, H[i]);
+      } else {
+        v = H[kShH - 2];
+#pragma unroll
+        for (int i = kShH - 3; i >= 0; --i) v = fma(v, k2, H[i]);
+        v *= kap;
+      }
+      eo[t][par][dl] = v;
+    }
+    __syncthreads();
+    for (int it = tid; it < kShChunk * kShDirs; it += kShThreads) {
+      const int t = it / kShDirs, jl = it - t * kShDirs;
+      const int jj = bo.blk * kShDirs + jl;
+      if (s0 + t >= nser || jj >= Mh) continue;
+      const int q = (s0 + t) * Mh + jj;
+      const double kap = krep[6 * (size_t)Qh + q];
+      const double E = eo[t][0][jl], O = eo[t][1][jl];
+      double sn, cs;
+      sincos(kap * dcs[jl], &sn, &cs);
+      double ax = cs * E - sn * O, ay = sn * E + cs * O;
+      if (kap < 1e-10) { ax = 0.0; ay = 0.0; }  // A(0) = 0 (triangle_mesh_dof.cpp:28)
+      if (out_mode == 0) {
+        Aout[(size_t)d.col * ldA + q] = make_double2(ax, ay);
+      } else {
+        const double sw = sqrt(krep[7 * (size_t)Qh + q]);
+        const float2 xv = make_float2((float)(ax * sw), (float)(ay * sw));
+        Xout[(size_t)d.col * ldX + q] = xv;
+        mx = fmaxf(mx, fmaxf(fabsf(xv.x), fabsf(xv.y)));
+      }
+    }
+    __syncthreads();
+  }
+  if (out_mode == 1) {
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if ((tid & (kLanes - 1)) == 0) atomicMax(colmax + d.col, __float_as_uint(mx));
+  }
 }
 
 }  // namespace
@@ -496,8 +581,8 @@ void launch_series_sh(hipStream_t s, const ObjDesc* d_objs, int N, const double*
                       int Mh, int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                       unsigned* d_colmax) {
   if (N <= 0 || Mh <= 0) return;
-  hipLaunchKernelGGL(k_series_sh, dim3((Mh + kLanes - 1) / kLanes, N), dim3(kLanes * kWaves), 0,
-                     s, d_objs, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
+  hipLaunchKernelGGL(k_series_sh, dim3((Mh + kShDirs - 1) / kShDirs, N), dim3(kShThreads), 0, s,
+                     d_objs, d_krep, Qh, Mh, out_mode, d_A, ldA, d_X, ldX, d_colmax);
 }
 
 void launch_series_tri_raw(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_dirs,
