@@ -82,22 +82,25 @@ __device__ __forceinline__ float j1ox_small(float x) {
   return s * y + 0.5f;
 }
 
-// sinc(h) from h and sin(h): polynomial below |h| = kSincPoly, sin(h)/h above
-// (both evaluated, then selected, so lanes do not diverge).
-__device__ __forceinline__ float sinc_from(float h, float sin_h) {
-  // 1 - h^2/6 + h^4/120 - h^6/5040: the first omitted term, h^8/9!, is below
-  // 1.1e-8 for |h| < 0.5
-  const float h2 = h * h;
-  float p = h2 * (-1.0f / 5040.0f) + (1.0f / 120.0f);
-  p = p * h2 - (1.0f / 6.0f);
-  p = p * h2 + 1.0f;
-  const float r = sin_h * __builtin_amdgcn_rcpf(h);
-  return fabsf(h) < kSincPoly ? p : r;
+// sinc(h) / 2 from w = 2 h and sin(h): polynomial below |h| = kSincPoly,
+// sin(h) / w above (both evaluated, then selected, so lanes do not diverge).
+// The per-face terms are accumulated at half scale (their factor 2 is
+// applied once per patch), so the half spans need no multiplication by 1/2.
+__device__ __forceinline__ float half_sinc_from(float w, float sin_h) {
+  // (1 - h^2/6 + h^4/120 - h^6/5040) / 2 with h = w/2: the first omitted
+  // term, h^8/9!, is below 1.1e-8 for |h| < 0.5
+  const float w2 = w * w;
+  float p = w2 * (-1.0f / 645120.0f) + (1.0f / 3840.0f);
+  p = p * w2 - (1.0f / 48.0f);
+  p = p * w2 + 0.5f;
+  const float r = sin_h * __builtin_amdgcn_rcpf(w);
+  return fabsf(w) < 2.0f * kSincPoly ? p : r;
 }
 
 // Phi(a, b) = sum_{n<NT} c_n h_n(a, b), c_n = 2 i^n / (n+2)!, in FP64, times
-// (g2/2) e^{i u0} with e^{i u0} = Q0^2.  |h_n| <= (n+1) H^n for span H, so the
-// truncation error is bounded by the first omitted term 2(NT+1) H^NT / (NT+2)!.
+// (g2/2) e^{i u0} with e^{i u0} = Q0^2, accumulated at half scale.
+// |h_n| <= (n+1) H^n for span H, so the truncation error is bounded by the
+// first omitted term 2(NT+1) H^NT / (NT+2)!.
 template <int NT>
 __device__ __forceinline__ void tri_taylor(double al, double be, float g2, float2 Q0,
                                            float2& acc) {
@@ -114,7 +117,7 @@ __device__ __forceinline__ void tri_taylor(double al, double be, float g2, float
   }
   const float px = Q0.x * Q0.x - Q0.y * Q0.y;
   const float py = 2.0f * Q0.x * Q0.y;
-  const float tr = (float)re * (0.5f * g2), ti = (float)im * (0.5f * g2);
+  const float tr = (float)re * (0.25f * g2), ti = (float)im * (0.25f * g2);
   acc.x = fmaf(px, tr, fmaf(-py, ti, acc.x));
   acc.y = fmaf(px, ti, fmaf(py, tr, acc.y));
 }
@@ -123,30 +126,32 @@ __device__ __forceinline__ void tri_taylor(double al, double be, float g2, float
 // The formula is symmetric in p and q, so only the middle node matters: it
 // is node 0 when a and b differ in sign, else the one of nodes 1, 2 whose
 // phase is smaller in magnitude.  u_m is the median of (0, a, b)
-// (v_med3); Q_p = Q1 when m = 0 (else Q0), Q_q = Q1 when m = 2 (else Q2);
-// everything by selects.  Half spans and the span are formed in FP32 from
+// (v_med3); Q_p = Q1 when m = 0 (else Q0), Q_q = Q1 when m = 2 (else Q2),
+// each read from the phasor table at its vertex's slot (s0, s1, s2).  Half spans and the span are formed in FP32 from
 // the FP32-rounded alpha, beta: their absolute error, 6e-8 |k| diam, changes
 // the result by a relative 6e-8 at most, because the span D itself is at
 // least that large here.
-__device__ __forceinline__ void tri_main(double al, double be, float g2, float2 Q0, float2 Q1,
-                                         float2 Q2, float2& acc) {
+__device__ __forceinline__ void tri_main(double al, double be, float g2, int s0, int s1, int s2,
+                                         const float2 (*tab)[kLanes], int lane, float2& acc) {
   const float a = (float)al, b = (float)be;
   const bool m0 = a * b <= 0.0f;
   const bool m2 = (!m0) & (fabsf(b) <= fabsf(a));
   const float um = __builtin_amdgcn_fmed3f(0.0f, a, b);
   const float up = m0 ? a : 0.0f;
   const float uq = m2 ? a : b;
-  const bool m1 = !(m0 | m2);
-  const float2 Qm = make_float2(m0 ? Q0.x : (m1 ? Q1.x : Q2.x), m0 ? Q0.y : (m1 ? Q1.y : Q2.y));
-  const float2 Qp = make_float2(m0 ? Q1.x : Q0.x, m0 ? Q1.y : Q0.y);
-  const float2 Qq = make_float2(m2 ? Q1.x : Q2.x, m2 ? Q1.y : Q2.y);
-  const float hp = 0.5f * (um - up);
-  const float hq = 0.5f * (uq - um);
+  // the three phasors read in (p, m, q) order: the selects act on the
+  // vertices' table slots, not on the phasors
+  const int vm = m0 ? s0 : (m2 ? s2 : s1);
+  const int vp = m0 ? s1 : s0;
+  const int vq = m2 ? s1 : s2;
+  const float2 Qm = tab[vm][lane], Qp = tab[vp][lane], Qq = tab[vq][lane];
+  const float wp = um - up;                        // 2 hp
+  const float wq = uq - um;                        // 2 hq
   const float D = uq - up;
   const float sin_p = Qm.y * Qp.x - Qm.x * Qp.y;   // Im(Qm conj Qp) = sin(hp)
   const float sin_q = Qq.y * Qm.x - Qq.x * Qm.y;   // Im(Qq conj Qm) = sin(hq)
-  const float sp = sinc_from(hp, sin_p);
-  const float sq = sinc_from(hq, sin_q);
+  const float sp = half_sinc_from(wp, sin_p);      // sinc(hp) / 2
+  const float sq = half_sinc_from(wq, sin_q);
   const float nx = Qq.x * sq - Qp.x * sp;
   const float ny = Qq.y * sq - Qp.y * sp;
   const float tx = Qm.x * nx - Qm.y * ny;
@@ -161,12 +166,13 @@ __device__ __forceinline__ void tri_main(double al, double be, float g2, float2 
 // max(0, a, b) - min(0, a, b) = max(|a|, |b|, |a - b|), taken in FP32 (both
 // sides of the threshold are accurate near it).  Lanes below kLaneTaylor
 // take the 7-term Taylor sum (first omitted term < 1e-8 there).
-__device__ __forceinline__ void tri_term(double al, double be, float g2, float2 Q0, float2 Q1,
-                                         float2 Q2, float2& acc) {
+__device__ __forceinline__ void tri_term(double al, double be, float g2, float2 Q0, int s0, int s1,
+                                         int s2, const float2 (*tab)[kLanes], int lane,
+                                         float2& acc) {
   const float a = (float)al, b = (float)be;
   const float H = fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(a - b));
   if (H < kLaneTaylor) tri_taylor<7>(al, be, g2, Q0, acc);
-  else tri_main(al, be, g2, Q0, Q1, Q2, acc);
+  else tri_main(al, be, g2, s0, s1, s2, tab, lane, acc);
 }
 
 // J1(x)/x for x >= 4 in FP32 from the modulus and phase of J1 + i Y1:
@@ -272,7 +278,8 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
         const float g2 = hx * rf[0] + hy * rf[1] + hz * rf[2];
         const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
         const int code = (int)(loc >> 24);
-        const float2 Q0 = tab[loc & 0xff][lane];
+        const int s0 = loc & 0xff, s1 = (loc >> 8) & 0xff, s2 = (loc >> 16) & 0xff;
+        const float2 Q0 = tab[s0][lane];
         // code <= cut[j]  =>  H <= kmax * 2^(code-64) <= thr[j] for every lane
         if (code <= cut[3]) {
           if (code <= cut[0]) tri_taylor<4>(al, be, g2, Q0, pacc);
@@ -285,13 +292,11 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
           else if (code <= cut[6]) tri_taylor<10>(al, be, g2, Q0, pacc);
           else tri_taylor<11>(al, be, g2, Q0, pacc);
         } else {
-          const float2 Q1 = tab[(loc >> 8) & 0xff][lane];
-          const float2 Q2 = tab[(loc >> 16) & 0xff][lane];
-          tri_term(al, be, g2, Q0, Q1, Q2, pacc);
+          tri_term(al, be, g2, Q0, s0, s1, s2, tab, lane, pacc);
         }
       }
-      accx += (double)pacc.x;
-      accy += (double)pacc.y;
+      accx += 2.0 * (double)pacc.x;   // the terms' factor 2 (half-scale sums)
+      accy += 2.0 * (double)pacc.y;
       __syncthreads();
     }
   } else {
