@@ -185,8 +185,12 @@ __global__ void __launch_bounds__(kWaves * 64) k_gram_planes(const uint4* __rest
   for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {
     const int qi = find_query(qs, nq, item, &GramQuery::item0);
     const GramQuery q = qs[qi];
+    // item = 8 (octet * ntiles + tile) + slice % 8 (see gram_plan_layout);
+    // slices past the query's last are empty items
     const int local = (int)(item - q.item0);
-    const int t = local / q.nslice, slice = local - t * q.nslice;
+    const int ot = local >> 3;
+    const int t = ot % q.ntiles, slice = 8 * (ot / q.ntiles) + (local & 7);
+    if (slice >= q.nslice) continue;   // uniform over the workgroup
     const int nt = (q.n + kTileCols - 1) / kTileCols;
     int ti, tj;
     decode_tile(t, nt, ti, tj);
@@ -376,6 +380,7 @@ GramQuery gram_query(int c0, int n, int K) {
   ns = std::min<long long>(ns, std::max(1, S / 8));
   ns = std::min<long long>(ns, std::max<long long>(1, (3LL * K) / (8LL * n)));
   ns = std::max<long long>(ns, 1);
+  if (ns > 8) ns = (ns + 7) / 8 * 8;   // whole groups of eight slices (gram_plan_layout)
   q.slice_len = (int)((S + ns - 1) / ns);
   q.nslice = (S + q.slice_len - 1) / q.slice_len;
   return q;
@@ -390,7 +395,13 @@ void gram_plan_layout(std::vector<GramQuery>& qs, GramPlanHeader& h) {
     q.part = part;
     q.entry0 = entry;
     q.v = v;
-    item += (long long)q.ntiles * q.nslice;
+    // Items in groups of eight slices: item0 + 8 (octet * ntiles + tile) +
+    // slice % 8.  Workgroups go to the eight XCDs round-robin by linear id
+    // and the grids are multiples of 8, so every item of XCD x has a slice
+    // = x (mod 8): each XCD reads only the K ranges of its slices, for all
+    // tiles at once, and shares them through its L2 (eight times fewer HBM
+    // reads of the planes than with the tiles' slices spread over all XCDs).
+    item += 8LL * ((q.nslice + 7) / 8) * q.ntiles;
     part += (long long)q.nslice * q.n * q.n;
     entry += (long long)q.n * q.n;
     v += (long long)q.n * q.n;
@@ -403,7 +414,10 @@ void gram_plan_layout(std::vector<GramQuery>& qs, GramPlanHeader& h) {
 }
 
 int gram_planes_grid(long long nitems, int cap) {
-  return (int)std::max<long long>(1, std::min<long long>(nitems, cap));
+  // a multiple of 8 (nitems is; cap is rounded down), so that a workgroup's
+  // items all fall on its XCD's slices
+  const long long c = std::max(8, cap - cap % 8);
+  return (int)std::max<long long>(8, std::min<long long>(nitems, c));
 }
 
 void launch_split_planes(hipStream_t s, int fp16, const float* d_X, int ldK, int ncols,
