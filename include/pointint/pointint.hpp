@@ -6,7 +6,8 @@
 //   geometry types / factories   cpp/include/geometry/{types,geometry,packing}.hpp
 //   quadrature                   cpp/include/quadrature/{gauss_legendre,lebedev_io,kgrid}.hpp
 //   flags                        cpp/include/computation_flags.hpp
-//   DoF parameterisations        cpp/include/dof/{affine_dof,triangle_mesh_dof}.hpp
+//   DoF parameterisations        cpp/include/dof/{dof_parameterization,affine_dof,triangle_mesh_dof}.hpp
+//   kernel registry              cpp/include/dof/cuda/dof_cuda_interface.hpp
 //   matrix driver                cpp/include/compute_intersection_volume_multi_object.hpp:51-120
 //   pairwise driver + single-k   cpp/include/compute_intersection_volume.hpp:48-135
 //   self volume                  cpp/include/compute_volume.hpp
@@ -28,6 +29,7 @@
 
 #include <array>
 #include <complex>
+#include <functional>
 #include <memory>
 #include <string>
 #include <utility>
@@ -131,29 +133,92 @@ inline bool needs_volume(ComputationFlags f) { return (static_cast<int>(f) & 1) 
 inline bool needs_gradient(ComputationFlags f) { return (static_cast<int>(f) & 2) != 0; }
 inline bool needs_hessian(ComputationFlags f) { return (static_cast<int>(f) & 4) != 0; }
 
-// ---- DoF parameterisations ----
+// ---- DoF parameterisations (cpp/include/dof/dof_parameterization.hpp:10-38) ----
 struct DoFParameterization {
   virtual ~DoFParameterization() = default;
   virtual int num_dofs() const = 0;
-  virtual int dof_kind() const = 0;  // PII_DOF_AFFINE or PII_DOF_TRIMESH
+  // PII_DOF_AFFINE (0) or PII_DOF_TRIMESH (1) for the built-in kinds, which
+  // the drivers evaluate with this library's kernels; -1 for a caller's own
+  // parameterisation, which the drivers reach through CudaKernelRegistry.
+  virtual int dof_kind() const { return -1; }
+  // A(k) and dA(k)/dtheta (num_dofs complex values) at one k-vector; the
+  // built-in kinds evaluate them on the device in FP64.
+  virtual std::complex<double> compute_A(const Geometry& geom, const Vector3d& k,
+                                         const VectorXd& dofs) const = 0;
+  virtual VectorXcd compute_A_gradient(const Geometry& geom, const Vector3d& k,
+                                       const VectorXd& dofs) const = 0;
+  // Divergence-theorem volume V = (1/3) sum_S x.n dS of the posed geometry
+  // and dV/dtheta (host arithmetic).
+  virtual double compute_volume(const Geometry& geom, const VectorXd& dofs) const = 0;
+  virtual VectorXd compute_volume_gradient(const Geometry& geom, const VectorXd& dofs) const = 0;
 };
 // [tx, ty, tz, A00, A01, A02, A10, A11, A12, A20, A21, A22]
+// (cpp/include/dof/affine_dof.hpp, cpp/src/dof/affine_dof.cpp)
 struct AffineDoF : DoFParameterization {
   int num_dofs() const override { return 12; }
   int dof_kind() const override { return 0; }
-  // |det A| x the divergence-theorem volume of the reference geometry
-  // (cpp/src/dof/affine_dof.cpp:324-384), on the host.
-  double compute_volume(const Geometry& geom, const VectorXd& dofs) const;
+  std::complex<double> compute_A(const Geometry& geom, const Vector3d& k,
+                                 const VectorXd& dofs) const override;
+  VectorXcd compute_A_gradient(const Geometry& geom, const Vector3d& k,
+                               const VectorXd& dofs) const override;
+  // |det A| x the volume of the reference geometry (affine_dof.cpp:324-384)
+  double compute_volume(const Geometry& geom, const VectorXd& dofs) const override;
+  // 0 for t; sign(det A) cof(A) V_ref for A (affine_dof.cpp:386-460)
+  VectorXd compute_volume_gradient(const Geometry& geom, const VectorXd& dofs) const override;
 };
 // [v0x, v0y, v0z, v1x, ...]
+// (cpp/include/dof/triangle_mesh_dof.hpp, cpp/src/dof/triangle_mesh_dof.cpp)
 struct TriangleMeshDoF : DoFParameterization {
   explicit TriangleMeshDoF(int num_vertices) : nv_(num_vertices) {}
   int num_dofs() const override { return 3 * nv_; }
   int dof_kind() const override { return 1; }
+  std::complex<double> compute_A(const Geometry& geom, const Vector3d& k,
+                                 const VectorXd& dofs) const override;
+  VectorXcd compute_A_gradient(const Geometry& geom, const Vector3d& k,
+                               const VectorXd& dofs) const override;
+  // (1/6) sum_f v_i . (v_j x v_k) over the faces' vertex ids, faces with a
+  // negative id skipped (triangle_mesh_dof.cpp:145-179)
+  double compute_volume(const Geometry& geom, const VectorXd& dofs) const override;
+  // (1/6) sum_f of v_j x v_k into vertex i, cyclically (triangle_mesh_dof.cpp:181-222)
+  VectorXd compute_volume_gradient(const Geometry& geom, const VectorXd& dofs) const override;
 
  private:
   int nv_;
 };
+
+// ---- kernel registry (cpp/include/dof/cuda/dof_cuda_interface.hpp:19-67) ----
+// A form-factor plug-in writes A(k_q) for the Q nodes of kgrid (double2, HIP
+// device memory) and its gradient plug-in the Q x num_dofs row-major
+// dA/dtheta.  The drivers look every object up by
+// make_key(get_dof_type_name(dof), geometry type): the built-in entries
+// ("AffineDoF_{Triangle,Disk,Gaussian}", "TriangleMeshDoF_Triangle",
+// registered when the library loads) run this library's fused kernels; an
+// entry a caller registers or replaces runs through its own functions
+// (columns gathered on the host, one FP64 device Gram for V, the gradients
+// and the Gauss-Newton blocks).  blockSize is passed through and ignored by
+// the built-in entries.
+using CudaComputeAkFunc = std::function<void(const Geometry& geom, const KGrid& kgrid,
+                                             const VectorXd& dofs, double2* d_A_out,
+                                             int blockSize)>;
+using CudaComputeAkGradientFunc = std::function<void(const Geometry& geom, const KGrid& kgrid,
+                                                     const VectorXd& dofs, double2* d_grad_A_out,
+                                                     int blockSize)>;
+class CudaKernelRegistry {
+ public:
+  static void register_kernels(const std::string& dof_type, GeometryType geom_type,
+                               CudaComputeAkFunc compute_Ak,
+                               CudaComputeAkGradientFunc compute_Ak_gradient);
+  static bool has_kernels(const std::string& dof_type, GeometryType geom_type);
+  static std::pair<CudaComputeAkFunc, CudaComputeAkGradientFunc> get_kernels(
+      const std::string& dof_type, GeometryType geom_type);
+  static std::string make_key(const std::string& dof_type, GeometryType geom_type);
+};
+// "AffineDoF", "TriangleMeshDoF" or "Unknown" (dof_cuda_interface.cpp:57-69).
+std::string get_dof_type_name(const std::shared_ptr<DoFParameterization>& dof);
+// (Re-)register the built-in entries (affine_dof_cuda.cu:1323-1358,
+// triangle_mesh_dof_cuda.cu:460-478).
+void register_affine_dof_cuda_kernels();
+void register_triangle_mesh_dof_cuda_kernels();
 
 // ---- results ----
 struct IntersectionVolumeMatrixResult {
@@ -165,6 +230,12 @@ struct IntersectionVolumeMatrixResult {
   // Re(dA_i/dtheta_i (dA_j/dtheta_j)^H); hessian_ii / hessian_jj are zero.
   std::vector<std::vector<MatrixXd>> hessian_ii, hessian_jj, hessian_ij;
   int num_objects() const { return volume_matrix.rows(); }
+};
+
+// cpp/include/compute_volume.hpp:14-21
+struct VolumeResult {
+  double volume = 0.0;
+  VectorXd grad;  // dV/dtheta (GRADIENT)
 };
 
 struct IntersectionVolumeResult {
@@ -208,7 +279,14 @@ VectorXcd compute_Ak_gradient_cuda(const Geometry& geom, const Vector3d& k,
                                    const std::shared_ptr<DoFParameterization>& dof,
                                    const VectorXd& dofs, int blockSize = 256);
 
-// Divergence-theorem self volume with identity DoFs (compute_volume.hpp).
+// Divergence-theorem self volume (cpp/include/compute_volume.hpp:31-44):
+// dof->compute_volume / compute_volume_gradient by the flags, and the
+// identity-AffineDoF overload.
+VolumeResult compute_volume_cpu(const Geometry& ref_geom,
+                                const std::shared_ptr<DoFParameterization>& dof,
+                                const VectorXd& dofs,
+                                ComputationFlags flags = ComputationFlags::VOLUME_ONLY,
+                                bool enable_profiling = false);
 double compute_volume_cpu(const Geometry& geom, bool enable_profiling = false);
 
 // cpp/include/analytical_intersection.hpp

@@ -21,6 +21,7 @@ OUT_CXX = os.path.join(HERE, "libpointint_cxx.so")
 OUT_EXAMPLE = os.path.join(HERE, "example_cube_pair")
 OUT_ALIGN = os.path.join(HERE, "pointint_align")
 ROCM_INCLUDE = "/opt/rocm/include"
+ROCM_LIB = "/opt/rocm/lib"
 BUILD_DIR = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -68,9 +69,10 @@ def build(verbose: bool = False) -> str:
 
 
 def build_cxx() -> None:
-    """The reference-named C++ API (host code on the C ABI: matrix driver and
-    alignment) and its two programs, compiled with g++ and linked against
-    libpointint_hip.so."""
+    """The reference-named C++ API (host code on the C ABI: matrix driver,
+    kernel registry and alignment) and its two programs, compiled with g++ and
+    linked against libpointint_hip.so (and the HIP runtime, for the device
+    buffers of caller-registered plug-ins)."""
     cxx = os.environ.get("CXX", "g++")
     common = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE,
               "-I", ROCM_INCLUDE, "-Wall"]
@@ -84,10 +86,10 @@ def build_cxx() -> None:
 
     run(OUT_CXX, [cxx, *common, "-shared", os.path.join(CXX_DIR, "pointint_cxx.cpp"),
                   os.path.join(CXX_DIR, "align.cpp"), "-L", HERE, "-lpointint_hip",
-                  "-Wl,-rpath,$ORIGIN"], "C++ API")
+                  "-L", ROCM_LIB, "-lamdhip64", "-Wl,-rpath,$ORIGIN"], "C++ API")
     for out, src in ((OUT_EXAMPLE, "example_cube_pair.cpp"), (OUT_ALIGN, "align_main.cpp")):
         run(out, [cxx, *common, os.path.join(CXX_DIR, src), "-L", HERE, "-lpointint_cxx",
-                  "-lpointint_hip", "-Wl,-rpath,$ORIGIN"], src)
+                  "-lpointint_hip", "-L", ROCM_LIB, "-lamdhip64", "-Wl,-rpath,$ORIGIN"], src)
 
 
 if __name__ == "__main__":

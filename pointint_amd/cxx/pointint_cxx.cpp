@@ -1,17 +1,24 @@
 // C++ API with the reference's names (include/pointint/pointint.hpp),
-// implemented on the C ABI of libpointint_hip.so.  Host code only: no HIP
-// calls here, every device operation goes through pii_* entry points.
+// implemented on the C ABI of libpointint_hip.so.  Host code only: every
+// device computation goes through pii_* entry points; the HIP runtime is
+// called directly only for the device buffers handed to caller-registered
+// form-factor plug-ins (CudaKernelRegistry).
 //
 // Error behaviour follows the reference drivers
 // (cpp/src/compute_intersection_volume_multi_object.cu:250-314): a message
 // on std::cerr and a zero (or empty) result, no exceptions.
 #include "pointint/pointint.hpp"
 
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <iomanip>
 #include <iostream>
 #include <mutex>
 #include <stdexcept>
+#include <unordered_map>
 
 #include "cxx_internal.hpp"
 #include "pointint_hip.h"
@@ -128,6 +135,186 @@ double3 d3(double x, double y, double z) {
   return v;
 }
 
+}  // namespace
+
+// ---- kernel registry (cpp/src/dof/cuda/dof_cuda_interface.cpp) ----
+namespace detail {
+
+struct RegistryEntry {
+  CudaComputeAkFunc ak;
+  CudaComputeAkGradientFunc grad;
+  bool builtin = false;  // this library's kernels (the drivers' fused path)
+};
+
+std::unordered_map<std::string, RegistryEntry>& registry() {
+  static std::unordered_map<std::string, RegistryEntry> r;
+  return r;
+}
+std::mutex& registry_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+void put_entry(const std::string& key, RegistryEntry e) {
+  std::lock_guard<std::mutex> lk(registry_mutex());
+  registry()[key] = std::move(e);
+}
+
+bool find_entry(const std::string& key, RegistryEntry& e) {
+  std::lock_guard<std::mutex> lk(registry_mutex());
+  auto it = registry().find(key);
+  if (it == registry().end()) return false;
+  e = it->second;
+  return true;
+}
+
+// True when the registry entry of (dof type, geometry type) is one of this
+// library's built-in entries.
+bool builtin_entry(const std::string& dof_type, GeometryType t) {
+  RegistryEntry e;
+  return find_entry(CudaKernelRegistry::make_key(dof_type, t), e) && e.builtin;
+}
+
+KGrid one_node_grid(const Vector3d& k) {
+  KGrid g;
+  const double km = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  g.dirs.push_back(km > 0 ? Vector3d{k[0] / km, k[1] / km, k[2] / km} : Vector3d{1.0, 0.0, 0.0});
+  g.kmag.push_back(km);
+  g.w.push_back(1.0);
+  return g;
+}
+
+// Runs the registry entry of dof's type on geom over kgrid: A(k_q) into a
+// (Q), and with need_grad the row-major Q x num_dofs dA/dtheta into g, through
+// device buffers handed to the plug-ins.
+bool run_plugin(const Geometry& geom, const KGrid& kgrid,
+                const std::shared_ptr<DoFParameterization>& dof, const VectorXd& dofs,
+                int blockSize, bool need_grad, std::vector<std::complex<double>>& a,
+                std::vector<std::complex<double>>& g) {
+  const std::string type = get_dof_type_name(dof);
+  RegistryEntry e;
+  if (!find_entry(CudaKernelRegistry::make_key(type, geom.type), e) || !e.ak ||
+      (need_grad && !e.grad)) {
+    std::cerr << "Error: CUDA kernels not available for DoF type '" << type
+              << "' with geometry type " << (int)geom.type << std::endl;
+    return false;
+  }
+  const size_t Q = kgrid.kmag.size(), nd = dofs.size();
+  const size_t n_a = Q, n_g = need_grad ? Q * nd : 0;
+  double2* d = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&d), sizeof(double2) * (n_a + n_g + 1)) != hipSuccess) {
+    std::cerr << "Error: device allocation for the form-factor plug-in failed" << std::endl;
+    return false;
+  }
+  bool ok = hipMemset(d, 0, sizeof(double2) * (n_a + n_g + 1)) == hipSuccess;
+  if (ok) e.ak(geom, kgrid, dofs, d, blockSize);
+  if (ok && need_grad) e.grad(geom, kgrid, dofs, d + n_a, blockSize);
+  std::vector<double2> h(n_a + n_g);
+  ok = ok && hipDeviceSynchronize() == hipSuccess &&
+       hipMemcpy(h.data(), d, sizeof(double2) * (n_a + n_g), hipMemcpyDeviceToHost) == hipSuccess;
+  (void)hipFree(d);
+  if (!ok) {
+    std::cerr << "Error: the form-factor plug-in of '" << type << "' failed" << std::endl;
+    return false;
+  }
+  a.resize(n_a);
+  for (size_t q = 0; q < n_a; ++q) a[q] = {h[q].x, h[q].y};
+  g.resize(n_g);
+  for (size_t i = 0; i < n_g; ++i) g[i] = {h[n_a + i].x, h[n_a + i].y};
+  return true;
+}
+
+// Built-in entries: the library's FP64 form factor / Jacobian of one object
+// written into the plug-in's device buffer (the plug-in contract of
+// cpp/include/dof/cuda/dof_cuda_interface.hpp:24-40).
+void builtin_ak(int kind, const Geometry& geom, const KGrid& kgrid, const VectorXd& dofs,
+                double2* d_out, bool grad) {
+  pii_context* ctx = context();
+  if (!ctx) return;
+  CallObjects co;
+  co.kg = upload(ctx, kgrid);
+  if (!co.kg) return;
+  pii_geometry* g = upload(ctx, geom);
+  if (!g) return;
+  co.geoms.push_back(g);
+  double* out = reinterpret_cast<double*>(d_out);
+  if (grad)
+    report(pii_form_factor_jacobian_device(ctx, g, kind, dofs.data(), (int)dofs.size(), co.kg,
+                                           out),
+           "compute_Ak_gradient plug-in");
+  else
+    report(pii_form_factor_device(ctx, g, kind, dofs.data(), (int)dofs.size(), co.kg,
+                                  PII_FF_FP64, out),
+           "compute_Ak plug-in");
+}
+
+void register_builtin(const char* dof_type, GeometryType t, int kind) {
+  RegistryEntry e;
+  e.ak = [kind](const Geometry& g, const KGrid& kg, const VectorXd& d, double2* out, int) {
+    builtin_ak(kind, g, kg, d, out, false);
+  };
+  e.grad = [kind](const Geometry& g, const KGrid& kg, const VectorXd& d, double2* out, int) {
+    builtin_ak(kind, g, kg, d, out, true);
+  };
+  e.builtin = true;
+  put_entry(CudaKernelRegistry::make_key(dof_type, t), std::move(e));
+}
+
+}  // namespace detail
+
+void CudaKernelRegistry::register_kernels(const std::string& dof_type, GeometryType geom_type,
+                                          CudaComputeAkFunc compute_Ak,
+                                          CudaComputeAkGradientFunc compute_Ak_gradient) {
+  detail::RegistryEntry e;
+  e.ak = std::move(compute_Ak);
+  e.grad = std::move(compute_Ak_gradient);
+  detail::put_entry(make_key(dof_type, geom_type), std::move(e));
+}
+
+bool CudaKernelRegistry::has_kernels(const std::string& dof_type, GeometryType geom_type) {
+  detail::RegistryEntry e;
+  return detail::find_entry(make_key(dof_type, geom_type), e);
+}
+
+std::pair<CudaComputeAkFunc, CudaComputeAkGradientFunc> CudaKernelRegistry::get_kernels(
+    const std::string& dof_type, GeometryType geom_type) {
+  detail::RegistryEntry e;
+  if (!detail::find_entry(make_key(dof_type, geom_type), e)) return {};
+  return {e.ak, e.grad};
+}
+
+std::string CudaKernelRegistry::make_key(const std::string& dof_type, GeometryType geom_type) {
+  static const char* const names[] = {"Triangle", "Disk", "Gaussian"};
+  const int t = (int)geom_type;
+  return dof_type + "_" + (t >= 0 && t < 3 ? names[t] : "Unknown");
+}
+
+std::string get_dof_type_name(const std::shared_ptr<DoFParameterization>& dof) {
+  if (!dof) return "Unknown";
+  if (std::dynamic_pointer_cast<AffineDoF>(dof)) return "AffineDoF";
+  if (std::dynamic_pointer_cast<TriangleMeshDoF>(dof)) return "TriangleMeshDoF";
+  return "Unknown";
+}
+
+void register_affine_dof_cuda_kernels() {
+  detail::register_builtin("AffineDoF", GEOM_TRIANGLE, PII_DOF_AFFINE);
+  detail::register_builtin("AffineDoF", GEOM_DISK, PII_DOF_AFFINE);
+  detail::register_builtin("AffineDoF", GEOM_GAUSSIAN, PII_DOF_AFFINE);
+}
+
+void register_triangle_mesh_dof_cuda_kernels() {
+  detail::register_builtin("TriangleMeshDoF", GEOM_TRIANGLE, PII_DOF_TRIMESH);
+}
+
+namespace {
+// the built-in entries, registered when the library loads (as the
+// reference's static registrars do)
+struct BuiltinRegistrar {
+  BuiltinRegistrar() {
+    register_affine_dof_cuda_kernels();
+    register_triangle_mesh_dof_cuda_kernels();
+  }
+} g_builtin_registrar;
 }  // namespace
 
 int Geometry::num_elements() const {
@@ -314,10 +501,90 @@ static void fill_hessian_blocks(IntersectionVolumeMatrixResult& res, const std::
     }
 }
 
+namespace detail {
+
+// The matrix driver through registry plug-ins: every object's A(k_q) (and,
+// for GRADIENT / HESSIAN, its Q x num_dofs dA/dtheta) from its entry's
+// functions, then one FP64 device Gram over [A columns | derivative
+// columns]: its blocks are V, grad_matrix[i][j] = (1/8 pi^3) sum_q w_q
+// Re(dA_i conj A_j) and the Gauss-Newton blocks
+// (compute_intersection_volume_multi_object.cu:437-600).
+IntersectionVolumeMatrixResult volume_matrix_plugins(
+    const std::vector<Geometry>& geoms, const std::vector<std::shared_ptr<DoFParameterization>>& dofs,
+    const std::vector<VectorXd>& dof_vectors, const KGrid& kgrid, ComputationFlags flags,
+    int blockSize) {
+  IntersectionVolumeMatrixResult res;
+  const int n = (int)geoms.size();
+  res.volume_matrix = MatrixXd::Zero(n, n);
+  const bool deriv = needs_gradient(flags) || needs_hessian(flags);
+  const size_t Q = kgrid.kmag.size();
+  std::vector<std::vector<std::complex<double>>> A(n), G(n);
+  std::vector<size_t> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const auto d = dofs[i] ? dofs[i] : std::make_shared<AffineDoF>();
+    if (!run_plugin(geoms[i], kgrid, d, dof_vectors[i], blockSize, deriv, A[i], G[i])) return res;
+    off[i + 1] = off[i] + (deriv ? dof_vectors[i].size() : 0);
+  }
+  const size_t ncol = n + off[n];
+  std::vector<double> J(2 * ncol * Q);
+  for (int i = 0; i < n; ++i) {
+    for (size_t q = 0; q < Q; ++q) {
+      J[2 * ((size_t)i * Q + q)] = A[i][q].real();
+      J[2 * ((size_t)i * Q + q) + 1] = A[i][q].imag();
+    }
+    const size_t nd = off[i + 1] - off[i];
+    for (size_t t = 0; t < nd; ++t)
+      for (size_t q = 0; q < Q; ++q) {
+        const std::complex<double> v = G[i][q * nd + t];
+        const size_t c = n + off[i] + t;
+        J[2 * (c * Q + q)] = v.real();
+        J[2 * (c * Q + q) + 1] = v.imag();
+      }
+  }
+  pii_context* ctx = context();
+  if (!ctx) return res;
+  CallObjects co;
+  co.kg = upload(ctx, kgrid);
+  if (!co.kg) return res;
+  std::vector<double> W(ncol * ncol);
+  if (!report(pii_gram(ctx, J.data(), (int)ncol, co.kg, PII_GRAM_FP64, W.data()),
+              "compute_intersection_volume_matrix_cuda"))
+    return res;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) res.volume_matrix(i, j) = W[(size_t)i * ncol + j];
+  if (needs_gradient(flags)) {
+    res.grad_matrix.assign(n, std::vector<VectorXd>(n));
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        VectorXd& g = res.grad_matrix[i][j];
+        g.resize(off[i + 1] - off[i]);
+        for (size_t t = 0; t < g.size(); ++t) g[t] = W[(n + off[i] + t) * ncol + j];
+      }
+  }
+  if (needs_hessian(flags)) {
+    res.hessian_ii.assign(n, std::vector<MatrixXd>(n));
+    res.hessian_jj.assign(n, std::vector<MatrixXd>(n));
+    res.hessian_ij.assign(n, std::vector<MatrixXd>(n));
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const int di = (int)(off[i + 1] - off[i]), dj = (int)(off[j + 1] - off[j]);
+        res.hessian_ii[i][j] = MatrixXd::Zero(di, di);
+        res.hessian_jj[i][j] = MatrixXd::Zero(dj, dj);
+        MatrixXd& b = res.hessian_ij[i][j];
+        b = MatrixXd(di, dj);
+        for (int a = 0; a < di; ++a)
+          for (int c = 0; c < dj; ++c) b(a, c) = W[(n + off[i] + a) * ncol + n + off[j] + c];
+      }
+  }
+  return res;
+}
+
+}  // namespace detail
+
 IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
     const std::vector<Geometry>& geoms, const std::vector<std::shared_ptr<DoFParameterization>>& dofs,
     const std::vector<VectorXd>& dof_vectors, const KGrid& kgrid, ComputationFlags flags,
-    int /*blockSize*/, bool enable_profiling) {
+    int blockSize, bool enable_profiling) {
   IntersectionVolumeMatrixResult res;
   const int n = (int)geoms.size();
   res.volume_matrix = MatrixXd::Zero(n, n);
@@ -328,6 +595,22 @@ IntersectionVolumeMatrixResult compute_intersection_volume_matrix_cuda(
               << ") or DoF vectors (" << dof_vectors.size() << ")" << std::endl;
     return res;
   }
+  // every object goes through its registry entry
+  // (compute_intersection_volume_multi_object.cu:263-290): this library's
+  // fused path when all are built-in, the plug-ins otherwise
+  bool all_builtin = true;
+  for (int i = 0; i < n; ++i) {
+    const std::string type = dofs[i] ? get_dof_type_name(dofs[i]) : "AffineDoF";
+    detail::RegistryEntry e;
+    if (!detail::find_entry(CudaKernelRegistry::make_key(type, geoms[i].type), e)) {
+      std::cerr << "Error: CUDA kernels not available for DoF type '" << type
+                << "' with geometry type " << (int)geoms[i].type << std::endl;
+      return res;
+    }
+    all_builtin = all_builtin && e.builtin && (!dofs[i] || dofs[i]->dof_kind() >= 0);
+  }
+  if (!all_builtin)
+    return detail::volume_matrix_plugins(geoms, dofs, dof_vectors, kgrid, flags, blockSize);
   pii_context* ctx = context();
   if (!ctx) return res;
   CallObjects co;
@@ -361,9 +644,6 @@ IntersectionVolumeMatrixResult volume_matrix_resident(
     nd[i] = (int)dof_vectors[i].size();
     dp[i] = dof_vectors[i].data();
   }
-  struct Keep {  // kg / co.geoms of the driver body below
-    const pii_kgrid* kg;
-  } co{kg};
   if (enable_profiling) pii_context_set_profiling(ctx, 1);
   bool all_affine = true;
   for (int i = 0; i < n; ++i) all_affine = all_affine && kinds[i] == PII_DOF_AFFINE;
@@ -550,9 +830,11 @@ double compute_intersection_volume_cuda(const Geometry& g1, const Geometry& g2,
   return m.volume_matrix.rows() == 2 ? m.volume_matrix(0, 1) : 0.0;
 }
 
-std::complex<double> compute_Ak_cuda(const Geometry& geom, const Vector3d& k,
-                                     const std::shared_ptr<DoFParameterization>& dof,
-                                     const VectorXd& dofs, int /*blockSize*/) {
+namespace detail {
+
+// A(k) of a built-in DoF kind at one k-vector, FP64 on the device.
+std::complex<double> ak_at(const Geometry& geom, const Vector3d& k, int kind,
+                           const VectorXd& dofs, const char* who) {
   pii_context* ctx = context();
   if (!ctx) return 0.0;
   CallObjects co;
@@ -560,23 +842,22 @@ std::complex<double> compute_Ak_cuda(const Geometry& geom, const Vector3d& k,
   const double dir[3] = {km > 0 ? k[0] / km : 1.0, km > 0 ? k[1] / km : 0.0,
                          km > 0 ? k[2] / km : 0.0};
   const double w = 1.0;
-  if (!report(pii_kgrid_create(ctx, 1, dir, &km, &w, &co.kg), "compute_Ak_cuda")) return 0.0;
+  if (!report(pii_kgrid_create(ctx, 1, dir, &km, &w, &co.kg), who)) return 0.0;
   pii_geometry* g = upload(ctx, geom);
   if (!g) return 0.0;
   co.geoms.push_back(g);
   double out[2] = {0, 0};
-  if (!report(pii_form_factor(ctx, g, dof ? dof->dof_kind() : PII_DOF_AFFINE, dofs.data(),
-                              (int)dofs.size(), co.kg, PII_FF_FP64, out),
-              "compute_Ak_cuda"))
+  if (!report(pii_form_factor(ctx, g, kind, dofs.data(), (int)dofs.size(), co.kg, PII_FF_FP64,
+                              out),
+              who))
     return 0.0;
   return {out[0], out[1]};
 }
 
-// cpp/src/compute_intersection_volume.cu:558-626: the gradient plug-in on a
-// one-node grid (|k| < 1e-10 gives zeros without a device call)
-VectorXcd compute_Ak_gradient_cuda(const Geometry& geom, const Vector3d& k,
-                                   const std::shared_ptr<DoFParameterization>& dof,
-                                   const VectorXd& dofs, int /*blockSize*/) {
+// dA(k)/dtheta of a built-in DoF kind at one k-vector
+// (cpp/src/compute_intersection_volume.cu:558-626: zeros for |k| < 1e-10).
+VectorXcd ak_gradient_at(const Geometry& geom, const Vector3d& k, int kind, const VectorXd& dofs,
+                         const char* who) {
   const int nd = (int)dofs.size();
   VectorXcd zero(nd, {0.0, 0.0});
   pii_context* ctx = context();
@@ -586,19 +867,65 @@ VectorXcd compute_Ak_gradient_cuda(const Geometry& geom, const Vector3d& k,
   CallObjects co;
   const double dir[3] = {k[0] / km, k[1] / km, k[2] / km};
   const double w = 1.0;
-  if (!report(pii_kgrid_create(ctx, 1, dir, &km, &w, &co.kg), "compute_Ak_gradient_cuda"))
-    return zero;
+  if (!report(pii_kgrid_create(ctx, 1, dir, &km, &w, &co.kg), who)) return zero;
   pii_geometry* g = upload(ctx, geom);
   if (!g) return zero;
   co.geoms.push_back(g);
   std::vector<double> out(2 * (size_t)nd);
-  if (!report(pii_form_factor_jacobian(ctx, g, dof ? dof->dof_kind() : PII_DOF_AFFINE,
-                                       dofs.data(), nd, co.kg, out.data()),
-              "compute_Ak_gradient_cuda"))
+  if (!report(pii_form_factor_jacobian(ctx, g, kind, dofs.data(), nd, co.kg, out.data()), who))
     return zero;
   VectorXcd grad(nd);
   for (int t = 0; t < nd; ++t) grad[t] = {out[2 * t], out[2 * t + 1]};
   return grad;
+}
+
+}  // namespace detail
+
+std::complex<double> AffineDoF::compute_A(const Geometry& geom, const Vector3d& k,
+                                          const VectorXd& dofs) const {
+  return detail::ak_at(geom, k, PII_DOF_AFFINE, dofs, "AffineDoF::compute_A");
+}
+VectorXcd AffineDoF::compute_A_gradient(const Geometry& geom, const Vector3d& k,
+                                        const VectorXd& dofs) const {
+  return detail::ak_gradient_at(geom, k, PII_DOF_AFFINE, dofs, "AffineDoF::compute_A_gradient");
+}
+std::complex<double> TriangleMeshDoF::compute_A(const Geometry& geom, const Vector3d& k,
+                                                const VectorXd& dofs) const {
+  return detail::ak_at(geom, k, PII_DOF_TRIMESH, dofs, "TriangleMeshDoF::compute_A");
+}
+VectorXcd TriangleMeshDoF::compute_A_gradient(const Geometry& geom, const Vector3d& k,
+                                              const VectorXd& dofs) const {
+  return detail::ak_gradient_at(geom, k, PII_DOF_TRIMESH, dofs,
+                                "TriangleMeshDoF::compute_A_gradient");
+}
+
+// cpp/src/compute_intersection_volume.cu:495-556: through the registry entry
+// of the DoF type (a caller's plug-in on a one-node k-grid)
+std::complex<double> compute_Ak_cuda(const Geometry& geom, const Vector3d& k,
+                                     const std::shared_ptr<DoFParameterization>& dof,
+                                     const VectorXd& dofs, int blockSize) {
+  const int kind = dof ? dof->dof_kind() : PII_DOF_AFFINE;
+  if (dof && !detail::builtin_entry(get_dof_type_name(dof), geom.type)) {
+    std::vector<std::complex<double>> a, g;
+    if (!detail::run_plugin(geom, detail::one_node_grid(k), dof, dofs, blockSize, false, a, g))
+      return 0.0;
+    return a[0];
+  }
+  return detail::ak_at(geom, k, kind, dofs, "compute_Ak_cuda");
+}
+
+VectorXcd compute_Ak_gradient_cuda(const Geometry& geom, const Vector3d& k,
+                                   const std::shared_ptr<DoFParameterization>& dof,
+                                   const VectorXd& dofs, int blockSize) {
+  const int kind = dof ? dof->dof_kind() : PII_DOF_AFFINE;
+  if (dof && !detail::builtin_entry(get_dof_type_name(dof), geom.type)) {
+    const int nd = (int)dofs.size();
+    std::vector<std::complex<double>> a, g;
+    if (!detail::run_plugin(geom, detail::one_node_grid(k), dof, dofs, blockSize, true, a, g))
+      return VectorXcd(nd, {0.0, 0.0});
+    return VectorXcd(g.begin(), g.end());
+  }
+  return detail::ak_gradient_at(geom, k, kind, dofs, "compute_Ak_gradient_cuda");
 }
 
 // cpp/src/compute_volume.cpp:61-73 with cpp/src/dof/affine_dof.cpp:324-384 (identity DoFs)
@@ -626,6 +953,100 @@ double AffineDoF::compute_volume(const Geometry& g, const VectorXd& dofs) const 
   const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
                      A[2] * (A[3] * A[7] - A[4] * A[6]);
   return compute_volume_cpu(g) * std::fabs(det);
+}
+
+VectorXd AffineDoF::compute_volume_gradient(const Geometry& g, const VectorXd& dofs) const {
+  VectorXd grad(12, 0.0);
+  if (dofs.size() != 12) {
+    std::cerr << "AffineDoF::compute_volume_gradient: expected 12 DoFs, got " << dofs.size()
+              << std::endl;
+    return grad;
+  }
+  const double* A = dofs.data() + 3;  // row-major
+  // cofactors C_rc = d det A / d A_rc
+  const double C[9] = {A[4] * A[8] - A[5] * A[7], A[5] * A[6] - A[3] * A[8],
+                       A[3] * A[7] - A[4] * A[6], A[2] * A[7] - A[1] * A[8],
+                       A[0] * A[8] - A[2] * A[6], A[1] * A[6] - A[0] * A[7],
+                       A[1] * A[5] - A[2] * A[4], A[2] * A[3] - A[0] * A[5],
+                       A[0] * A[4] - A[1] * A[3]};
+  const double det = A[0] * C[0] + A[1] * C[1] + A[2] * C[2];
+  const double sign = det >= 0.0 ? 1.0 : -1.0;
+  const double v_ref = compute_volume_cpu(g);
+  for (int i = 0; i < 9; ++i) grad[3 + i] = sign * C[i] * v_ref;
+  return grad;
+}
+
+double TriangleMeshDoF::compute_volume(const Geometry& g, const VectorXd& dofs) const {
+  if (g.type != GEOM_TRIANGLE || (int)dofs.size() != num_dofs()) {
+    std::cerr << "TriangleMeshDoF::compute_volume: needs a triangle mesh and " << num_dofs()
+              << " DoFs" << std::endl;
+    return 0.0;
+  }
+  const int nv = nv_;
+  double v = 0.0;
+  for (const TriPacked& t : g.tris) {
+    const int i = t.vid0, j = t.vid1, k = t.vid2;
+    if (i < 0 || j < 0 || k < 0 || i >= nv || j >= nv || k >= nv) continue;
+    const double* a = &dofs[3 * (size_t)i];
+    const double* b = &dofs[3 * (size_t)j];
+    const double* c = &dofs[3 * (size_t)k];
+    v += a[0] * (b[1] * c[2] - b[2] * c[1]) + a[1] * (b[2] * c[0] - b[0] * c[2]) +
+         a[2] * (b[0] * c[1] - b[1] * c[0]);
+  }
+  return v / 6.0;
+}
+
+VectorXd TriangleMeshDoF::compute_volume_gradient(const Geometry& g, const VectorXd& dofs) const {
+  VectorXd grad(num_dofs(), 0.0);
+  if (g.type != GEOM_TRIANGLE || (int)dofs.size() != num_dofs()) {
+    std::cerr << "TriangleMeshDoF::compute_volume_gradient: needs a triangle mesh and "
+              << num_dofs() << " DoFs" << std::endl;
+    return grad;
+  }
+  const int nv = nv_;
+  auto cross_into = [&](int o, const double* p, const double* q) {
+    grad[3 * (size_t)o] += p[1] * q[2] - p[2] * q[1];
+    grad[3 * (size_t)o + 1] += p[2] * q[0] - p[0] * q[2];
+    grad[3 * (size_t)o + 2] += p[0] * q[1] - p[1] * q[0];
+  };
+  for (const TriPacked& t : g.tris) {
+    const int i = t.vid0, j = t.vid1, k = t.vid2;
+    if (i < 0 || j < 0 || k < 0 || i >= nv || j >= nv || k >= nv) continue;
+    const double* a = &dofs[3 * (size_t)i];
+    const double* b = &dofs[3 * (size_t)j];
+    const double* c = &dofs[3 * (size_t)k];
+    cross_into(i, b, c);
+    cross_into(j, c, a);
+    cross_into(k, a, b);
+  }
+  for (double& x : grad) x /= 6.0;
+  return grad;
+}
+
+// cpp/src/compute_volume.cpp:15-59
+VolumeResult compute_volume_cpu(const Geometry& ref_geom,
+                                const std::shared_ptr<DoFParameterization>& dof,
+                                const VectorXd& dofs, ComputationFlags flags,
+                                bool enable_profiling) {
+  const auto t0 = std::chrono::high_resolution_clock::now();
+  VolumeResult r;
+  if (!dof) {
+    std::cerr << "compute_volume_cpu: NULL DoF parameterization" << std::endl;
+    return r;
+  }
+  if (needs_volume(flags)) r.volume = dof->compute_volume(ref_geom, dofs);
+  if (needs_gradient(flags)) r.grad = dof->compute_volume_gradient(ref_geom, dofs);
+  if (enable_profiling) {
+    const double ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::high_resolution_clock::now() - t0)
+                          .count();
+    std::cout << std::fixed << std::setprecision(3)
+              << "\n=== CPU Volume Computation (Unified Interface) ===\nGeometry: "
+              << ref_geom.num_elements() << " elements\nDoF type: " << dof->num_dofs()
+              << " DoFs\nFlags: " << static_cast<int>(flags) << "\n  Total time: " << ms
+              << " ms" << std::endl;
+  }
+  return r;
 }
 
 // cpp/src/analytical_intersection.cpp:15-60
