@@ -312,6 +312,27 @@ int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* ge
                           const pii_kgrid* kg, const double* dL_dV, double* const* grads,
                           double* V);
 
+/* Device-resident variants, no counterpart in the reference (whose drivers
+ * take host Eigen vectors): for callers whose DoFs and results live in
+ * device memory, a framework's tensors (pointint_amd.autograd).  dofs[i],
+ * V, dL_dV and grads[i] are device memory of the context's device (else
+ * PII_ERR_INVALID); the work is ordered after the caller's stream
+ * (pii_context_set_caller_stream) and finished on return.  DoF values stay
+ * on the device: the plan reads a device-computed bounding box of each
+ * vertex-DoF object and the 12 values of each affine DoF vector; dL_dV
+ * (n x n) is read to the host for the mixing matrix.  Gradients of vertex
+ * DoFs of indexed meshes and of affine DoFs are written on the device;
+ * meshes whose DoF vertices are shared ids (pii_geometry_create_tris) are
+ * summed on the host.  Results equal the host-pointer entries' (V bit for
+ * bit; affine gradients to rounding, their sums run in another order). */
+int pii_volume_matrix_device(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                             const int* dof_kinds, const double* const* dofs, const int* ndofs,
+                             const pii_kgrid* kg, int ff_precision, int gram_mode, double* V);
+int pii_volume_matrix_vjp_device(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                                 const int* dof_kinds, const double* const* dofs,
+                                 const int* ndofs, const pii_kgrid* kg, const double* dL_dV,
+                                 double* const* grads, double* V);
+
 /* Measurement helper, no counterpart in the reference: the device's
  * sustained VALU issue rate for the roofline figures of the benchmark.  kind
  * 0: FP64 FMA (flop/s), 1: FP32 FMA (flop/s, non-packed, as this library is

@@ -17,6 +17,7 @@ from .core import (  # noqa: F401
     compute_form_factor_into,
     compute_form_factor_jacobian,
     compute_volume_matrix,
+    compute_volume_matrix_device,
     gauss_legendre,
     gram_from_J,
     kgrid_from_rule,
@@ -27,6 +28,7 @@ from .core import (  # noqa: F401
     make_point_cloud,
     make_triangle_mesh,
     volume_matrix_vjp,
+    volume_matrix_vjp_device,
     volume_matrix_hessian,
     volume_matrix_derivatives,
 )

@@ -65,6 +65,8 @@ SIGNATURES = {
     "pii_geometry_destroy": ([_P], _I),
     "pii_geometry_info": ([_P, _P, _P, _P], _I),
     "pii_geometry_cache_moments": ([_P, _I, _P], _I),
+    "pii_volume_matrix_device": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _P], _I),
+    "pii_volume_matrix_vjp_device": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "pii_form_factor": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),
     "pii_form_factor_device": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),
     "pii_form_factor_device_noalloc": ([_P, _P, _I, _P, _I, _P, _I, _P], _I),

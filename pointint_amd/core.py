@@ -326,13 +326,72 @@ def compute_form_factor_into(out, geom: Geometry, kgrid: KGrid, dofs=None, dof_k
         raise ValueError("out must be a contiguous float64 CUDA tensor of 2 Q elements")
     kinds, dl = _dof_args([geom], None if dofs is None else [dofs], [dof_kind])
     # the library's writes follow what torch has queued on its current stream
-    import torch
-    N.check(N.load().pii_context_set_caller_stream(
-        geom.ctx.handle, C.c_void_p(torch.cuda.current_stream(out.device).cuda_stream)))
+    _set_caller_stream(geom.ctx, out.device)
     N.check(N.load().pii_form_factor_device(geom.ctx.handle, geom.handle, kinds[0],
                                             dl[0].ctypes.data, dl[0].shape[0], kgrid.handle,
                                             _FF[precision], out.data_ptr()))
     return out
+
+
+def _device_tensor(t, n: int, what: str):
+    if not (t.is_cuda and t.is_contiguous() and t.dtype.is_floating_point
+            and t.element_size() == 8 and t.numel() == n):
+        raise ValueError(f"{what} must be a contiguous float64 CUDA tensor of {n} elements")
+
+
+def _set_caller_stream(ctx: "Context", device) -> None:
+    import torch
+    N.check(N.load().pii_context_set_caller_stream(
+        ctx.handle, C.c_void_p(torch.cuda.current_stream(device).cuda_stream)))
+
+
+def compute_volume_matrix_device(V, geoms: Sequence[Geometry], kgrid: KGrid, dofs, dof_kinds,
+                                 ff: str = "fast", gram: str = "fp16x3"):
+    """V (n x n) into `V`, a contiguous float64 CUDA tensor, from DoF vectors
+    that are contiguous float64 CUDA tensors (pii_volume_matrix_device: the
+    DoF values stay on the device).  Returns `V`."""
+    n = len(geoms)
+    _device_tensor(V, n * n, "V")
+    kinds = [_DOF[k] for k in dof_kinds]
+    for d in dofs:
+        _device_tensor(d, d.numel(), "dofs")
+    ctx = geoms[0].ctx
+    _set_caller_stream(ctx, V.device)
+    gh = np.fromiter((g._hv for g in geoms), dtype=np.uintp, count=n)
+    kh = np.asarray(kinds, dtype=np.int32)
+    dh = np.fromiter((d.data_ptr() for d in dofs), dtype=np.uintp, count=n)
+    nh = np.fromiter((d.numel() for d in dofs), dtype=np.int32, count=n)
+    N.check(N.load().pii_volume_matrix_device(ctx.handle, n, gh.ctypes.data, kh.ctypes.data,
+                                              dh.ctypes.data, nh.ctypes.data, kgrid.handle,
+                                              _FF[ff], _GRAM[gram], V.data_ptr()))
+    return V
+
+
+def volume_matrix_vjp_device(grads, geoms: Sequence[Geometry], kgrid: KGrid, dL_dV, dofs,
+                             dof_kinds, V=None):
+    """Gradients of L = sum_ij dL_dV[i, j] V_ij w.r.t. each object's DoFs into
+    `grads` (contiguous float64 CUDA tensors, one per object), with dL_dV, the
+    DoFs and the optional `V` also on the device (pii_volume_matrix_vjp_device)."""
+    n = len(geoms)
+    _device_tensor(dL_dV, n * n, "dL_dV")
+    if V is not None:
+        _device_tensor(V, n * n, "V")
+    kinds = [_DOF[k] for k in dof_kinds]
+    for d, g in zip(dofs, grads):
+        _device_tensor(d, d.numel(), "dofs")
+        _device_tensor(g, d.numel(), "grads")
+    ctx = geoms[0].ctx
+    _set_caller_stream(ctx, dL_dV.device)
+    gh = np.fromiter((g._hv for g in geoms), dtype=np.uintp, count=n)
+    kh = np.asarray(kinds, dtype=np.int32)
+    dh = np.fromiter((d.data_ptr() for d in dofs), dtype=np.uintp, count=n)
+    nh = np.fromiter((d.numel() for d in dofs), dtype=np.int32, count=n)
+    oh = np.fromiter((g.data_ptr() for g in grads), dtype=np.uintp, count=n)
+    N.check(N.load().pii_volume_matrix_vjp_device(
+        ctx.handle, n, gh.ctypes.data, kh.ctypes.data, dh.ctypes.data, nh.ctypes.data,
+        kgrid.handle, dL_dV.data_ptr(), oh.ctypes.data,
+        V.data_ptr() if V is not None else None))
+    return grads
 
 
 def compute_form_factor_jacobian(geom: Geometry, kgrid: KGrid, dofs=None,

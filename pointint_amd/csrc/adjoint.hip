@@ -190,4 +190,42 @@ void launch_adjoint_tri(hipStream_t s, const ObjDesc* d_objs, int nobj, int max_
                      d_objs, d_krep, Qh, d_Bw, ldB, kchunk, d_gworld, d_goff);
 }
 
+namespace {
+
+__global__ void __launch_bounds__(256) k_affine_dof_grads(const AffineGradJob* __restrict__ jobs) {
+  __shared__ double red[12][256];
+  const AffineGradJob jb = jobs[blockIdx.x];
+  const int tid = threadIdx.x;
+  double acc[12];
+#pragma unroll
+  for (int t = 0; t < 12; ++t) acc[t] = 0.0;
+  for (int v = tid; v < jb.nv; v += blockDim.x) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double g = jb.gw[3 * (size_t)v + r];
+      acc[r] += g;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        acc[3 + 3 * r + c] = fma(g, jb.verts[3 * (size_t)v + c], acc[3 + 3 * r + c]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 12; ++t) red[t][tid] = acc[t];
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (tid < h)
+#pragma unroll
+      for (int t = 0; t < 12; ++t) red[t][tid] += red[t][tid + h];
+    __syncthreads();
+  }
+  if (tid < 12) jb.out[tid] = red[tid][0];
+}
+
+}  // namespace
+
+void launch_affine_dof_grads(hipStream_t s, const AffineGradJob* d_jobs, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_affine_dof_grads, dim3(n), dim3(256), 0, s, d_jobs);
+}
+
 }  // namespace pii

@@ -247,7 +247,8 @@ static void world_bounds(const GeometryDev& g, int kind, const double* dofs, dou
     for (int v = 0; v < g.nv; ++v) {
       const double x = dofs[3 * (size_t)v] - c[0], y = dofs[3 * (size_t)v + 1] - c[1],
                    z = dofs[3 * (size_t)v + 2] - c[2];
-      r2 = std::max(r2, x * x + y * y + z * z);
+      // explicit FMAs: the device bounds kernel (k_dof_bounds) rounds alike
+      r2 = std::max(r2, std::fma(z, z, std::fma(y, y, x * x)));
     }
     R = std::sqrt(r2) * (1.0 + 1e-12);
     return;
@@ -270,12 +271,9 @@ static void world_bounds(const GeometryDev& g, int kind, const double* dofs, dou
   for (int v = 0; v < g.nv; ++v) {
     const int dv = g.h_dofvid[v];
     if (dv < 0) continue;
-    double d2 = 0.0;
-    for (int k = 0; k < 3; ++k) {
-      const double t = dofs[3 * (size_t)dv + k] - c[k];
-      d2 += t * t;
-    }
-    r2 = std::max(r2, d2);
+    const double x = dofs[3 * (size_t)dv] - c[0], y = dofs[3 * (size_t)dv + 1] - c[1],
+                 z = dofs[3 * (size_t)dv + 2] - c[2];
+    r2 = std::max(r2, std::fma(z, z, std::fma(y, y, x * x)));
   }
   R = std::sqrt(r2) * (1.0 + 1e-12);
 }
@@ -330,6 +328,9 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   N = n;
   NC = std::max(n, n_cap);
   for (int i = 0; i < N; ++i) validate_object(*geoms[i], kinds[i], ndofs[i]);
+  // host values of the DoFs for the plan (device-resident DoFs: host copies
+  // of the affine ones; vertex DoFs come with plan_bounds)
+  const double* const* hdofs = (device_dofs && plan_dofs) ? plan_dofs : dofs;
   // Moment series plan per object: leading shells with |k| R <= kSeriesReach,
   // when at least kSeriesMinShells of them (below that the per-element
   // kernels are cheaper than one pass over the elements per direction).
@@ -348,11 +349,16 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
       if (g.n == 0) continue;
       const bool tri = g.type == PII_GEOM_TRIANGLE;
       double R;
-      world_bounds(g, kinds[i], dofs[i], ctr_i[i].data(), R);
+      if (plan_bounds && kinds[i] != PII_DOF_AFFINE) {
+        for (int k = 0; k < 3; ++k) ctr_i[i][k] = plan_bounds[4 * (size_t)i + k];
+        R = plan_bounds[4 * (size_t)i + 3];
+      } else {
+        world_bounds(g, kinds[i], hdofs[i], ctr_i[i].data(), R);
+      }
       double cap2 = HUGE_VAL;  // bound on |k|^2 from the surfel factor
       if (!tri) {
         double an2 = 0.0;
-        for (int t = 3; t < 12; ++t) an2 += dofs[i][t] * dofs[i][t];
+        for (int t = 3; t < 12; ++t) an2 += hdofs[i][t] * hdofs[i][t];
         const double f = (g.type == PII_GEOM_DISK ? 0.25 : 0.5) * g.ref_smax * g.ref_smax * an2;
         if (f > 0.0) cap2 = (g.type == PII_GEOM_DISK ? 3.0 : 0.49) / f;
       }
@@ -400,8 +406,12 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   char* hs = w.stage.as<char>();
   double* h_dofs = reinterpret_cast<double*>(hs + dofs_at);
   for (int i = 0; i < N; ++i) {
-    std::memcpy(h_dofs + dof_off[i], dofs[i], sizeof(double) * ndofs[i]);
     const size_t end = (i + 1 < N) ? dof_off[i + 1] : total_dofs;
+    if (device_dofs) {  // copied device to device after the upload
+      std::fill(h_dofs + dof_off[i], h_dofs + end, 0.0);
+      continue;
+    }
+    std::memcpy(h_dofs + dof_off[i], dofs[i], sizeof(double) * ndofs[i]);
     std::fill(h_dofs + dof_off[i] + ndofs[i], h_dofs + end, 0.0);
   }
   double* d_dofs = reinterpret_cast<double*>(static_cast<char*>(w.misc.p) + dofs_at);
@@ -438,7 +448,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     kser[slot] = kser_i[i];
     ctr[slot] = ctr_i[i];
     if (use_sh && nser[slot] > 0 && geoms[i]->type == PII_GEOM_TRIANGLE && geoms[i]->shc.p &&
-        kinds[i] == PII_DOF_AFFINE && similarity_scale(dofs[i] + 3, shs[slot])) {
+        kinds[i] == PII_DOF_AFFINE && similarity_scale(hdofs[i] + 3, shs[slot])) {
       ++nsh;
       continue;  // no moment records
     }
@@ -485,7 +495,7 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
                          kinds[i] == PII_DOF_AFFINE;
     for (int k = 0; k < 3; ++k) d.rctr[k] = aff_tri ? g.ref_c[k] : d.ctr[k];
     if (aff_tri) {
-      const double* A = dofs[i] + 3;
+      const double* A = hdofs[i] + 3;
       const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
                          A[2] * (A[3] * A[7] - A[4] * A[6]);
       double fro2 = 0.0;
@@ -528,6 +538,10 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
   total_e = h_eprefix[N];
   if (!enqueue_now) return;
   PII_HIP(hipMemcpyAsync(w.misc.p, hs, stage_bytes, hipMemcpyHostToDevice, s));
+  if (device_dofs)
+    for (int i = 0; i < N; ++i)
+      PII_HIP(hipMemcpyAsync(d_dofs + dof_off[i], dofs[i], sizeof(double) * ndofs[i],
+                             hipMemcpyDeviceToDevice, s));
   if (prof) {
     prof->begin_call();
     prof->phase_begin(0);
@@ -555,8 +569,8 @@ void volume_matrix_fp64(Context& c, const Batch& b, const KGridDev& kg, double* 
                    c.w.part.as<double>(), nsplit, c.w.V.as<double>());
   c.phase_end(2);
   PII_HIP(hipGetLastError());
-  PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N,
-                         hipMemcpyDeviceToHost, s));
+  // V_host may be device memory (pii_volume_matrix_device)
+  PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N, hipMemcpyDefault, s));
   PII_HIP(hipStreamSynchronize(s));
 }
 
@@ -1011,7 +1025,55 @@ void require_device_pointer(const void* p, int device, const char* who) {
   const hipError_t e = hipPointerGetAttributes(&a, p);
   if (e != hipSuccess) (void)hipGetLastError();
   if (e != hipSuccess || a.type != hipMemoryTypeDevice || a.device != device)
-    throw Error(PII_ERR_INVALID, std::string(who) + ": out is not device memory of the context's device");
+    throw Error(PII_ERR_INVALID,
+                std::string(who) + ": not device memory of the context's device");
+}
+
+// Plan inputs of a batch whose DoFs are in device memory: host copies of the
+// affine DoF vectors (12 doubles each) and, for vertex-DoF objects on the
+// fast path, the series centre and radius from a device bounds kernel, as
+// world_bounds computes them from host values.
+void device_dof_plan(Context& c, int n, const GeometryDev* const* gs, const int* kinds,
+                     const double* const* d_dofs, const int* ndofs, bool fast,
+                     std::vector<std::vector<double>>& host_copies,
+                     std::vector<const double*>& plan_dofs, std::vector<double>& bounds) {
+  host_copies.assign(n, {});
+  plan_dofs.assign(n, nullptr);
+  bounds.assign(4 * (size_t)n, 0.0);
+  std::vector<DofBoundsJob> jobs;
+  std::vector<int> job_obj;
+  for (int i = 0; i < n; ++i) {
+    if (kinds[i] == PII_DOF_AFFINE) {
+      host_copies[i].resize(12);
+      PII_HIP(hipMemcpyAsync(host_copies[i].data(), d_dofs[i], sizeof(double) * 12,
+                             hipMemcpyDeviceToHost, c.stream));
+      plan_dofs[i] = host_copies[i].data();
+    } else if (fast && gs[i]->type == PII_GEOM_TRIANGLE) {
+      jobs.push_back({gs[i]->dofvid.as<int32_t>(), d_dofs[i], gs[i]->nv, 0});
+      job_obj.push_back(i);
+    }
+  }
+  std::vector<double> out(5 * jobs.size());
+  if (!jobs.empty()) {
+    c.w.C.ensure(sizeof(DofBoundsJob) * jobs.size() + sizeof(double) * out.size() + 256);
+    char* base = static_cast<char*>(c.w.C.p);
+    double* d_out = reinterpret_cast<double*>(base + align_up(sizeof(DofBoundsJob) * jobs.size(), 256));
+    PII_HIP(hipMemcpyAsync(base, jobs.data(), sizeof(DofBoundsJob) * jobs.size(),
+                           hipMemcpyHostToDevice, c.stream));
+    launch_dof_bounds(c.stream, reinterpret_cast<const DofBoundsJob*>(base), (int)jobs.size(),
+                      d_out);
+    PII_HIP(hipGetLastError());
+    PII_HIP(hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost,
+                           c.stream));
+  }
+  PII_HIP(hipStreamSynchronize(c.stream));
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const int i = job_obj[j];
+    const double* o = &out[5 * j];
+    for (int k = 0; k < 3; ++k) bounds[4 * (size_t)i + k] = o[k];
+    bounds[4 * (size_t)i + 3] = o[4] > 0.0 ? std::sqrt(o[3]) * (1.0 + 1e-12) : 0.0;
+  }
+  (void)ndofs;
 }
 
 }  // namespace pii
@@ -1190,6 +1252,85 @@ int pii_volume_matrix_vjp(pii_context* ctx, int n, const pii_geometry* const* ge
     if (n > 0) zero_fill(V, (long long)n * n);
     if (grads && dof_total(n, ndofs) >= 0)
       for (int i = 0; i < n; ++i) zero_fill(grads[i], ndofs[i]);
+  });
+}
+
+int pii_volume_matrix_device(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                             const int* dof_kinds, const double* const* dofs, const int* ndofs,
+                             const pii_kgrid* kg, int ff_precision, int gram_mode, double* V) {
+  return guarded([&] {
+    require(ctx && kg && V, "pii_volume_matrix_device: NULL argument");
+    require(n > 0 && geoms && dof_kinds && dofs && ndofs, "pii_volume_matrix_device: bad arrays");
+    for (int i = 0; i < n; ++i)
+      require(geoms[i] && dofs[i], "pii_volume_matrix_device: NULL object");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    require_device_pointer(V, ctx->c.device, "pii_volume_matrix_device: V");
+    for (int i = 0; i < n; ++i)
+      require_device_pointer(dofs[i], ctx->c.device, "pii_volume_matrix_device: dofs");
+    std::vector<const GeometryDev*> gs(n);
+    for (int i = 0; i < n; ++i) gs[i] = &geoms[i]->g;
+    for (int i = 0; i < n; ++i) validate_object(*gs[i], dof_kinds[i], ndofs[i]);
+    order_after_caller(ctx);
+    const bool fast = ff_precision == PII_FF_FAST;
+    std::vector<std::vector<double>> hc;
+    std::vector<const double*> pd;
+    std::vector<double> bounds;
+    device_dof_plan(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, fast, hc, pd, bounds);
+    Batch b;
+    b.device_dofs = true;
+    b.plan_dofs = pd.data();
+    b.plan_bounds = bounds.data();
+    if (ff_precision == PII_FF_FP64 && gram_mode == PII_GRAM_FP64) {
+      b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, false);
+      volume_matrix_fp64(ctx->c, b, kg->k, V);
+    } else if (fast) {
+      b.use_sh = true;
+      b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
+      volume_matrix_fast(ctx->c, b, kg->k, gram_mode, V);
+    } else {
+      throw Error(PII_ERR_INVALID, "unsupported precision / Gram mode combination");
+    }
+  });
+}
+
+int pii_volume_matrix_vjp_device(pii_context* ctx, int n, const pii_geometry* const* geoms,
+                                 const int* dof_kinds, const double* const* dofs,
+                                 const int* ndofs, const pii_kgrid* kg, const double* dL_dV,
+                                 double* const* grads, double* V) {
+  return guarded([&] {
+    require(ctx && kg && dL_dV && grads, "pii_volume_matrix_vjp_device: NULL argument");
+    require(n > 0 && geoms && dof_kinds && dofs && ndofs,
+            "pii_volume_matrix_vjp_device: bad arrays");
+    for (int i = 0; i < n; ++i)
+      require(geoms[i] && dofs[i] && grads[i], "pii_volume_matrix_vjp_device: NULL object");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    PII_HIP(hipSetDevice(ctx->c.device));
+    require_device_pointer(dL_dV, ctx->c.device, "pii_volume_matrix_vjp_device: dL_dV");
+    if (V) require_device_pointer(V, ctx->c.device, "pii_volume_matrix_vjp_device: V");
+    for (int i = 0; i < n; ++i) {
+      require_device_pointer(dofs[i], ctx->c.device, "pii_volume_matrix_vjp_device: dofs");
+      require_device_pointer(grads[i], ctx->c.device, "pii_volume_matrix_vjp_device: grads");
+    }
+    std::vector<const GeometryDev*> gs(n);
+    for (int i = 0; i < n; ++i) gs[i] = &geoms[i]->g;
+    for (int i = 0; i < n; ++i) validate_object(*gs[i], dof_kinds[i], ndofs[i]);
+    order_after_caller(ctx);
+    std::vector<std::vector<double>> hc;
+    std::vector<const double*> pd;
+    std::vector<double> bounds;
+    device_dof_plan(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, hc, pd, bounds);
+    // dL/dV (n x n) is small: it joins the host-built mixing matrix
+    std::vector<double> G((size_t)n * n);
+    PII_HIP(hipMemcpyAsync(G.data(), dL_dV, sizeof(double) * G.size(), hipMemcpyDeviceToHost,
+                           ctx->c.stream));
+    PII_HIP(hipStreamSynchronize(ctx->c.stream));
+    Batch b;
+    b.device_dofs = true;
+    b.plan_dofs = pd.data();
+    b.plan_bounds = bounds.data();
+    b.build(ctx->c, n, gs.data(), dof_kinds, dofs, ndofs, true, ctx->c.series_grid(kg->k));
+    volume_matrix_vjp(ctx->c, b, gs.data(), kg->k, G.data(), grads, V, true);
   });
 }
 

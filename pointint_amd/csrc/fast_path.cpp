@@ -170,8 +170,8 @@ void volume_matrix_fast(Context& c, const Batch& b, const KGridDev& kg, int gram
   }
   c.phase_end(2);
   PII_HIP(hipGetLastError());
-  PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost,
-                         s));
+  // V_host may be device memory (the device-output entries)
+  PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N, hipMemcpyDefault, s));
   PII_HIP(hipStreamSynchronize(s));
 }
 
@@ -448,7 +448,7 @@ void volume_matrix_hessian(Context& c, const Batch& b, const KGridDev& kg, int g
 
 void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geoms,
                        const KGridDev& kg, const double* G, double* const* grads,
-                       double* V_host) {
+                       double* V_host, bool grads_on_device) {
   hipStream_t s = c.stream;
   const int N = b.N, Qh = kg.Qh;
   const int ldX = (int)align_up((size_t)Qh, 64);
@@ -510,6 +510,58 @@ void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geo
                      c.w.B.as<float2>(), ldX, ksplit, d_gw, d_goff);
   c.phase_end(3);
   PII_HIP(hipGetLastError());
+  if (grads_on_device) {
+    // Gradients into device memory: vertex DoFs of indexed meshes are the
+    // world-vertex gradients themselves, affine DoFs their moments (a
+    // reduction kernel); vertex DoFs with shared ids are summed on the host.
+    std::vector<AffineGradJob> jobs;
+    bool host_map = false;
+    for (int slot = 0; slot < N; ++slot) {
+      const int i = b.order[slot];
+      const GeometryDev& g = *geoms[i];
+      const ObjDesc& d = b.h_objs[slot];
+      if (d.dof_kind == PII_DOF_TRIMESH && g.dof_identity)
+        PII_HIP(hipMemcpyAsync(grads[i], d_gw + goff[slot], sizeof(double) * 3 * (size_t)g.nv,
+                               hipMemcpyDeviceToDevice, s));
+      else if (d.dof_kind == PII_DOF_AFFINE)
+        jobs.push_back({d_gw + goff[slot], g.verts.as<double>(), g.nv, 0, grads[i]});
+      else
+        host_map = true;
+    }
+    if (!jobs.empty()) {
+      c.w.C.ensure(sizeof(AffineGradJob) * jobs.size());
+      PII_HIP(hipMemcpyAsync(c.w.C.p, jobs.data(), sizeof(AffineGradJob) * jobs.size(),
+                             hipMemcpyHostToDevice, s));
+      launch_affine_dof_grads(s, c.w.C.as<AffineGradJob>(), (int)jobs.size());
+      PII_HIP(hipGetLastError());
+    }
+    if (V_host)
+      PII_HIP(hipMemcpyAsync(V_host, c.w.V.p, sizeof(double) * (size_t)N * N, hipMemcpyDefault,
+                             s));
+    if (host_map) {
+      std::vector<double> gw((size_t)std::max(goff[N], 1)), out;
+      PII_HIP(hipMemcpyAsync(gw.data(), d_gw, sizeof(double) * goff[N], hipMemcpyDeviceToHost,
+                             s));
+      PII_HIP(hipStreamSynchronize(s));
+      for (int slot = 0; slot < N; ++slot) {
+        const int i = b.order[slot];
+        const GeometryDev& g = *geoms[i];
+        const ObjDesc& d = b.h_objs[slot];
+        if (d.dof_kind != PII_DOF_TRIMESH || g.dof_identity) continue;
+        out.assign(3 * (size_t)g.ndof_verts, 0.0);
+        for (int v = 0; v < g.nv; ++v) {
+          const int dv = g.h_dofvid[v];
+          if (dv < 0) continue;
+          for (int cc = 0; cc < 3; ++cc) out[3 * dv + cc] += gw[goff[slot] + 3 * v + cc];
+        }
+        PII_HIP(hipMemcpyAsync(grads[i], out.data(), sizeof(double) * out.size(),
+                               hipMemcpyHostToDevice, s));
+        PII_HIP(hipStreamSynchronize(s));
+      }
+    }
+    PII_HIP(hipStreamSynchronize(s));
+    return;
+  }
   // world-vertex gradients come back through pinned staging (one DMA copy)
   c.w.hout.ensure(sizeof(double) * std::max(goff[N], 1));
   const double* gw = c.w.hout.as<double>();

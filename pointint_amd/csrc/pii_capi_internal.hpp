@@ -77,6 +77,14 @@ struct Batch {
   // harmonic form of the series (nsh of them).
   bool use_sh = false;
   int nsh = 0;
+  // Set before build for device-resident DoFs: dofs[i] are device pointers
+  // of the context's device, copied device-to-device after the staging
+  // upload; the plan reads plan_dofs[i] (a host copy, affine objects) and
+  // plan_bounds[4 i ..] (series centre and radius, vertex-DoF objects)
+  // instead of the DoF values.
+  bool device_dofs = false;
+  const double* const* plan_dofs = nullptr;
+  const double* plan_bounds = nullptr;
   // prof (may be null) records the pre-pass phase with HIP events.  series_kg
   // (fast path only, may be null) enables the moment series on that grid.
   // cols (may be null: column i) gives object i's output column.  n_cap > n
@@ -154,6 +162,6 @@ void volume_matrix_affine_all(Context& c, const Batch& b, const KGridDev& kg, in
 // Vertex-DoF / affine-DoF gradients of L = sum_ij G_ij V_ij (triangles).
 void volume_matrix_vjp(Context& c, const Batch& b, const GeometryDev* const* geoms,
                        const KGridDev& kg, const double* G, double* const* grads,
-                       double* V_host);
+                       double* V_host, bool grads_on_device = false);
 
 }  // namespace pii

@@ -270,6 +270,27 @@ void launch_prepass(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix, 
 // for any batch laid out with N descriptors.
 void launch_prepass_fixed(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix,
                           const int* d_eprefix, int N, int fast, int grid);
+// Bounds of vertex-DoF objects whose DoFs are in device memory: per job,
+// out[5 j ..] = {centre, max |x - centre|^2, valid} over the DoF vertices the
+// geometry's vertices refer to (the host plan's world bounds).
+struct DofBoundsJob {
+  const int32_t* dofvid;   // nv entries: geometry vertex -> DoF vertex (or -1)
+  const double* dofs;      // 3 x DoF vertices, device memory
+  int nv;
+  int pad_;
+};
+void launch_dof_bounds(hipStream_t s, const DofBoundsJob* d_jobs, int n, double* d_out);
+// dL/dtheta of affine objects from their world-vertex gradients gW:
+// out[0..2] = sum_v gW_v, out[3 + 3 r + c] = sum_v gW_v,r V_v,c (reference
+// vertices V), one workgroup per job, fixed-order sums.
+struct AffineGradJob {
+  const double* gw;      // 3 nv world-vertex gradients
+  const double* verts;   // 3 nv reference vertices
+  int nv;
+  int pad_;
+  double* out;           // 12 doubles, device memory
+};
+void launch_affine_dof_grads(hipStream_t s, const AffineGradJob* d_jobs, int n);
 // Zeroes n 32-bit words (a kernel: graph-safe, unlike a memset node).
 void launch_clear_u32(hipStream_t s, unsigned* d_p, int n);
 // Writes the device's constant 100 MHz clock to d_out[i] when the stream

@@ -135,7 +135,73 @@ __global__ void k_stamp(unsigned long long* __restrict__ out, int i) {
   if (threadIdx.x == 0) out[i] = __builtin_amdgcn_s_memrealtime();
 }
 
+// Centre and squared radius of a vertex-DoF object's DoF vertices (those a
+// geometry vertex refers to), as the host plan computes them from host DoFs:
+// out[5 j ..] = {c (bounding-box centre), max |x - c|^2, valid}.  One
+// workgroup per object; min / max / max are exact, so the result does not
+// depend on the reduction order.
+__global__ void __launch_bounds__(256) k_dof_bounds(const DofBoundsJob* __restrict__ jobs,
+                                                    double* __restrict__ out) {
+  __shared__ double red[6][256];
+  const DofBoundsJob jb = jobs[blockIdx.x];
+  const int tid = threadIdx.x;
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int v = tid; v < jb.nv; v += blockDim.x) {
+    const int dv = jb.dofvid[v];
+    if (dv < 0) continue;
+    for (int k = 0; k < 3; ++k) {
+      const double x = jb.dofs[3 * (size_t)dv + k];
+      lo[k] = fmin(lo[k], x);
+      hi[k] = fmax(hi[k], x);
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    red[k][tid] = lo[k];
+    red[3 + k][tid] = hi[k];
+  }
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (tid < h)
+      for (int k = 0; k < 3; ++k) {
+        red[k][tid] = fmin(red[k][tid], red[k][tid + h]);
+        red[3 + k][tid] = fmax(red[3 + k][tid], red[3 + k][tid + h]);
+      }
+    __syncthreads();
+  }
+  const bool valid = red[0][0] <= red[3][0];
+  double c[3];
+  for (int k = 0; k < 3; ++k) c[k] = valid ? 0.5 * (red[k][0] + red[3 + k][0]) : 0.0;
+  __syncthreads();
+  double r2 = 0.0;
+  for (int v = tid; v < jb.nv; v += blockDim.x) {
+    const int dv = jb.dofvid[v];
+    if (dv < 0) continue;
+    const double x = jb.dofs[3 * (size_t)dv] - c[0], y = jb.dofs[3 * (size_t)dv + 1] - c[1],
+                 z = jb.dofs[3 * (size_t)dv + 2] - c[2];
+    r2 = fmax(r2, fma(z, z, fma(y, y, x * x)));
+  }
+  red[0][tid] = r2;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (tid < h) red[0][tid] = fmax(red[0][tid], red[0][tid + h]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double* o = out + 5 * (size_t)blockIdx.x;
+    o[0] = c[0];
+    o[1] = c[1];
+    o[2] = c[2];
+    o[3] = valid ? red[0][0] : 0.0;
+    o[4] = valid ? 1.0 : 0.0;
+  }
+}
+
 }  // namespace
+
+void launch_dof_bounds(hipStream_t s, const DofBoundsJob* d_jobs, int n, double* d_out) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_dof_bounds, dim3(n), dim3(256), 0, s, d_jobs, d_out);
+}
 
 void launch_prepass_fixed(hipStream_t s, const ObjDesc* d_objs, const int* d_vprefix,
                           const int* d_eprefix, int N, int fast, int grid) {
