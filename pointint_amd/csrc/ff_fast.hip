@@ -15,12 +15,10 @@
 //   g[p, m, q] = -i Q_m (Q_q sinc(h_q) - Q_p sinc(h_p)) / (u_q - u_p),
 //   h_p = (u_m - u_p)/2,  h_q = (u_q - u_m)/2,
 //
-// and sin(h) = Im(Q_b conj Q_a) comes from the phasors themselves.  When the
-// span u_q - u_p is small the divided difference is summed from its Taylor
-// series in FP64:  Phi(a, b) = sum_n 2 i^n h_n(a, b) / (n+2)!, h_n the complete
-// homogeneous symmetric polynomial: for a whole face x wave when a per-face
-// span bound puts every lane below kSpanTaylor (order from the bound), else
-// per lane below kLaneTaylor.
+// and sin(h) = Im(Q_b conj Q_a) comes from the phasors themselves.  On a lane
+// whose span u_q - u_p is below kLaneTaylor the divided difference is summed
+// from its Taylor series in FP64 instead:  Phi(a, b) = sum_n 2 i^n h_n(a, b) /
+// (n+2)!, h_n the complete homogeneous symmetric polynomial.
 //
 // A workgroup (4 waves) owns 64 k-nodes (one per lane, same nodes in every
 // wave) of one object.  Faces are grouped in patches of <= 64 vertices; per
@@ -37,11 +35,12 @@ namespace {
 
 constexpr int kLanes = 64;
 constexpr int kWaves = 4;
-constexpr double kSpanTaylor = 1.0;   // Taylor branch below this span (radians)
-// Inside a face x wave whose span bound exceeds kSpanTaylor, single lanes with
-// a span below kLaneTaylor take the Taylor sum (7 terms suffice there); the
-// others, spans >= 0.3, take the main formula, whose FP32 phasor error over
-// the span stays below 1e-6 of the term.
+// Lanes with a span below kLaneTaylor take the Taylor sum (7 terms suffice
+// there); the others, spans >= 0.3, take the main formula, whose FP32 phasor
+// error over the span stays below 1e-6 of the term.  (A wave-uniform Taylor
+// path for faces whose whole wave has a small span, chosen per face from a
+// span bound in the record, cost 8-10 % on its compare-and-branch alone at
+// the high |k| this kernel now serves, where it almost never applies.)
 constexpr float kLaneTaylor = 0.3f;
 constexpr float kSincPoly = 0.5f;     // sinc(h) by polynomial below |h| = 0.5
 
@@ -123,28 +122,29 @@ __device__ __forceinline__ void tri_taylor(double al, double be, float g2, float
 }
 
 // Main branch: middle node m and the other two (p, q); see the file comment.
-// The formula is symmetric in p and q, so only the middle node matters: it
-// is node 0 when a and b differ in sign, else the one of nodes 1, 2 whose
-// phase is smaller in magnitude.  u_m is the median of (0, a, b)
-// (v_med3); Q_p = Q1 when m = 0 (else Q0), Q_q = Q1 when m = 2 (else Q2),
-// each read from the phasor table at its vertex's slot (s0, s1, s2).  Half spans and the span are formed in FP32 from
-// the FP32-rounded alpha, beta: their absolute error, 6e-8 |k| diam, changes
-// the result by a relative 6e-8 at most, because the span D itself is at
-// least that large here.
-__device__ __forceinline__ void tri_main(double al, double be, float g2, int s0, int s1, int s2,
-                                         const float2 (*tab)[kLanes], int lane, float2& acc) {
+// The formula is symmetric in p and q, so only the middle node matters: u_m
+// is the median of (0, a, b) (v_med3), so m is node 0 when that median is 0
+// (0 lies between a and b), else the one of nodes 1, 2 whose phase is
+// smaller in magnitude.  Q_p = Q1 when m = 0 (else Q0), Q_q = Q1 when m = 2
+// (else Q2), each read from the phasor table through its vertex's row
+// pointer (r0, r1, r2).  Half spans and the span are formed in FP32 from the
+// FP32-rounded alpha, beta: their absolute error, 6e-8 |k| diam, changes the
+// result by a relative 6e-8 at most, because the span D itself is at least
+// that large here.
+__device__ __forceinline__ void tri_main(double al, double be, float g2, const float2* r0,
+                                         const float2* r1, const float2* r2, float2& acc) {
   const float a = (float)al, b = (float)be;
-  const bool m0 = a * b <= 0.0f;
-  const bool m2 = (!m0) & (fabsf(b) <= fabsf(a));
   const float um = __builtin_amdgcn_fmed3f(0.0f, a, b);
+  const bool m0 = um == 0.0f;
+  const bool m2 = (!m0) & (fabsf(b) <= fabsf(a));
   const float up = m0 ? a : 0.0f;
   const float uq = m2 ? a : b;
   // the three phasors read in (p, m, q) order: the selects act on the
-  // vertices' table slots, not on the phasors
-  const int vm = m0 ? s0 : (m2 ? s2 : s1);
-  const int vp = m0 ? s1 : s0;
-  const int vq = m2 ? s1 : s2;
-  const float2 Qm = tab[vm][lane], Qp = tab[vp][lane], Qq = tab[vq][lane];
+  // LDS addresses, not on the phasors
+  const float2* rm = m0 ? r0 : (m2 ? r2 : r1);
+  const float2* rp = m0 ? r1 : r0;
+  const float2* rq = m2 ? r1 : r2;
+  const float2 Qm = *rm, Qp = *rp, Qq = *rq;
   const float wp = um - up;                        // 2 hp
   const float wq = uq - um;                        // 2 hq
   const float D = uq - up;
@@ -165,14 +165,15 @@ __device__ __forceinline__ void tri_main(double al, double be, float g2, int s0,
 // gamma * e^{i u0} * Phi(alpha, beta) with a per-lane branch on the span
 // max(0, a, b) - min(0, a, b) = max(|a|, |b|, |a - b|), taken in FP32 (both
 // sides of the threshold are accurate near it).  Lanes below kLaneTaylor
-// take the 7-term Taylor sum (first omitted term < 1e-8 there).
-__device__ __forceinline__ void tri_term(double al, double be, float g2, float2 Q0, int s0, int s1,
-                                         int s2, const float2 (*tab)[kLanes], int lane,
-                                         float2& acc) {
+// take the 7-term Taylor sum (first omitted term < 1e-8 there).  The two
+// branches keep separate accumulators, so neither copies the other's.
+__device__ __forceinline__ void tri_term(double al, double be, float g2, const float2* r0,
+                                         const float2* r1, const float2* r2, float2& tacc,
+                                         float2& macc) {
   const float a = (float)al, b = (float)be;
   const float H = fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(a - b));
-  if (H < kLaneTaylor) tri_taylor<7>(al, be, g2, Q0, acc);
-  else tri_main(al, be, g2, s0, s1, s2, tab, lane, acc);
+  if (H < kLaneTaylor) tri_taylor<7>(al, be, g2, *r0, tacc);
+  else tri_main(al, be, g2, r0, r1, r2, macc);
 }
 
 // J1(x)/x for x >= 4 in FP32 from the modulus and phase of J1 + i Y1:
@@ -237,22 +238,6 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
   double accx = 0.0, accy = 0.0;
 
   if (d.type == PII_GEOM_TRIANGLE) {
-    // Span bound per face: H <= |k| * 2^code (code in the record's top byte).
-    // cut[j] = largest code with kmax * 2^code below the Taylor thresholds
-    // 0.1, 0.25, 0.5, 1 for the 64 nodes of this workgroup.
-    double kmax = km;
-    for (int off = 32; off > 0; off >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, off));
-    // Taylor order per face from the span bound: first omitted term
-    // 2(N+1)H^N/(N+2)! < 1e-8 for N = 4, 5, 6, 7, 8, 9, 10 up to
-    // H = 0.029, 0.084, 0.175, 0.30, 0.46, 0.65, 0.85; N = 12 up to H = 1.
-    constexpr int kOrders = 8;
-    int cut[kOrders];
-    const double thr[kOrders] = {0.029, 0.084, 0.175, 0.30, 0.46, 0.65, 0.85, 1.0};
-#pragma unroll
-    for (int j = 0; j < kOrders; ++j) {
-      const int c = (kmax > 0.0) ? (int)floor(log2(thr[j] / kmax)) : 127;
-      cut[j] = __builtin_amdgcn_readfirstlane(min(127, max(-128, c)) + 64);
-    }
     const auto* hdr = as_const(d.patch_hdr);
     const auto* pverts = as_const(d.patch_verts);
     const auto* world = as_const(d.world);
@@ -269,7 +254,7 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
         tab[s][lane] = half_phasor(u);
       }
       __syncthreads();
-      float2 pacc = make_float2(0.f, 0.f);
+      float2 tacc = make_float2(0.f, 0.f), macc = make_float2(0.f, 0.f);
       for (int f = wave; f < fcnt; f += kWaves) {
         const auto* r = rec + (size_t)kRecFastTri * (fbeg + f);
         const double al = kx * r[0] + ky * r[1] + kz * r[2];
@@ -277,26 +262,12 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
         const auto* rf = reinterpret_cast<const PII_CONST_AS float*>(r + 6);
         const float g2 = hx * rf[0] + hy * rf[1] + hz * rf[2];
         const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
-        const int code = (int)(loc >> 24);
-        const int s0 = loc & 0xff, s1 = (loc >> 8) & 0xff, s2 = (loc >> 16) & 0xff;
-        const float2 Q0 = tab[s0][lane];
-        // code <= cut[j]  =>  H <= kmax * 2^(code-64) <= thr[j] for every lane
-        if (code <= cut[3]) {
-          if (code <= cut[0]) tri_taylor<4>(al, be, g2, Q0, pacc);
-          else if (code <= cut[1]) tri_taylor<5>(al, be, g2, Q0, pacc);
-          else if (code <= cut[2]) tri_taylor<6>(al, be, g2, Q0, pacc);
-          else tri_taylor<7>(al, be, g2, Q0, pacc);
-        } else if (code <= cut[7]) {
-          if (code <= cut[4]) tri_taylor<8>(al, be, g2, Q0, pacc);
-          else if (code <= cut[5]) tri_taylor<9>(al, be, g2, Q0, pacc);
-          else if (code <= cut[6]) tri_taylor<10>(al, be, g2, Q0, pacc);
-          else tri_taylor<11>(al, be, g2, Q0, pacc);
-        } else {
-          tri_term(al, be, g2, Q0, s0, s1, s2, tab, lane, pacc);
-        }
+        tri_term(al, be, g2, &tab[loc & 0xff][lane], &tab[(loc >> 8) & 0xff][lane],
+                 &tab[(loc >> 16) & 0xff][lane], tacc, macc);
       }
-      accx += 2.0 * (double)pacc.x;   // the terms' factor 2 (half-scale sums)
-      accy += 2.0 * (double)pacc.y;
+      // the terms' factor 2 (half-scale sums)
+      accx += 2.0 * ((double)tacc.x + (double)macc.x);
+      accy += 2.0 * ((double)tacc.y + (double)macc.y);
       __syncthreads();
     }
   } else {

@@ -80,17 +80,8 @@ __device__ __forceinline__ void element_record(const ObjDesc* __restrict__ objs,
       rf[0] = (float)(2.0 * sx);
       rf[1] = (float)(2.0 * sy);
       rf[2] = (float)(2.0 * sz);
-      // Span code: 2^(code-64) >= max(|e1|, |e2|, |e2-e1|), the triangle's
-      // diameter, which bounds the spread of k.v over its corners by |k| 2^(code-64)
-      // (diam is scaled up by 1e-12 so that rounding of log2 cannot undercut it).
-      const double l1 = e1x * e1x + e1y * e1y + e1z * e1z;
-      const double l2 = e2x * e2x + e2y * e2y + e2z * e2z;
-      const double l3 = (e2x - e1x) * (e2x - e1x) + (e2y - e1y) * (e2y - e1y) +
-                        (e2z - e1z) * (e2z - e1z);
-      const double diam = sqrt(fmax(l1, fmax(l2, l3)));
-      int code = 0;
-      if (diam > 0.0) code = min(127, max(-64, (int)ceil(log2(diam * (1.0 + 1e-12))))) + 64;
-      reinterpret_cast<uint32_t*>(rf)[3] = (d.patch_floc[e] & 0xffffffu) | ((uint32_t)code << 24);
+      // the corners' slots in the patch's phasor table, 8 bits each
+      reinterpret_cast<uint32_t*>(rf)[3] = d.patch_floc[e] & 0xffffffu;
       if (d.nser > 0 && d.mrec) {
         // corners relative to the series centre, and the area vector
         // (objects with cached harmonics have none)
