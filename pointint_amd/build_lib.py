@@ -41,9 +41,15 @@ def sources():
     return out
 
 
+# Per-file options: the per-face kernel keeps its matrix-core products in
+# VGPRs (no AGPR copies inside its face loop).
+PER_FILE = {"ff_fast.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src):
     obj = os.path.join(BUILD_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, "-x", "hip", *COMMON, "-c", src, "-o", obj]
+    cmd = [HIPCC, "-x", "hip", *COMMON, *PER_FILE.get(os.path.basename(src), []), "-c", src,
+           "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -48,10 +48,15 @@ struct ObjDesc {
 
 // Element record layouts written by the pre-pass.
 //   plain triangle : 12 doubles {a, e1, e2, S}            (world frame)
-//   fast triangle  :  8 doubles {e1, e2, (2S.xyz f32, loc u32)} in patch order
+//   fast triangle  : 10 doubles {e1, e2, (2S.xyz f32, loc u32), (o0, o1, o2, 0) u32}
+//                    in patch order; loc holds the corners' slots in the patch's
+//                    vertex table (8 bits each), o_j = slot_j * kWaveNodes the
+//                    offsets (in table entries) of those slots in a per-wave
+//                    table of kWaveNodes nodes (k_ff_fast)
 //   disk/Gaussian  :  8 doubles {c' = A c + t, A n, rho|sigma, area|w}
 constexpr int kRecPlainTri = 12;
-constexpr int kRecFastTri = 8;
+constexpr int kRecFastTri = 10;
+constexpr int kWaveNodes = 16;  // nodes per wave in k_ff_fast
 constexpr int kRecSurfel = 8;
 constexpr int kRecMoment = 12;
 
@@ -78,6 +83,12 @@ static_assert(kGammaStride >= 2 * kSeriesOrder + 1, "Gamma record too short");
 template <class T>
 __device__ __forceinline__ const PII_CONST_AS T* as_const(const T* p) {
   return (const PII_CONST_AS T*)(p);
+}
+// The same data indexed per lane: global (vector memory) loads.
+#define PII_GLOBAL_AS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const PII_GLOBAL_AS T* as_global(const T* p) {
+  return (const PII_GLOBAL_AS T*)(p);
 }
 
 // Workgroup -> (block, object) for a grid of gridDim.x blocks per object and

@@ -20,11 +20,15 @@
 // from its Taylor series in FP64 instead:  Phi(a, b) = sum_n 2 i^n h_n(a, b) /
 // (n+2)!, h_n the complete homogeneous symmetric polynomial.
 //
-// A workgroup (4 waves) owns 64 k-nodes (one per lane, same nodes in every
-// wave) of one object.  Faces are grouped in patches of <= 64 vertices; per
-// patch the waves first fill an LDS table of the vertices' half phasors for
-// the 64 nodes, then each wave streams a quarter of the patch's face records
-// (wave-uniform, scalar loads) and reads its three phasors from LDS.
+// A workgroup (4 waves) owns 64 k-nodes of one object; for triangles each
+// wave owns 16 of them and works alone (no barrier inside the patch loop).
+// Faces are grouped in patches of <= 64 vertices.  Per patch a wave fills its
+// own LDS table of the vertices' half phasors for its 16 nodes, then takes
+// the patch's faces 16 at a time: the dot products k.v (vertex phases) and
+// alpha = k.e1, beta = k.e2 (FP64) and gamma = khat.2S (FP32) come from the
+// matrix cores, 16 faces x 16 nodes per product with K = (x, y, z, pad); each
+// lane then evaluates 4 (face, node) terms, its node and one face of each
+// group of 4 faces, and reads its three phasors from the table.
 #include "device_common.hpp"
 #include "j1_modphase.hpp"
 #include "pii_internal.hpp"
@@ -35,6 +39,8 @@ namespace {
 
 constexpr int kLanes = 64;
 constexpr int kWaves = 4;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 // Lanes with a span below kLaneTaylor take the Taylor sum (7 terms suffice
 // there); the others, spans >= 0.3, take the main formula, whose FP32 phasor
 // error over the span stays below 1e-6 of the term.  (A wave-uniform Taylor
@@ -203,18 +209,64 @@ __device__ __forceinline__ float j1ox_large(float x) {
   return m * (y * sy) * __builtin_amdgcn_cosf(f);
 }
 
-__global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ objs,
+// Per-lane operands of one block of 16 faces for lane (fq, nl): component
+// min(fq, 2) of face nl's e1, e2 (rows of the FP64 products, whose element r
+// in lane (fq, nl) is row fq + 4 r; the fourth row of the B operand is 0), of
+// face (nl >> 2) + 4 (nl & 3)'s 2S (rows of the FP32 product, whose element
+// r is row 4 fq + r, which must hold face fq + 4 r), and the table offsets
+// of the faces fq + 4 r the lane evaluates.  Rows past the patch read
+// whatever follows it in the record buffer (padded by 32 records past the
+// last object); the kernel replaces their terms' inputs.
+// A value at a 32-bit byte offset from a uniform base (saddr + voffset form).
+template <class T>
+__device__ __forceinline__ T ldg(const PII_GLOBAL_AS char* base, uint32_t off) {
+  return *reinterpret_cast<const PII_GLOBAL_AS T*>(base + off);
+}
+
+struct FaceOffsets {
+  uint32_t a, g, o[4];  // byte offsets in a block's records
+  __device__ __forceinline__ FaceOffsets(int fq, int nl) {
+    constexpr uint32_t rb = kRecFastTri * sizeof(double);
+    const uint32_t c = (uint32_t)min(fq, 2);
+    a = (uint32_t)nl * rb + 8u * c;
+    g = (uint32_t)((nl >> 2) + 4 * (nl & 3)) * rb + 48u + 4u * c;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = (uint32_t)(4 * r + fq) * rb + 64u;
+  }
+};
+struct FaceBlock {
+  double e1, e2;
+  float sg;
+  uint32_t o[4][3];
+  __device__ __forceinline__ void load(const PII_GLOBAL_AS char* blk, const FaceOffsets& f) {
+    e1 = ldg<double>(blk, f.a);
+    e2 = ldg<double>(blk, f.a + 24);
+    sg = ldg<float>(blk, f.g);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      o[r][0] = ldg<uint32_t>(blk, f.o[r]);
+      o[r][1] = ldg<uint32_t>(blk, f.o[r] + 4);
+      o[r][2] = ldg<uint32_t>(blk, f.o[r] + 8);
+    }
+  }
+};
+
+// Five waves per SIMD: the table (32 KB per workgroup) allows five workgroups
+// per compute unit, and registers are held to that.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+k_ff_fast(const ObjDesc* __restrict__ objs,
                                                  const double* __restrict__ krep, int Qh,
                                                  int out_mode, double2* __restrict__ Aout,
                                                  int ldA, float2* __restrict__ Xout, int ldX,
                                                  unsigned* __restrict__ colmax,
                                                  double2* __restrict__ part, int ldP) {
-  // Vertex phasor table; after the last patch its first 4 KB hold the
-  // cross-wave reduction (every wave has passed the patch loop's final barrier).
-  __shared__ float2 tab[kPatchVerts][kLanes];
-  static_assert(sizeof(double2) * kWaves * kLanes <= sizeof(float2) * kPatchVerts * kLanes,
+  // Per-wave vertex phasor tables (triangles); after the patch loop its first
+  // 4 KB hold the cross-wave reduction (behind a barrier).
+  __shared__ float2 tab[kWaves][kPatchVerts][kWaveNodes];
+  static_assert(sizeof(double2) * kWaves * kLanes <= sizeof(tab),
                 "reduction buffer must fit in the phasor table");
-  double2 (*red)[kLanes] = reinterpret_cast<double2 (*)[kLanes]>(&tab[0][0]);
+  static_assert(kWaves * kWaveNodes == kLanes, "the waves' nodes make the block's");
+  double2 (*red)[kLanes] = reinterpret_cast<double2 (*)[kLanes]>(&tab[0][0][0]);
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
   const int lane = threadIdx.x & (kLanes - 1);
@@ -231,48 +283,104 @@ __global__ void __launch_bounds__(256) k_ff_fast(const ObjDesc* __restrict__ obj
     return;
   }
   const int qc = q < Qh ? q : Qh - 1;
-  const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
-  const float hx = (float)krep[3 * (size_t)Qh + qc], hy = (float)krep[4 * (size_t)Qh + qc],
-              hz = (float)krep[5 * (size_t)Qh + qc];
   const double km = krep[6 * (size_t)Qh + qc];
   double accx = 0.0, accy = 0.0;
 
   if (d.type == PII_GEOM_TRIANGLE) {
+    // Lane = (face group fq, node nl): node q0 + 16 wave + nl, and in every
+    // round the face fb + 4 r + fq of the current 16-face block.
+    const int fq = lane >> 4, nl = lane & (kWaveNodes - 1);
+    const int qt = min(q0 + wave * kWaveNodes + nl, Qh - 1);
+    // B operands of the products: component fq of this lane's node (0 in
+    // the fourth row, so the A operands' fourth column is free)
+    const double bk = fq < 3 ? krep[(size_t)fq * Qh + qt] : 0.0;
+    const float bh = fq < 3 ? (float)krep[(size_t)(3 + fq) * Qh + qt] : 0.0f;
+    const dbl4 z4 = {0.0, 0.0, 0.0, 0.0};
+    const flt4 z4f = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float2* tw = &tab[wave][0][nl];  // this lane's node column of the wave's table
+    const FaceOffsets fo(fq, nl);
     const auto* hdr = as_const(d.patch_hdr);
-    const auto* pverts = as_const(d.patch_verts);
-    const auto* world = as_const(d.world);
-    const auto* rec = as_const(d.rec);
+    const auto* pverts = reinterpret_cast<const PII_GLOBAL_AS char*>(as_global(d.patch_verts));
+    const auto* world = reinterpret_cast<const PII_GLOBAL_AS char*>(as_global(d.world));
+    const auto* rec = reinterpret_cast<const PII_GLOBAL_AS char*>(as_global(d.rec));
+    constexpr size_t kRecBytes = kRecFastTri * sizeof(double);
+    const uint32_t wc = 8u * (uint32_t)min(fq, 2);  // byte offset of the lane's component
     // patches [pb, pe) of this workgroup's split (all when not split)
     const int pchunk = (d.npatch + (int)gridDim.z - 1) / (int)gridDim.z;
     const int pb = (int)blockIdx.z * pchunk, pe = min(d.npatch, pb + pchunk);
+    // Loads run one step ahead: the next patch's vertex ids, and the next
+    // face block's operands (past a patch's last block: the next patch's
+    // first, since the patches' faces are consecutive in the records).
+    int gvn[4];
+    FaceBlock cur, nxt;
+    if (pb < pe) {
+      const int vbeg = hdr[4 * pb], vcnt = hdr[4 * pb + 1];
+#pragma unroll
+      for (int vb = 0; vb < 4; ++vb)
+        gvn[vb] = ldg<int>(pverts, 4u * (uint32_t)(vbeg + min(vb * 16 + nl, vcnt - 1)));
+      cur.load(rec + kRecBytes * hdr[4 * pb + 2], fo);
+    }
     for (int p = pb; p < pe; ++p) {
-      const int vbeg = hdr[4 * p], vcnt = hdr[4 * p + 1];
+      const int vcnt = hdr[4 * p + 1];
       const int fbeg = hdr[4 * p + 2], fcnt = hdr[4 * p + 3];
-      for (int s = wave; s < vcnt; s += kWaves) {
-        const int gv = pverts[vbeg + s];
-        const double u = kx * world[3 * gv] + ky * world[3 * gv + 1] + kz * world[3 * gv + 2];
-        tab[s][lane] = half_phasor(u);
+      // this patch's vertex coordinates, then the next patch's vertex ids
+      double wv[4];
+#pragma unroll
+      for (int vb = 0; vb < 4; ++vb) wv[vb] = ldg<double>(world, 24u * (uint32_t)gvn[vb] + wc);
+      if (p + 1 < pe) {
+        const int vbeg1 = hdr[4 * p + 4], vcnt1 = hdr[4 * p + 5];
+#pragma unroll
+        for (int vb = 0; vb < 4; ++vb)
+          gvn[vb] = ldg<int>(pverts, 4u * (uint32_t)(vbeg1 + min(vb * 16 + nl, vcnt1 - 1)));
       }
-      __syncthreads();
+      // Vertex phases t = u / (4 pi), u = k.v, 16 vertices x 16 nodes per
+      // product; lane (fq, nl) gets vertex 16 vb + fq + 4 r of node nl in
+      // element r.  The half phasor e^{i u/2} is e^{2 pi i t}.
+#pragma unroll
+      for (int vb = 0; vb < 4; ++vb) {
+        if (vb * 16 < vcnt) {
+          const dbl4 t = __builtin_amdgcn_mfma_f64_16x16x4f64(wv[vb] * 0.07957747154594767, bk,
+                                                              z4, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            tab[wave][vb * 16 + fq + 4 * r][nl] = phasor_rev_hw(t[r] - rint(t[r]));
+        }
+      }
       float2 tacc = make_float2(0.f, 0.f), macc = make_float2(0.f, 0.f);
-      for (int f = wave; f < fcnt; f += kWaves) {
-        const auto* r = rec + (size_t)kRecFastTri * (fbeg + f);
-        const double al = kx * r[0] + ky * r[1] + kz * r[2];
-        const double be = kx * r[3] + ky * r[4] + kz * r[5];
-        const auto* rf = reinterpret_cast<const PII_CONST_AS float*>(r + 6);
-        const float g2 = hx * rf[0] + hy * rf[1] + hz * rf[2];
-        const uint32_t loc = reinterpret_cast<const PII_CONST_AS uint32_t*>(rf)[3];
-        tri_term(al, be, g2, &tab[loc & 0xff][lane], &tab[(loc >> 8) & 0xff][lane],
-                 &tab[(loc >> 16) & 0xff][lane], tacc, macc);
+      for (int fb = 0; fb < fcnt; fb += 16) {
+        const int m = fcnt - fb;  // faces left in the patch (uniform)
+        nxt.load(rec + kRecBytes * (fbeg + (fb + 16 < fcnt ? fb + 16 : fcnt)), fo);
+        const dbl4 A1 = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.e1, bk, z4, 0, 0, 0);
+        const dbl4 A2 = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.e2, bk, z4, 0, 0, 0);
+        const flt4 G = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.sg, bh, z4f, 0, 0, 0);
+        // rounds with a face in the patch (uniform); in the patch's last
+        // block, lanes whose face lies past the patch (their inputs are
+        // whatever the records there hold) sit out
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (4 * r < m && 4 * r + fq < m)
+            tri_term(A1[r], A2[r], G[r], tw + cur.o[r][0], tw + cur.o[r][1], tw + cur.o[r][2],
+                     tacc, macc);
+        }
+        cur = nxt;
       }
       // the terms' factor 2 (half-scale sums)
       accx += 2.0 * ((double)tacc.x + (double)macc.x);
       accy += 2.0 * ((double)tacc.y + (double)macc.y);
-      __syncthreads();
     }
+    // node totals: sum the four face groups; lane l of wave w then keeps
+    // node l when l lies in w's 16 (the cross-wave sum below adds zeros)
+    accx += __shfl_xor(accx, 16);
+    accy += __shfl_xor(accy, 16);
+    accx += __shfl_xor(accx, 32);
+    accy += __shfl_xor(accy, 32);
+    if ((lane >> 4) != wave) { accx = 0.0; accy = 0.0; }
+    __syncthreads();  // every wave is done with its table (red aliases it)
   } else {
-    // Disks / Gaussians: A = S(|k'_perp|) (k'.n)/|k| e^{i k.c'}, k' = A^T k,
+    // Disks / Gaussians, one node per lane (q0 + lane) and a quarter of the
+    // elements per wave: A = S(|k'_perp|) (k'.n)/|k| e^{i k.c'}, k' = A^T k,
     // with c' and A n precomputed (cpp/src/dof/affine_dof.cpp:103-167).
+    const double kx = krep[qc], ky = krep[(size_t)Qh + qc], kz = krep[2 * (size_t)Qh + qc];
     const auto* p = as_const(d.dofs);
     const double kpx = p[3] * kx + p[6] * ky + p[9] * kz;
     const double kpy = p[4] * kx + p[7] * ky + p[10] * kz;

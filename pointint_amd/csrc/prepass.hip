@@ -80,8 +80,15 @@ __device__ __forceinline__ void element_record(const ObjDesc* __restrict__ objs,
       rf[0] = (float)(2.0 * sx);
       rf[1] = (float)(2.0 * sy);
       rf[2] = (float)(2.0 * sz);
-      // the corners' slots in the patch's phasor table, 8 bits each
-      reinterpret_cast<uint32_t*>(rf)[3] = d.patch_floc[e] & 0xffffffu;
+      // the corners' slots in the patch's phasor table, 8 bits each, and
+      // their offsets in a per-wave table of k_ff_fast
+      const uint32_t loc = d.patch_floc[e] & 0xffffffu;
+      uint32_t* ru = reinterpret_cast<uint32_t*>(rf);
+      ru[3] = loc;
+      ru[4] = (loc & 0xffu) * kWaveNodes;
+      ru[5] = ((loc >> 8) & 0xffu) * kWaveNodes;
+      ru[6] = ((loc >> 16) & 0xffu) * kWaveNodes;
+      ru[7] = 0u;
       if (d.nser > 0 && d.mrec) {
         // corners relative to the series centre, and the area vector
         // (objects with cached harmonics have none)
