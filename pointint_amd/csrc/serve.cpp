@@ -7,13 +7,18 @@
 // Admission.  A query is refused at once (PII_ERR_BUDGET) when its HBM
 // footprint does not fit the budget left by the queries admitted and not yet
 // finished, or -- with a latency target set -- when the predicted completion
-// (time since its arrival + the outstanding work ahead of it and its own,
+// (after the launched batches, then the queued work ahead of it, the 90th
+// percentile of the recent queries' work in place of its own, so that the
+// test does not favour small queries, and one batch's worth behind it,
 // priced by the measured throughput) would exceed the target.
 //
 // Batching.  A worker thread keeps up to kSlots batches in flight, each on its
 // own HIP stream with its own workspaces.  Whenever a slot is free it takes
 // the queued queries in arrival order, up to max_batch_objects objects and
-// kMaxQueries queries, and runs one fused pass over all their objects:
+// kMaxQueries queries (with a latency target, at most an eighth of the target
+// of device work: a query completes with its whole batch, and the admission
+// test counts that much work joining its batch behind it), and runs one fused
+// pass over all their objects:
 // pre-pass, moment series, per-face form factors, then one split-precision
 // Gram launch over every query's column range, and the copy of the matrices
 // to pinned host memory.  Queries that arrive while batches run join the next
@@ -63,7 +68,7 @@ struct Request {
   std::vector<const GeometryDev*> geoms;
   std::vector<double> dofs;   // n x 12
   size_t footprint = 0;
-  double work = 0;            // triangles: the unit of the latency model
+  double work = 0;            // the unit of the latency model: see kObjectWork
   double t_arrival = 0, t_submit = 0, t_launch = 0, t_done = 0;
   std::vector<double> V;
   int state = 0;              // 0 queued, 1 running, 2 done
@@ -72,7 +77,15 @@ struct Request {
 };
 
 constexpr int kSlots = 2;
-constexpr int kMaxQueries = 64;   // per batch: bounds the column padding between queries
+constexpr int kMaxQueries = 64;
+// Work of a query in the latency model: its triangles plus kObjectWork per
+// object, the per-object cost of the low-|k| series (cached harmonics or the
+// direct series of small meshes) in triangles of the per-face kernel, as
+// measured on the serving workload (Lebedev 1202 x L=64: 17 us per object
+// against 8.4 us per thousand triangles).
+constexpr double kObjectWork = 2000.0;
+// batches in the throughput estimate (ratio of sums)
+constexpr int kRateWindow = 16;   // per batch: bounds the column padding between queries
 constexpr int kStamps = 8;
 constexpr int kFixedGrid = 1024;  // workgroups of the grid-stride pre-pass
 constexpr int kGramGrid = 512;    // workgroups of the persistent Gram
@@ -121,12 +134,36 @@ struct Server {
   size_t reserved = 0;
   int max_batch_objects = 1024;
   double latency_target_ms = 0.0;     // 0: admission by memory only
-  // latency model: ms of device time per triangle of queued work, from the
-  // marginal completion time of each batch (exponential average)
+  // latency model: ms of device time per unit of work, from the marginal
+  // completion time of the recent batches
   double ms_per_work = 0.0;
   bool calibrated = false;
+  double rate_dt[kRateWindow] = {}, rate_work[kRateWindow] = {};
+  int rate_n = 0;
   double last_done = 0.0;
   double outstanding_work = 0.0;
+  // admitted work not yet in a batch, and the predicted time (engine clock)
+  // at which the device finishes every launched batch
+  double queued_work = 0.0;
+  double t_free = 0.0;
+  // the recent submitted queries' work and its 90th percentile
+  std::vector<double> recent_work;
+  int64_t n_noted = 0;
+  double work_q90 = 0.0;
+  void note_query_work(double w) {
+    constexpr size_t kRecent = 1024;
+    if (recent_work.size() < kRecent) recent_work.push_back(w);
+    else recent_work[(size_t)(n_noted % (int64_t)kRecent)] = w;
+    ++n_noted;
+    if (n_noted <= 64 || n_noted % 64 == 0) {
+      std::vector<double> v(recent_work);
+      const size_t k = (size_t)(0.9 * (double)(v.size() - 1));
+      std::nth_element(v.begin(), v.begin() + (std::ptrdiff_t)k, v.end());
+      work_q90 = v[k];
+    }
+  }
+  // device time of one batch at most, with a latency target: an eighth of it
+  double batch_ms() const { return 0.125 * latency_target_ms; }
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Request*> queue;
@@ -378,16 +415,27 @@ struct Server {
       units_tri_node += s.tri_node;
       units_face_dir += s.face_dir;
     }
-    // throughput: the time this batch added after the previous completion
+    // throughput: the time each batch added after the previous completion,
+    // over its work, as a ratio of sums over the last kRateWindow batches
     if (s.work > 0) {
       const double dt = t - std::max(s.t_launch, last_done);
-      const double sample = dt * 1e3 / s.work;
-      ms_per_work = calibrated ? 0.7 * ms_per_work + 0.3 * sample : sample;
+      rate_dt[rate_n % kRateWindow] = dt;
+      rate_work[rate_n % kRateWindow] = s.work;
+      ++rate_n;
+      double sdt = 0.0, sw = 0.0;
+      for (int i = 0; i < std::min(rate_n, kRateWindow); ++i) {
+        sdt += rate_dt[i];
+        sw += rate_work[i];
+      }
+      ms_per_work = sdt * 1e3 / sw;
       calibrated = true;
     }
     last_done = t;
     s.reqs.clear();
     s.busy = false;
+    bool any_busy = false;
+    for (const Slot& o : slots) any_busy = any_busy || o.busy;
+    if (!any_busy) t_free = t;  // the device is idle now
   }
 
   void fail(Slot& s, const std::string& msg) {
@@ -425,15 +473,26 @@ struct Server {
         if (s.busy || queue.empty()) continue;
         int nobj = 0;
         const double t = now_s();
+        // With a latency target, a batch holds at most batch_ms() of work: a
+        // query completes with its whole batch, so the work queued behind it
+        // that joins its batch delays it by at most that much.
+        const double wcap = (latency_target_ms > 0.0 && calibrated && ms_per_work > 0.0)
+                                ? batch_ms() / ms_per_work : HUGE_VAL;
+        const double start = std::max(t, t_free);
+        double bwork = 0.0;
         while (!queue.empty() && (int)s.reqs.size() < kMaxQueries) {
           Request* r = queue.front();
-          if (!s.reqs.empty() && nobj + r->n > max_batch_objects) break;
+          if (!s.reqs.empty() && (nobj + r->n > max_batch_objects || bwork + r->work > wcap))
+            break;
           queue.pop_front();
           s.reqs.push_back(r);
           r->state = 1;
           r->t_launch = t;
           nobj += r->n;
+          bwork += r->work;
+          queued_work -= r->work;
         }
+        t_free = calibrated ? start + bwork * ms_per_work * 1e-3 : t;
         ++n_batches;
         n_objects += nobj;
         s.busy = true;
@@ -576,7 +635,7 @@ int pii_server_submit_at(pii_server* srv, int n, const pii_geometry* const* geom
       if (geoms[i]->device != S.device)
         throw Error(PII_ERR_INVALID, "pii_server_submit: geometry on another device");
       r->geoms[i] = &geoms[i]->g;
-      r->work += (double)geoms[i]->g.n;
+      r->work += (double)geoms[i]->g.n + kObjectWork;
     }
     r->dofs.assign(affine_dofs, affine_dofs + 12 * (size_t)n);
     r->footprint = S.footprint(n, r->geoms);
@@ -591,9 +650,17 @@ int pii_server_submit_at(pii_server* srv, int n, const pii_geometry* const* geom
                                       " + " + std::to_string(r->footprint) + " > " +
                                       std::to_string(S.budget) + " bytes)");
     }
+    S.note_query_work(r->work);
     if (S.latency_target_ms > 0.0 && S.calibrated) {
-      const double waited = (r->t_submit - r->t_arrival) * 1e3;
-      const double predicted = waited + (S.outstanding_work + r->work) * S.ms_per_work;
+      // completion if the queue drains in order: after the launched batches
+      // and the work queued ahead of it, priced by the measured throughput,
+      // plus the work that may join its batch behind it.  Its own work is
+      // taken as the 90th percentile of the recent queries', so that the
+      // test does not depend on the query's size and the served mix of scene
+      // sizes is the offered one.
+      const double start = std::max(r->t_submit, S.t_free);
+      const double predicted = (start - r->t_arrival) * 1e3 +
+                               (S.queued_work + S.work_q90) * S.ms_per_work + S.batch_ms();
       if (predicted > S.latency_target_ms) {
         ++S.n_refused_lat;
         throw Error(PII_ERR_BUDGET, "query refused: predicted latency " +
@@ -604,6 +671,7 @@ int pii_server_submit_at(pii_server* srv, int n, const pii_geometry* const* geom
     ++S.n_admitted;
     S.reserved += r->footprint;
     S.outstanding_work += r->work;
+    S.queued_work += r->work;
     r->handles.resize(n);
     for (int i = 0; i < n; ++i) {
       r->handles[i] = const_cast<pii_geometry*>(geoms[i]);

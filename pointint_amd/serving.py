@@ -213,7 +213,7 @@ def poisson_load(server: Server, pool: Sequence[core.Geometry], rate: float, dur
             V, t = server.wait_times(qid, k)
             rep.latencies_ms.append((t[3] - t[0]) * 1e3)
             rep.queue_ms.append((t[2] - t[0]) * 1e3)
-            rep.records.append((ta, t[3] - t0, k, tris))
+            rep.records.append((ta, t[3] - t0, k, tris, t[2] - t0))
             rep.completed += 1
             rep.pairs_completed += k * (k - 1) // 2
             if keep > 0:
@@ -245,7 +245,7 @@ def poisson_load(server: Server, pool: Sequence[core.Geometry], rate: float, dur
         rep.submit_lag_ms.append(max(0.0, clock() - t_sched) * 1e3)
         if qid is None:
             rep.refused += 1
-            rep.records.append((ta, float("nan"), k, tris))
+            rep.records.append((ta, float("nan"), k, tris, float("nan")))
             continue
         rep.admitted += 1
         with lock:
