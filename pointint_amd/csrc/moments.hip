@@ -397,12 +397,13 @@ constexpr int kShChunk = 8;                       // shells per evaluation round
 // Degree l of order m for the sums of parity par (n = 2 i + par in H[i]; the
 // degrees of the other parity only advance the recurrence).
 template <int l>
-__device__ __forceinline__ void sh_degree_par(double (&H)[kShH], const double2* cb, int cm,
-                                              int m, int par, double x, double zr, double zi,
+__device__ __forceinline__ void sh_degree_par(double (&H)[kShH], const double2* cb,
+                                              const double2* ab, int cm, int m, int par,
+                                              double x, double zr, double zi, double seed,
                                               double& qa, double& qb) {
   if (l >= m) {
-    const double q =
-        l == m ? kShRec.seed[m] : kShRec.a[l][m] * fma(x, qa, -kShRec.b[l][m] * qb);
+    const double2 r = ab[l];  // (a_lm, b_lm)
+    const double q = l == m ? seed : r.x * fma(x, qa, -r.y * qb);
     qb = qa;
     qa = q;
     if (((l + 1) & 1) == par) {
@@ -416,7 +417,7 @@ __device__ __forceinline__ void sh_degree_par(double (&H)[kShH], const double2* 
       }
     }
   }
-  if constexpr (l < sh::kL) sh_degree_par<l + 1>(H, cb, cm, m, par, x, zr, zi, qa, qb);
+  if constexpr (l < sh::kL) sh_degree_par<l + 1>(H, cb, ab, cm, m, par, x, zr, zi, seed, qa, qb);
 }
 
 __global__ void __launch_bounds__(kShThreads) k_series_sh(
@@ -424,6 +425,7 @@ __global__ void __launch_bounds__(kShThreads) k_series_sh(
     int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
     unsigned* __restrict__ colmax) {
   __shared__ double2 cbuf[2][kShPairs0];
+  __shared__ double2 abuf[2][sh::kL + 1];  // recurrence factors (a_lm, b_lm) of the order
   __shared__ double eo[kShChunk][2][kShDirs];
   __shared__ double dcs[kShDirs];
   const BlockObj bo = xcd_block_map();
@@ -445,17 +447,25 @@ __global__ void __launch_bounds__(kShThreads) k_series_sh(
   double H[kShH];
 #pragma unroll
   for (int i = 0; i < kShH; ++i) H[i] = 0.0;
-  const double2* src = reinterpret_cast<const double2*>(d.shc);
+  // global (not flat) loads: a flat load in flight would make every LDS read
+  // of the sums below wait for all outstanding LDS reads
+  const auto* src = as_global(d.shc);
+  auto pair = [&](const PII_GLOBAL_AS double* g, int i) {
+    return make_double2(g[2 * i], g[2 * i + 1]);
+  };
   // staging of one order's pairs: every thread up to kShStage of them
-  double2 r0 = {}, r1 = {}, r2 = {}, r3 = {};
+  double2 r0 = {}, r1 = {}, r2 = {}, r3 = {}, rab = {};
   static_assert(kShStage == 4, "staging registers");
+  static_assert(sh::kL + 1 <= kShThreads, "one thread per recurrence factor");
   auto fetch = [&](int m) {
     const int last = kShLayout.C[sh::kL + 1] - kShLayout.C[m] - 1;
-    const double2* g = src + kShLayout.mstart[m];
-    r0 = g[min(tid, last)];
-    r1 = g[min(tid + kShThreads, last)];
-    r2 = g[min(tid + 2 * kShThreads, last)];
-    r3 = g[min(tid + 3 * kShThreads, last)];
+    const auto* g = src + 2 * (size_t)kShLayout.mstart[m];
+    r0 = pair(g, min(tid, last));
+    r1 = pair(g, min(tid + kShThreads, last));
+    r2 = pair(g, min(tid + 2 * kShThreads, last));
+    r3 = pair(g, min(tid + 3 * kShThreads, last));
+    const int l = min(tid, sh::kL);
+    rab = make_double2(kShRec.a[l][m], kShRec.b[l][m]);
   };
   auto put = [&](int b, int m) {
     const int np = kShLayout.C[sh::kL + 1] - kShLayout.C[m];
@@ -463,6 +473,7 @@ __global__ void __launch_bounds__(kShThreads) k_series_sh(
     if (tid + kShThreads < np) cbuf[b][tid + kShThreads] = r1;
     if (tid + 2 * kShThreads < np) cbuf[b][tid + 2 * kShThreads] = r2;
     if (tid + 3 * kShThreads < np) cbuf[b][tid + 3 * kShThreads] = r3;
+    if (tid <= sh::kL) abuf[b][tid] = rab;
   };
   fetch(0);
   put(0, 0);
@@ -472,7 +483,8 @@ __global__ void __launch_bounds__(kShThreads) k_series_sh(
     const int cur = m & 1;
     if (m < sh::kL) fetch(m + 1);
     double qa = 0.0, qb = 0.0;
-    sh_degree_par<0>(H, &cbuf[cur][0], kShLayout.C[m], m, par, x, zr, zi, qa, qb);
+    sh_degree_par<0>(H, &cbuf[cur][0], &abuf[cur][0], kShLayout.C[m], m, par, x, zr, zi,
+                     kShRec.seed[m], qa, qb);
     if (m < sh::kL) put(cur ^ 1, m + 1);
     __syncthreads();
     const double t = zr * px - zi * py;
