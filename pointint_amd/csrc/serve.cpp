@@ -9,8 +9,8 @@
 // finished, or -- with a latency target set -- when the predicted completion
 // (after the launched batches, then the queued work ahead of it, the 90th
 // percentile of the recent queries' work in place of its own, so that the
-// test does not favour small queries, and one batch's worth behind it,
-// priced by the measured throughput) would exceed the target.
+// test does not favour small queries, and the largest batches of the slots
+// in flight, priced by the measured throughput) would exceed the target.
 //
 // Batching.  A worker thread keeps up to kSlots batches in flight, each on its
 // own HIP stream with its own workspaces.  Whenever a slot is free it takes
@@ -654,13 +654,15 @@ int pii_server_submit_at(pii_server* srv, int n, const pii_geometry* const* geom
     if (S.latency_target_ms > 0.0 && S.calibrated) {
       // completion if the queue drains in order: after the launched batches
       // and the work queued ahead of it, priced by the measured throughput,
-      // plus the work that may join its batch behind it.  Its own work is
+      // plus the work that may join its batch behind it, its batch sharing
+      // the device with the other batches in flight (kSlots batches' worth).  Its own work is
       // taken as the 90th percentile of the recent queries', so that the
       // test does not depend on the query's size and the served mix of scene
       // sizes is the offered one.
       const double start = std::max(r->t_submit, S.t_free);
       const double predicted = (start - r->t_arrival) * 1e3 +
-                               (S.queued_work + S.work_q90) * S.ms_per_work + S.batch_ms();
+                               (S.queued_work + S.work_q90) * S.ms_per_work +
+                               kSlots * S.batch_ms();
       if (predicted > S.latency_target_ms) {
         ++S.n_refused_lat;
         throw Error(PII_ERR_BUDGET, "query refused: predicted latency " +
