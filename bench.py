@@ -54,16 +54,14 @@ FLOP_PER_SERIES_NODE = 2.0 * SERIES_ORDER + 12.0
 FLOP_PER_DISK_K = 83.0
 FLOP_PER_SURFEL_DIR = 15.0 + 5.0 * SERIES_ORDER
 PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 (= non-packed FP32) peak
-# HBM bytes per launch of the dominant kernel on the default workload, from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB device copy
-# (fetch x2.00, write x1.00); profiles/ff_fast_counters_icospheres256_L64x1202.md
-# holds the counts and commands.  Counters cannot be read inside this run.
-# HBM bytes per launch of the dominant kernel from FETCH_SIZE / WRITE_SIZE
-# counter passes (2 x fetch + write, KiB scaled by a 1 GiB copy), per
-# workload; the serving figure is the average over the batches of a run
-MEASURED_TRAFFIC = {("icospheres_L3_N256", "k_ff_fast"): 60.8e6,
-                    ("mixed_N1024", "k_ff_fast"): 1334.9e6,
-                    ("serving", "k_ff_fast"): 407.9e6}
+# HBM bytes per launch of the dominant kernel, per workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes (2 x fetch + write, KiB, the factors of a
+# 1 GiB device copy; the profiles/ summaries hold the counts and commands);
+# the serving figure is the average over the batches of a run.  Counters
+# cannot be read inside this run.
+MEASURED_TRAFFIC = {("icospheres_L3_N256", "k_ff_fast"): 40.5e6,
+                    ("mixed_N1024", "k_ff_fast"): 1230.9e6,
+                    ("serving", "k_ff_fast"): 114.4e6}
 PEAK_MFMA_F16_TFLOPS = 2500.0
 
 
