@@ -204,3 +204,57 @@ def test_zero_and_tiny_k_shells(oracle):
         e = np.linalg.norm(grads[i] - r) / np.linalg.norm(r)
         print(f"vjp {i}: {e:.2e}")
         assert e < 1e-5, (i, e)
+
+
+def test_partial_face_blocks_and_small_meshes(oracle, grid):
+    """The per-face kernel takes a patch's faces 16 at a time, four per lane
+    group and round; a patch's last block is partial and its lanes past the
+    patch read whatever records follow (the next object's, or the padding
+    past the last object's).  Meshes of 1..47 faces (one partial block, a
+    full block plus a tail, several blocks) in one batch, the smallest last,
+    under affine and vertex DoFs, against the oracle on every node."""
+    dirs, kmag, wq, kg = grid
+    rng = np.random.default_rng(17)
+    V3, F3 = W.icosphere(3)
+    geoms, ogeoms, dofs, kinds = [], [], [], []
+    for nf in (47, 33, 31, 17, 16, 15, 5, 3, 2, 1):
+        # a connected cap of nf faces of a perturbed sphere (an open mesh)
+        c = V3[F3].mean(axis=1)
+        order = np.argsort(np.linalg.norm(c - V3[F3[0]].mean(axis=0), axis=1))
+        F = F3[order[:nf]]
+        used, inv = np.unique(F, return_inverse=True)
+        V = W.star_perturb(V3[used], rng, 0.1) * rng.uniform(0.3, 0.5) + rng.uniform(-0.2, 0.2, 3)
+        F = inv.reshape(-1, 3).astype(np.int32)
+        geoms.append(P.make_triangle_mesh(V, F))
+        ogeoms.append(oracle.geom_mesh(V, F))
+        if nf % 2:
+            dofs.append(np.concatenate([rng.uniform(-0.1, 0.1, 3),
+                                        (W.random_rotation(rng)).ravel()]))
+            kinds.append("affine")
+        else:
+            dofs.append((V + rng.normal(scale=0.01, size=V.shape)).ravel())
+            kinds.append("trimesh")
+    odk = {"affine": oracle.DOF_AFFINE, "trimesh": oracle.DOF_MESH}
+    J64 = np.stack([oracle.compute_A_grid(og, dirs, kmag, x, odk[k])
+                    for og, x, k in zip(ogeoms, dofs, kinds)], axis=1)
+    for i, (g, x, k) in enumerate(zip(geoms, dofs, kinds)):
+        A = P.compute_form_factor(g, kg, x, k, precision="fast")
+        e = wrel(A, J64[:, i], wq)
+        # per node on the shells with |k| >= 1 (the per-face kernel's), against
+        # the largest |A| of the shell
+        M = len(wq) // 24
+        worst = 0.0
+        for s in range(24):
+            sl = slice(s * M, (s + 1) * M)
+            if kmag[s * M] < 1.0:
+                continue
+            ref = np.max(np.abs(J64[sl, i]))
+            if ref > 0:
+                worst = max(worst, float(np.max(np.abs(A[sl] - J64[sl, i])) / ref))
+        print(f"{len(ogeoms[i]['el'])} faces, {k}: weighted {e:.2e}, per node {worst:.2e}")
+        assert e < 1e-6, (i, k, e)
+        assert worst < 1e-6, (i, k, worst)
+    V64 = oracle.gram(J64, wq)
+    V = P.compute_volume_matrix(geoms, kg, dofs, kinds, ff="fast", gram="fp64")
+    d = np.sqrt(np.abs(np.outer(np.diag(V64), np.diag(V64))))
+    assert np.max(np.abs(V - V64) / d) < 1e-5
