@@ -655,10 +655,10 @@ int pii_server_submit_at(pii_server* srv, int n, const pii_geometry* const* geom
       // completion if the queue drains in order: after the launched batches
       // and the work queued ahead of it, priced by the measured throughput,
       // plus the work that may join its batch behind it, its batch sharing
-      // the device with the other batches in flight (kSlots batches' worth).  Its own work is
-      // taken as the 90th percentile of the recent queries', so that the
-      // test does not depend on the query's size and the served mix of scene
-      // sizes is the offered one.
+      // the device with the other batches in flight (kSlots batches' worth).
+      // Its own work is taken as the 90th percentile of the recent queries',
+      // so that the test does not depend on the query's size and the served
+      // mix of scene sizes is the offered one.
       const double start = std::max(r->t_submit, S.t_free);
       const double predicted = (start - r->t_arrival) * 1e3 +
                                (S.queued_work + S.work_q90) * S.ms_per_work +
