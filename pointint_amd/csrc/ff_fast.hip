@@ -1,6 +1,7 @@
-// Fast form factor for gfx950: FP64 dot products, phases, range reduction and
-// small-span Taylor branch; the hardware FP32 sine/cosine on the reduced
-// phases, FP32 for the complex combination; FP64 accumulation across patches.
+// Fast form factor for gfx950: FP64 dot products, phases and range reduction;
+// the hardware FP32 sine/cosine on the reduced phases, FP32 for the complex
+// combination and the small-span Taylor branch; FP64 accumulation across
+// patches.
 //
 // Triangles (meshes and triangle soups) use a cancellation-free form of the
 // reference's element transform (cpp/src/element_functions.cpp:63-70 with the
@@ -17,7 +18,7 @@
 //
 // and sin(h) = Im(Q_b conj Q_a) comes from the phasors themselves.  On a lane
 // whose span u_q - u_p is below kLaneTaylor the divided difference is summed
-// from its Taylor series in FP64 instead:  Phi(a, b) = sum_n 2 i^n h_n(a, b) /
+// from its Taylor series instead:  Phi(a, b) = sum_n 2 i^n h_n(a, b) /
 // (n+2)!, h_n the complete homogeneous symmetric polynomial.
 //
 // A workgroup (4 waves) owns 64 k-nodes of one object; for triangles each
@@ -102,27 +103,29 @@ __device__ __forceinline__ float half_sinc_from(float w, float sin_h) {
   return fabsf(w) < 2.0f * kSincPoly ? p : r;
 }
 
-// Phi(a, b) = sum_{n<NT} c_n h_n(a, b), c_n = 2 i^n / (n+2)!, in FP64, times
+// Phi(a, b) = sum_{n<NT} c_n h_n(a, b), c_n = 2 i^n / (n+2)!, in FP32 from
+// the FP32-rounded alpha, beta (every term is positive-weighted and the sum
+// is near 1 for spans below kLaneTaylor, so FP32 carries it to 1e-7), times
 // (g2/2) e^{i u0} with e^{i u0} = Q0^2, accumulated at half scale.
 // |h_n| <= (n+1) H^n for span H, so the truncation error is bounded by the
 // first omitted term 2(NT+1) H^NT / (NT+2)!.
 template <int NT>
-__device__ __forceinline__ void tri_taylor(double al, double be, float g2, float2 Q0,
-                                           float2& acc) {
-  constexpr double c[12] = {1.0, 1.0 / 3.0, -1.0 / 12.0, -1.0 / 60.0, 1.0 / 360.0,
-                            1.0 / 2520.0, -1.0 / 20160.0, -1.0 / 181440.0, 1.0 / 1814400.0,
-                            1.0 / 19958400.0, -1.0 / 239500800.0, -1.0 / 3113510400.0};
-  double an = 1.0, hn = 1.0, re = 1.0, im = 0.0;
+__device__ __forceinline__ void tri_taylor(float a, float b, float g2, float2 Q0, float2& acc) {
+  constexpr float c[12] = {1.0f, 1.0f / 3.0f, -1.0f / 12.0f, -1.0f / 60.0f, 1.0f / 360.0f,
+                           1.0f / 2520.0f, -1.0f / 20160.0f, -1.0f / 181440.0f,
+                           1.0f / 1814400.0f, 1.0f / 19958400.0f, -1.0f / 239500800.0f,
+                           -1.0f / 3113510400.0f};
+  float an = 1.0f, hn = 1.0f, re = 1.0f, im = 0.0f;
 #pragma unroll
   for (int n = 1; n < NT; ++n) {
-    an *= al;
-    hn = fma(be, hn, an);
-    if (n & 1) im = fma(hn, c[n], im);
-    else re = fma(hn, c[n], re);
+    an *= a;
+    hn = fmaf(b, hn, an);
+    if (n & 1) im = fmaf(hn, c[n], im);
+    else re = fmaf(hn, c[n], re);
   }
   const float px = Q0.x * Q0.x - Q0.y * Q0.y;
   const float py = 2.0f * Q0.x * Q0.y;
-  const float tr = (float)re * (0.25f * g2), ti = (float)im * (0.25f * g2);
+  const float tr = re * (0.25f * g2), ti = im * (0.25f * g2);
   acc.x = fmaf(px, tr, fmaf(-py, ti, acc.x));
   acc.y = fmaf(px, ti, fmaf(py, tr, acc.y));
 }
@@ -178,7 +181,7 @@ __device__ __forceinline__ void tri_term(double al, double be, float g2, const f
                                          float2& macc) {
   const float a = (float)al, b = (float)be;
   const float H = fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(a - b));
-  if (H < kLaneTaylor) tri_taylor<7>(al, be, g2, *r0, tacc);
+  if (H < kLaneTaylor) tri_taylor<7>(a, b, g2, *r0, tacc);
   else tri_main(al, be, g2, r0, r1, r2, macc);
 }
 
