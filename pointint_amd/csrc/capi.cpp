@@ -433,8 +433,8 @@ void Batch::build(Workspace& w, hipStream_t s, Context* prof, int n,
     total_rec += align_up((size_t)g.n * per, 32);
   }
   w.world.ensure(std::max<size_t>(total_w, 1) * sizeof(double));
-  // k_ff_fast reads up to 32 fast-triangle records past a patch's last face
-  w.faces.ensure((std::max<size_t>(total_rec, 1) + 32 * kRecFastTri) * sizeof(double));
+  // (k_ff_fast's block loads are range-checked at each object's last record)
+  w.faces.ensure(std::max<size_t>(total_rec, 1) * sizeof(double));
   std::vector<int> nser(N, 0), nord(N, 0);
   std::vector<double> kser(N, 0.0);
   std::vector<std::array<double, 3>> ctr(N, {0.0, 0.0, 0.0});

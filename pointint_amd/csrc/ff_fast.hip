@@ -217,9 +217,10 @@ __device__ __forceinline__ float j1ox_large(float x) {
 // in lane (fq, nl) is row fq + 4 r; the fourth row of the B operand is 0), of
 // face (nl >> 2) + 4 (nl & 3)'s 2S (rows of the FP32 product, whose element
 // r is row 4 fq + r, which must hold face fq + 4 r), and the table offsets
-// of the faces fq + 4 r the lane evaluates.  Rows past the patch read
-// whatever follows it in the record buffer (padded by 32 records past the
-// last object); the kernel replaces their terms' inputs.
+// of the faces fq + 4 r the lane evaluates.  The loads go through a buffer
+// resource over the block's records that ends at the object's last record:
+// rows past the patch read the next patch's records, rows past the object
+// read 0 (the hardware's range check); the kernel masks both rows' terms.
 // A value at a 32-bit byte offset from a uniform base (saddr + voffset form).
 template <class T>
 __device__ __forceinline__ T ldg(const PII_GLOBAL_AS char* base, uint32_t off) {
@@ -237,19 +238,37 @@ struct FaceOffsets {
     for (int r = 0; r < 4; ++r) o[r] = (uint32_t)(4 * r + fq) * rb + 64u;
   }
 };
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+// Buffer resource over records [first, min(first + 16, n)) of an object's
+// fast-triangle records rec (uniform; 0x00020000: the gfx9 raw-buffer format
+// word).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const double* rec, int first,
+                                                             int n) {
+  constexpr int kBlockBytes = 16 * kRecFastTri * (int)sizeof(double);
+  // every input is uniform; readfirstlane keeps the descriptor in SGPRs
+  const int rem = __builtin_amdgcn_readfirstlane(n - first);
+  const int bytes = rem >= 16 ? kBlockBytes : max(rem, 0) * kRecFastTri * (int)sizeof(double);
+  const uint64_t base = reinterpret_cast<uint64_t>(rec + (size_t)kRecFastTri * first);
+  const uint64_t ub = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
+                      (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(ub), (short)0, bytes,
+                                           0x00020000);
+}
 struct FaceBlock {
   double e1, e2;
   float sg;
   uint32_t o[4][3];
-  __device__ __forceinline__ void load(const PII_GLOBAL_AS char* blk, const FaceOffsets& f) {
-    e1 = ldg<double>(blk, f.a);
-    e2 = ldg<double>(blk, f.a + 24);
-    sg = ldg<float>(blk, f.g);
+  __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t rs, const FaceOffsets& f) {
+    e1 = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, f.a, 0, 0));
+    e2 = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, f.a, 24, 0));
+    sg = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, f.g, 0, 0));
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      o[r][0] = ldg<uint32_t>(blk, f.o[r]);
-      o[r][1] = ldg<uint32_t>(blk, f.o[r] + 4);
-      o[r][2] = ldg<uint32_t>(blk, f.o[r] + 8);
+      const u32x3 v =
+          __builtin_bit_cast(u32x3, __builtin_amdgcn_raw_buffer_load_b96(rs, f.o[r], 0, 0));
+      o[r][0] = v.x;
+      o[r][1] = v.y;
+      o[r][2] = v.z;
     }
   }
 };
@@ -286,7 +305,6 @@ k_ff_fast(const ObjDesc* __restrict__ objs,
     return;
   }
   const int qc = q < Qh ? q : Qh - 1;
-  const double km = krep[6 * (size_t)Qh + qc];
   double accx = 0.0, accy = 0.0;
 
   if (d.type == PII_GEOM_TRIANGLE) {
@@ -305,23 +323,22 @@ k_ff_fast(const ObjDesc* __restrict__ objs,
     const auto* hdr = as_const(d.patch_hdr);
     const auto* pverts = reinterpret_cast<const PII_GLOBAL_AS char*>(as_global(d.patch_verts));
     const auto* world = reinterpret_cast<const PII_GLOBAL_AS char*>(as_global(d.world));
-    const auto* rec = reinterpret_cast<const PII_GLOBAL_AS char*>(as_global(d.rec));
-    constexpr size_t kRecBytes = kRecFastTri * sizeof(double);
     const uint32_t wc = 8u * (uint32_t)min(fq, 2);  // byte offset of the lane's component
     // patches [pb, pe) of this workgroup's split (all when not split)
     const int pchunk = (d.npatch + (int)gridDim.z - 1) / (int)gridDim.z;
     const int pb = (int)blockIdx.z * pchunk, pe = min(d.npatch, pb + pchunk);
     // Loads run one step ahead: the next patch's vertex ids, and the next
     // face block's operands (past a patch's last block: the next patch's
-    // first, since the patches' faces are consecutive in the records).
+    // first, since the patches' faces are consecutive in the records).  The
+    // face blocks alternate between two sets of registers.
     int gvn[4];
-    FaceBlock cur, nxt;
+    FaceBlock fa, fz;
     if (pb < pe) {
       const int vbeg = hdr[4 * pb], vcnt = hdr[4 * pb + 1];
 #pragma unroll
       for (int vb = 0; vb < 4; ++vb)
         gvn[vb] = ldg<int>(pverts, 4u * (uint32_t)(vbeg + min(vb * 16 + nl, vcnt - 1)));
-      cur.load(rec + kRecBytes * hdr[4 * pb + 2], fo);
+      fa.load(block_rsrc(d.rec, hdr[4 * pb + 2], d.n), fo);
     }
     for (int p = pb; p < pe; ++p) {
       const int vcnt = hdr[4 * p + 1];
@@ -350,22 +367,31 @@ k_ff_fast(const ObjDesc* __restrict__ objs,
         }
       }
       float2 tacc = make_float2(0.f, 0.f), macc = make_float2(0.f, 0.f);
-      for (int fb = 0; fb < fcnt; fb += 16) {
-        const int m = fcnt - fb;  // faces left in the patch (uniform)
-        nxt.load(rec + kRecBytes * (fbeg + (fb + 16 < fcnt ? fb + 16 : fcnt)), fo);
+      // one block of 16 faces (m of them in the patch, uniform): the three
+      // products, then the four rounds; in the patch's last block, lanes
+      // whose face lies past the patch sit out
+      auto face_block = [&](const FaceBlock& cur, int m) {
         const dbl4 A1 = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.e1, bk, z4, 0, 0, 0);
         const dbl4 A2 = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.e2, bk, z4, 0, 0, 0);
         const flt4 G = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.sg, bh, z4f, 0, 0, 0);
-        // rounds with a face in the patch (uniform); in the patch's last
-        // block, lanes whose face lies past the patch (their inputs are
-        // whatever the records there hold) sit out
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           if (4 * r < m && 4 * r + fq < m)
             tri_term(A1[r], A2[r], G[r], tw + cur.o[r][0], tw + cur.o[r][1], tw + cur.o[r][2],
                      tacc, macc);
         }
-        cur = nxt;
+      };
+      // the block after face fb's: the patch's next, or the next patch's first
+      auto after = [&](int fb) { return block_rsrc(d.rec, fbeg + min(fb + 16, fcnt), d.n); };
+      for (int fb = 0; fb < fcnt; fb += 32) {
+        fz.load(after(fb), fo);
+        face_block(fa, fcnt - fb);
+        if (fb + 16 >= fcnt) {  // odd block count: the next patch's first is in fz
+          fa = fz;
+          break;
+        }
+        fa.load(after(fb + 16), fo);
+        face_block(fz, fcnt - fb - 16);
       }
       // the terms' factor 2 (half-scale sums)
       accx += 2.0 * ((double)tacc.x + (double)macc.x);
@@ -389,6 +415,7 @@ k_ff_fast(const ObjDesc* __restrict__ objs,
     const double kpy = p[4] * kx + p[7] * ky + p[10] * kz;
     const double kpz = p[5] * kx + p[8] * ky + p[11] * kz;
     const double kp2 = kpx * kpx + kpy * kpy + kpz * kpz;
+    const double km = krep[6 * (size_t)Qh + qc];
     const double inv_km = 1.0 / km;
     const bool disk = (d.type == PII_GEOM_DISK);
     // |k'| <= kmax ||A||_F for every lane: bounds rho |k'_perp| per disk.
@@ -436,7 +463,7 @@ k_ff_fast(const ObjDesc* __restrict__ objs,
     accx += (double)px;
     accy += (double)py;
   }
-  if (km < 1e-10) { accx = 0.0; accy = 0.0; }
+  if (krep[6 * (size_t)Qh + qc] < 1e-10) { accx = 0.0; accy = 0.0; }  // |k| = 0
 
   // Sum the four waves' partial sums of the same 64 nodes.
   red[wave][lane] = make_double2(accx, accy);
