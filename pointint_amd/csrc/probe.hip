@@ -13,16 +13,21 @@ constexpr int kChains = 8;  // independent accumulators per lane
 
 // kChains independent FMA chains per lane, iters rounds: 2 * kChains * iters
 // flops per lane.  The result is written only when it is never true at run
-// time (guard), so the chains are not optimised away.
+// time (guard), so the chains are not optimised away.  The multiplier and
+// addend are moved to vector registers first: a three-operand FMA that
+// reads a scalar register issues at about 60 % of the rate on gfx950
+// (measured), which is not the form the kernels use.
 template <class T>
 __global__ void __launch_bounds__(256) k_fma_probe(int iters, T a, T b, T* __restrict__ sink,
                                                    int guard) {
   T x[kChains];
 #pragma unroll
   for (int c = 0; c < kChains; ++c) x[c] = (T)(threadIdx.x + c);
+  T av = a, bv = b;
+  asm volatile("" : "+v"(av), "+v"(bv));  // no instruction: places them in VGPRs
   for (int i = 0; i < iters; ++i) {
 #pragma unroll
-    for (int c = 0; c < kChains; ++c) x[c] = fma(x[c], a, b);
+    for (int c = 0; c < kChains; ++c) x[c] = fma(x[c], av, bv);
   }
   T s = 0;
 #pragma unroll
