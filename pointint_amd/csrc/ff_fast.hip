@@ -463,7 +463,10 @@ k_ff_fast(const ObjDesc* __restrict__ objs,
     accx += (double)px;
     accy += (double)py;
   }
-  if (krep[6 * (size_t)Qh + qc] < 1e-10) { accx = 0.0; accy = 0.0; }  // |k| = 0
+  // |k| = 0 (32-bit offsets from the grid's base: no 64-bit index stays live
+  // across the patch loop)
+  const auto* kg = reinterpret_cast<const PII_GLOBAL_AS char*>(as_global(krep));
+  if (ldg<double>(kg, 8u * (6u * (uint32_t)Qh + (uint32_t)qc)) < 1e-10) { accx = 0.0; accy = 0.0; }
 
   // Sum the four waves' partial sums of the same 64 nodes.
   red[wave][lane] = make_double2(accx, accy);
@@ -479,7 +482,7 @@ k_ff_fast(const ObjDesc* __restrict__ objs,
     if (out_mode == 0) {
       if (q < Qh) Aout[(size_t)d.col * ldA + q] = make_double2(sx, sy);
     } else {
-      const double sw = sqrt(krep[7 * (size_t)Qh + qc]);
+      const double sw = sqrt(ldg<double>(kg, 8u * (7u * (uint32_t)Qh + (uint32_t)qc)));
       const float2 x = make_float2((float)(sx * sw), (float)(sy * sw));
       if (q < ldX) Xout[(size_t)d.col * ldX + q] = (q < Qh) ? x : make_float2(0.f, 0.f);
       // column maximum of |X| (all 64 lanes take part in the shuffles)
