@@ -15,6 +15,7 @@ Prints one JSON line: completed pairs/s in the timed window as `value`,
 p50/p99 latency from the scheduled arrival, the plain FP64 GPU path and its
 per-pair error on a sample of the served scenes, the form-factor kernel's
 roofline figures, the FP64 CPU oracle on the same sample, and (extras) the
+reference's own pairwise benchmark (its only published timing) and the
 fixed 256-icosphere and 1,024-mixed-object matrices.  Other workloads:
 --workload icospheres | mixed | diff | align.
 """
@@ -467,6 +468,62 @@ def scene_geoms_dofs(geoms, sc):
     return [geoms[i] for i in sc.idx], list(sc.dofs)
 
 
+# The reference's only published timing for this path: its pairwise
+# benchmark (two Fibonacci-sphere clouds of 1,000 disks, radius 1, the second
+# shifted by d = 0, 0.1, ..., 2.0; Lebedev 29 (302 directions) x 32 radial
+# nodes, Q = 9,664; five timed runs per distance after one untimed), median
+# time per call with the clock around both geometry packings and the call,
+# 5.11 ms for its FP64 CUDA path on an unstated sm_89 GPU.
+PUBLISHED_PAIRWISE_MS = 5.11
+
+
+def measure_reference_pairwise(runs=5):
+    """The same call sequence through the reference-named Python module
+    (pointint_amd.pointint_cpp): `compute_intersection_volume_cuda` (fast
+    form factors, FP64 Gram) and, for the plain FP64 path, the same pair with
+    FP64 form factors.  Wall-clock per call, as the reference times it."""
+    import pointint_amd.pointint_cpp as pi
+    dirs, w = pi.load_lebedev(P.lebedev_path(29))
+    Ps, Ns, Rs = pi.create_sphere_pointcloud(1000)
+    kg = pi.build_kgrid(dirs, w, 32)
+    ctx = P.Context.default()
+    fast, plain, err = [], [], 0.0
+    for d in np.arange(0, 2.1, 0.1):
+        P2 = Ps + np.array([d, 0.0, 0.0])
+        vol_gt = pi.sphere_sphere_intersection_volume(np.zeros(3), 1.0, np.array([d, 0.0, 0.0]),
+                                                      1.0)
+        for timed in [False] + [True] * runs:
+            t0 = time.perf_counter()
+            g1 = pi.make_point_cloud(Ps, Ns, Rs)
+            g2 = pi.make_point_cloud(P2, Ns, Rs)
+            v = pi.compute_intersection_volume_cuda(g1, g2, kg)
+            t1 = time.perf_counter()
+            if timed:
+                fast.append(t1 - t0)
+        for timed in [False] + [True] * runs:
+            t0 = time.perf_counter()
+            g1 = pi.make_point_cloud(Ps, Ns, Rs)
+            g2 = pi.make_point_cloud(P2, Ns, Rs)
+            v64 = float(P.compute_volume_matrix([g1, g2], kg, ff="fp64", gram="fp64")[0, 1])
+            t1 = time.perf_counter()
+            if timed:
+                plain.append(t1 - t0)
+        if vol_gt > 1e-10:
+            err = max(err, abs(v - v64) / v64)
+    ctx.synchronize()
+    fast_ms = 1e3 * float(np.median(fast))
+    plain_ms = 1e3 * float(np.median(plain))
+    return {"workload": "reference pairwise benchmark: 2 x 1,000 disks (Fibonacci spheres, "
+                        "radius 1, shifts 0..2), Lebedev 29 x L=32, Q = 9,664",
+            "calls": len(fast), "ms_per_call_fast": fast_ms, "ms_per_call_fp64": plain_ms,
+            "published_ms_per_call_fp64_sm89": PUBLISHED_PAIRWISE_MS,
+            "vs_published_fast": PUBLISHED_PAIRWISE_MS / fast_ms,
+            "vs_published_fp64": PUBLISHED_PAIRWISE_MS / plain_ms,
+            "max_rel_err_fast_vs_fp64": err,
+            "note": "wall clock per call around both geometry packings and the call, as the "
+                    "reference times it; the published figure is from other hardware"}
+
+
 def run_serving(args):
     """The serving workload: a Poisson stream of scene queries against a
     device-resident pool of star-shaped meshes; one step = one segment of the
@@ -616,6 +673,7 @@ def run_serving(args):
         cpu = cpu_baseline_scenes(pool, kg, scenes, args.cpu_seconds)
     extras = {}
     if not args.no_extras:
+        extras["reference_pairwise_benchmark"] = measure_reference_pairwise()
         extras["icospheres_256_L3"] = measure_matrix(
             W.config_icospheres(n_obj=256, level=3, seed=2), "bf16x3", 10, 3, attribution=True)
         extras["mixed_1024"] = measure_matrix(W.config_mixed(), "fp16x3", 3, 1)

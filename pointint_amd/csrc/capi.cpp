@@ -558,8 +558,13 @@ void volume_matrix_fp64(Context& c, const Batch& b, const KGridDev& kg, double* 
   const int N = b.N, Q = kg.Q;
   const int ldJ = (int)align_up((size_t)Q, 64);
   c.w.J.ensure(sizeof(double2) * (size_t)ldJ * N);
+  int max_elems = 0;
+  for (const ObjDesc& d : b.h_objs) max_elems = std::max(max_elems, d.n);
+  const int fsplit = ff_fp64_split(N, Q, max_elems);
+  if (fsplit > 1) c.w.split.ensure(sizeof(double2) * ff_fp64_split_values(N, Q, fsplit));
   c.phase_begin(1);
-  launch_ff_fp64(s, b.d_objs, N, kg.full.as<double>(), Q, c.w.J.as<double2>(), ldJ);
+  launch_ff_fp64(s, b.d_objs, N, kg.full.as<double>(), Q, c.w.J.as<double2>(), ldJ, fsplit,
+                 fsplit > 1 ? c.w.split.as<double2>() : nullptr);
   c.phase_end(1);
   PII_HIP(hipGetLastError());
   const int nsplit = gram_fp64_splits(Q, N);
@@ -1095,7 +1100,10 @@ void form_factor_to(Context& c, const pii_geometry* g, int dof_kind, const doubl
     const int Q = kg->k.Q;
     const int ldJ = (int)align_up((size_t)Q, 64);
     c.w.J.ensure(sizeof(double2) * (size_t)ldJ);
-    launch_ff_fp64(c.stream, b.d_objs, 1, kg->k.full.as<double>(), Q, c.w.J.as<double2>(), ldJ);
+    const int fsplit = ff_fp64_split(1, Q, b.h_objs[0].n);
+    if (fsplit > 1) c.w.split.ensure(sizeof(double2) * ff_fp64_split_values(1, Q, fsplit));
+    launch_ff_fp64(c.stream, b.d_objs, 1, kg->k.full.as<double>(), Q, c.w.J.as<double2>(), ldJ,
+                   fsplit, fsplit > 1 ? c.w.split.as<double2>() : nullptr);
     PII_HIP(hipGetLastError());
     PII_HIP(hipMemcpyAsync(out, c.w.J.p, sizeof(double2) * Q, kind, c.stream));
     PII_HIP(hipStreamSynchronize(c.stream));

@@ -8,36 +8,6 @@
 
 namespace pii {
 
-// Element split that keeps a launch's rounds of workgroups full: base_wgs
-// workgroups of equal work on `slots` resident workgroups finish in
-// ceil(base/slots) rounds, the last one partly empty.  Returns the smallest
-// split s (at most smax) that gives at least two rounds (so that workgroups
-// finishing early are followed by others) filled to at least 90 %, else the
-// one that fills its rounds best.  A few objects on a large grid split many
-// ways; hundreds of objects not at all.
-static int split_rounds(int base_wgs, int slots, int smax) {
-  if (base_wgs <= 0 || slots <= 0 || smax <= 1) return 1;
-  int best = 1;
-  double best_eff = 0.0;
-  for (int s = 1; s <= smax; ++s) {
-    const double n = (double)base_wgs * s / slots;
-    const double eff = n / std::ceil(n);
-    if (n >= 2.0 && eff >= 0.9) return s;
-    const double score = std::min(n, 2.0) / 2.0 * eff;  // rounds below two count against
-    if (score > best_eff + 1e-12) {
-      best = s;
-      best_eff = score;
-    }
-  }
-  return best;
-}
-
-// Largest split that keeps at least min_per elements per workgroup (and at
-// most 64 splits).
-static int max_split(int max_elems, int min_per) {
-  return std::max(1, std::min(64, max_elems / std::max(1, min_per)));
-}
-
 void launch_form_factors_fast(hipStream_t s, const Batch& b, const KGridDev& kg,
                               int out_mode, double2* d_A, int ldA, float2* d_X, int ldX,
                               unsigned* d_colmax, Workspace& w, Context* prof) {

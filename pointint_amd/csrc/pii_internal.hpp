@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -196,6 +197,34 @@ int resident_workgroups(K kernel, int block, std::atomic<int>* cache) {
   return v;
 }
 
+// Element split of a launch with few workgroups: the smallest split s (at
+// most smax) that gives at least two rounds of the kernel's resident
+// workgroups (so that workgroups finishing early are followed by others)
+// filled to at least 90 %, else the one that fills its rounds best.  A few
+// objects on a large grid split many ways; hundreds of objects not at all.
+inline int split_rounds(int base_wgs, int slots, int smax) {
+  if (base_wgs <= 0 || slots <= 0 || smax <= 1) return 1;
+  int best = 1;
+  double best_eff = 0.0;
+  for (int s = 1; s <= smax; ++s) {
+    const double n = (double)base_wgs * s / slots;
+    const double eff = n / std::ceil(n);
+    if (n >= 2.0 && eff >= 0.9) return s;
+    const double score = std::min(n, 2.0) / 2.0 * eff;  // rounds below two count against
+    if (score > best_eff + 1e-12) {
+      best = s;
+      best_eff = score;
+    }
+  }
+  return best;
+}
+
+// Largest split that keeps at least min_per elements per workgroup (and at
+// most 64 splits).
+inline int max_split(int max_elems, int min_per) {
+  return std::max(1, std::min(64, max_elems / std::max(1, min_per)));
+}
+
 // Per-call device workspaces (grow-only, reused across calls on one stream).
 struct Workspace {
   DevBuf world, faces, J, X, part, V, misc, colmax, B, C, gw, mrec, gam;
@@ -300,7 +329,13 @@ void launch_stamp(hipStream_t s, unsigned long long* d_out, int i);
 // Plain FP64 form factor (reference formula) over the full grid:
 // J[col*ldJ + q] = A_col(k_q), complex FP64.
 void launch_ff_fp64(hipStream_t s, const ObjDesc* d_objs, int N, const double* d_kfull, int Q,
-                    double2* d_J, int ldJ);
+                    double2* d_J, int ldJ, int nsplit = 1, double2* d_part = nullptr);
+// Element split of the plain FP64 form factors for N objects (the largest
+// with max_elems elements) on Q nodes, and the partial-sum buffer it needs
+// (double2 values; none when the split is 1).  The partial sums are kept to
+// 8 MiB, so the split is the same wherever the workspace comes from.
+int ff_fp64_split(int N, int Q, int max_elems);
+size_t ff_fp64_split_values(int N, int Q, int nsplit);
 
 // Plain FP64 Gram over the full grid: V = Re(J^H diag(w) J)/(8 pi^3).
 int gram_fp64_splits(int Q, int N);
