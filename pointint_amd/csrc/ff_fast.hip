@@ -60,11 +60,7 @@ __device__ __forceinline__ float2 phasor_rev_hw(double r) {
   return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
 }
 
-// e^{i u/2} (half phasor) and e^{i u} (full phasor) of an FP64 phase.
-__device__ __forceinline__ float2 half_phasor(double u) {
-  const double t = u * 0.07957747154594767;  // 1/(4 pi)
-  return phasor_rev_hw(t - rint(t));
-}
+// e^{i u} (full phasor) of an FP64 phase.
 __device__ __forceinline__ float2 full_phasor_hw(double u) {
   const double t = u * 0.15915494309189535;  // 1/(2 pi)
   return phasor_rev_hw(t - rint(t));

@@ -115,24 +115,13 @@ __device__ __forceinline__ void tri_taylor_adj(double al, double be, float2 Q0, 
   G2 = make_float2(P0.x * f2.x - P0.y * f2.y, P0.x * f2.y + P0.y * f2.x);
 }
 
-// Sum over the 64 lanes of a wave with DPP moves (VALU only, no LDS):
-// quad_perm [1,0,3,2] and [2,3,0,1] give quad sums, row_half_mirror and
-// row_mirror row sums, row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3)
-// carry them into lane 63, which holds the total.  Lanes a move does not
-// write receive 0.  Every lane of the wave must be active.
+// A DPP move (VALU only, no LDS): quad_perm [1,0,3,2] (0xb1) and [2,3,0,1]
+// (0x4e) exchange within quads, row_half_mirror (0x141) and row_mirror
+// (0x140) within rows of 16 lanes.  Lanes a move does not write receive 0.
 template <int kCtrl, int kRowMask>
 __device__ __forceinline__ float dpp_move(float v) {
   return __int_as_float(
       __builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask, 0xf, false));
-}
-__device__ __forceinline__ float wave_total(float v) {
-  v += dpp_move<0xb1, 0xf>(v);
-  v += dpp_move<0x4e, 0xf>(v);
-  v += dpp_move<0x141, 0xf>(v);
-  v += dpp_move<0x140, 0xf>(v);
-  v += dpp_move<0x142, 0xa>(v);
-  v += dpp_move<0x143, 0xc>(v);
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Adds the wave totals of 12 per-lane values to acc[0..11] (FP64, LDS; one
