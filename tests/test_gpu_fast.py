@@ -199,7 +199,7 @@ def ico24(oracle):
 
 @pytest.mark.parametrize("gram", ["fp64", "bf16x3", "fp16x3"])
 def test_per_pair_relative_error(ico24, gram):
-    """north_star's volume gate per pair: |V_ij - V64_ij| / V64_ij <= 1e-5 on
+    """The volume gate per pair: |V_ij - V64_ij| / V64_ij <= 1e-5 on
     every overlapping pair (V64_ij >= 1e-3 sqrt(V64_ii V64_jj)), V64 from the
     FP64 oracle on the same grid.  The fp64 Gram on the same fast form factors
     isolates the form factors' share."""
