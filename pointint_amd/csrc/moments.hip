@@ -420,14 +420,21 @@ __device__ __forceinline__ void sh_degree_par(double (&H)[kShH], const double2* 
   if constexpr (l < sh::kL) sh_degree_par<l + 1>(H, cb, ab, cm, m, par, x, zr, zi, seed, qa, qb);
 }
 
-__global__ void __launch_bounds__(kShThreads) k_series_sh(
+// Four waves per SIMD: 31 KB of LDS per workgroup (the shells' even and odd
+// halves reuse the coefficient buffers once every order is summed) and
+// registers held to that.
+__global__ void __launch_bounds__(kShThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_series_sh(
     const ObjDesc* __restrict__ objs, const double* __restrict__ krep, int Qh, int Mh,
     int out_mode, double2* __restrict__ Aout, int ldA, float2* __restrict__ Xout, int ldX,
     unsigned* __restrict__ colmax) {
   __shared__ double2 cbuf[2][kShPairs0];
   __shared__ double2 abuf[2][sh::kL + 1];  // recurrence factors (a_lm, b_lm) of the order
-  __shared__ double eo[kShChunk][2][kShDirs];
   __shared__ double dcs[kShDirs];
+  static_assert(sizeof(double) * kShChunk * 2 * kShDirs <= sizeof(cbuf),
+                "the shells' halves fit in the coefficient buffers");
+  // after the order loop (whose last barrier ends the reads of cbuf)
+  double (*eo)[2][kShDirs] = reinterpret_cast<double (*)[2][kShDirs]>(&cbuf[0][0]);
   const BlockObj bo = xcd_block_map();
   const ObjDesc& d = objs[bo.obj];
   if (d.nser <= 0 || d.type != PII_GEOM_TRIANGLE || !d.shc) return;
